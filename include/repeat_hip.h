@@ -1,0 +1,173 @@
+/*
+ * repeat_hip.h -- C ABI of librepeat_hip.so, the gfx950 (MI355X) HIP
+ * implementation of pyREPEAT's charge-fit path:
+ *
+ *     vdW grid filter -> Ewald basis Phi (real + reciprocal) ->
+ *     normal equations -> constrained least squares (KKT)
+ *
+ * The reference has no FFI: its boundary is a set of Python calls inside
+ * repeat.py.  Each entry point below names the reference call it replaces.
+ *
+ * Conventions
+ *   - Every function returns an int status: RP_OK (0) or a negative RP_E*
+ *     code.  Nothing calls exit().  rp_last_error() gives the message of the
+ *     last failure on a context (or the last global failure when ctx is NULL).
+ *   - Host buffers are caller-owned and only read or written during the call.
+ *     Device memory is owned by the context and charged against its HBM budget.
+ *   - A context binds one device.  Calls on one context must not overlap.
+ *   - Units: Bohr, Hartree/e, e.
+ */
+#ifndef REPEAT_HIP_H
+#define REPEAT_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------- status */
+#define RP_OK 0
+#define RP_EINVAL -1     /* bad argument */
+#define RP_ENOMEM -2     /* device allocation failed */
+#define RP_EHIP -3       /* HIP runtime error */
+#define RP_ESINGULAR -4  /* KKT matrix exactly singular (LU zero pivot) */
+#define RP_EBUDGET -5    /* request exceeds the context's HBM budget */
+#define RP_ESTATE -6     /* call out of order (e.g. solve before accumulate) */
+#define RP_EEMPTY -7     /* the vdW filter kept no grid point */
+
+/* ---------------------------------------------------------------- enums */
+#define RP_F64 0
+#define RP_F32 1
+
+#define RP_ORDER_CUBE 0  /* Gaussian cube text order: x outer, z inner (repeat.py:788-796) */
+#define RP_ORDER_FLAT 1  /* flat order f = i + j*nx + k*nx*ny (repeat.py:2086) */
+
+#define RP_PREC_FP64 0   /* plain fp64: fp64 MFMA k-space GEMM, fp64 real space, fp64 Gram */
+#define RP_PREC_OZAKI 1  /* split form: int8 fixed-point slices on MFMA with exact int32
+                            accumulation for the k-space contraction and the Gram, fp32
+                            real space, fp64 across tiles and in the KKT solve */
+
+typedef struct rp_ctx rp_ctx;
+typedef struct rp_fit rp_fit;
+
+/* One charge-fit input: the content of a Gaussian cube (read_gauss_cube, repeat.py:770-804). */
+typedef struct rp_system {
+    int32_t n_grid[3];
+    double axis[9];              /* grid step vectors, row i = axis_vector[i] */
+    int32_t n_atoms;
+    const int32_t *atomic_number;/* Z per atom */
+    const double *atom_xyz;      /* n_atoms x 3 */
+    const void *esp;             /* nx*ny*nz values */
+    int32_t esp_dtype;           /* RP_F64 | RP_F32 */
+    int32_t esp_order;           /* RP_ORDER_CUBE | RP_ORDER_FLAT */
+} rp_system;
+
+/* Command-line options of repeat.py that reach the fit path (repeat.py:2655-2660, 2680). */
+typedef struct rp_options {
+    int32_t fit_type;            /* --fit-type: 1 Ewald (default), 0 bare 1/r */
+    int32_t precision;           /* RP_PREC_FP64 (--fp64) | RP_PREC_OZAKI */
+    double vdw_factor;           /* --vdw-factor, default 1.0 */
+    double vdw_max;              /* --vdw-max, default 1000.0 */
+    double r_cutoff;             /* --cutoff, default 20.0 */
+} rp_options;
+
+/* Sizes of a prepared fit. */
+typedef struct rp_fit_info {
+    int64_t n_total;             /* nx*ny*nz */
+    int64_t n_keep;              /* kept grid points M */
+    int64_t n_eff;               /* effective atoms of the filter */
+    int32_t n_atoms;             /* N */
+    int32_t n_k;                 /* reciprocal vectors K (754 for fit-type 1) */
+    int32_t n_shift;             /* real-space shifts (full list) */
+    int32_t n_shift_pruned;      /* after AABB pruning */
+    double volume;               /* signed cell volume */
+    double alpha, sqrt_alpha;
+    double v_mean;               /* mean ESP over kept points, subtracted from V */
+} rp_fit_info;
+
+/*
+ * Constraints of the bordered solve (fit_charges steps 5, repeat.py:2278-2313).
+ * The host turns the symmetry file into merge sets: reduced unknown I sums the
+ * original atoms red_src[red_ptr[I] .. red_ptr[I+1]) with multiplicities
+ * red_mult (the reference's column-then-row merges, repeat.py:2445-2470), and
+ * atom a takes the value of reduced unknown expand_src[a] (-1 -> 0.0) after the
+ * solve (repeat.py:2472-2489).  Without symmetry: red_* is the identity.
+ */
+typedef struct rp_constraints {
+    double q_tot;                /* --total-charge */
+    int32_t n_reduced;           /* N' */
+    const int32_t *red_ptr;      /* N'+1 */
+    const int32_t *red_src;
+    const double *red_mult;
+    const double *weights;       /* N': Lagrange row (group sizes) */
+    const double *diag_add;      /* N' or NULL: RESP strength / QEq hardness on A[i,i] */
+    const double *rhs_add;       /* N' or NULL: RESP strength*target / QEq chi on b[i] */
+    const int32_t *expand_src;   /* N */
+} rp_constraints;
+
+/* ---------------------------------------------------------------- context */
+int rp_ctx_create(int device, uint64_t hbm_budget_bytes, rp_ctx **out);
+int rp_ctx_destroy(rp_ctx *ctx);
+const char *rp_last_error(const rp_ctx *ctx);
+int rp_ctx_hbm_in_use(const rp_ctx *ctx, uint64_t *bytes);
+
+/* ---------------------------------------------------------------- fit path */
+/*
+ * ChargeCalculator.__init__ (repeat.py:1736-1786): upload the cube, build the
+ * 27-image effective atoms and the vdW keep mask on the device
+ * (_filter_grid, repeat.py:1922-2113), compact the kept points in ascending
+ * flat index with V - mean(V), set up the Ewald sums (EwaldCalculator.__init__,
+ * _setup_kspace, repeat.py:1077-1143) and prune the real-space shifts
+ * (prepare_pruned_shifts, repeat.py:1708-1731).
+ */
+int rp_fit_create(rp_ctx *ctx, const rp_system *sys, const rp_options *opt, rp_fit **out);
+int rp_fit_destroy(rp_fit *fit);
+int rp_fit_info_get(const rp_fit *fit, rp_fit_info *out);
+
+/* Filter results: any pointer may be NULL.  mask: n_total bytes (flat order),
+ * flat_idx / v / grid_xyz (M x 3): the kept points as _filter_grid returns them. */
+int rp_fit_grid(const rp_fit *fit, uint8_t *mask, int64_t *flat_idx, double *v, double *grid_xyz);
+
+/* Ewald setup: kvecs (K x 3), kcoefs (K), shifts (S x 3), shifts_pruned (S' x 3); NULL skips. */
+int rp_fit_ewald(const rp_fit *fit, double *kvecs, double *kcoefs, double *shifts, double *shifts_pruned);
+
+/*
+ * fit_charges steps 1-3 (repeat.py:2195-2262): stream Phi tiles over the kept
+ * points and accumulate Phi^T Phi, Phi^T V, sum(Phi), sum(V) on the device,
+ * in fp64 across tiles, without materialising Phi.
+ */
+int rp_fit_accumulate(rp_fit *fit);
+
+/* Accumulators of the last rp_fit_accumulate: ata (N x N), atb (N), sum_phi (N), sum_v (1). */
+int rp_fit_normal_equations(const rp_fit *fit, double *ata, double *atb, double *sum_phi, double *sum_v);
+
+/*
+ * fit_charges steps 4-5 (repeat.py:2266-2313) on the device: centring, symmetry
+ * merge, RESP/QEq diagonal terms, bordered [[A, w], [w^T, 0]] LU solve with
+ * partial pivoting, expansion to all atoms.  q_out: N charges; solution_out
+ * (N'+1, nullable): reduced charges and the Lagrange multiplier.
+ * Returns RP_ESINGULAR if an LU pivot is exactly zero (the reference then
+ * falls back to lstsq); *used_fallback is set to 0 otherwise.
+ */
+int rp_fit_solve(rp_fit *fit, const rp_constraints *con, double *q_out, double *solution_out,
+                 int32_t *used_fallback);
+
+/* compute_stats (repeat.py:2326-2441): v_coul[m] = (Phi q)[m] for every kept point, streamed. */
+int rp_fit_phi_matvec(rp_fit *fit, const double *q, double *v_coul);
+
+/*
+ * EwaldCalculator.compute_batch_parallel (repeat.py:1145-1197): the basis
+ * phi (m x N, row-major, fp64) of the fit's atoms at arbitrary points, with
+ * the fit's Ewald setup and pruned shifts.
+ */
+int rp_fit_phi(rp_fit *fit, int64_t m, const double *grid_xyz, double *phi_out);
+
+/* Device time (ms) of the phases of the last accumulate / solve on this fit:
+ * t[0] filter, t[1] accumulate, t[2] solve. */
+int rp_fit_timings(const rp_fit *fit, double *t3);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* REPEAT_HIP_H */

@@ -1,0 +1,596 @@
+"""CPU oracle for the REPEAT fit path (filter -> Ewald basis -> constrained LSQ).
+
+TEST INFRASTRUCTURE ONLY.  This module is the checker for the HIP product in
+``pyrepeat_amd``.  Only ``tests/``, ``__graft_entry__.smoke()`` and the CPU
+baseline leg of ``bench.py`` may import it; the product never does.
+
+It restates the reference's CPU algorithm in fp64 NumPy (plus the C kernels in
+``oracle_kernels.c`` for the O(M*N*K) loops), quirks included.  Each function
+names the reference lines it follows (file ``repeat.py`` unless stated).
+
+Quirks reproduced on purpose (parity mode):
+- the filter's clipping box is the AABB of the first grid *voxel* (corners of
+  the step vectors, ``repeat.py:1941-1943``), widened by ``max(r_max) + 0.1``;
+- the k-set is ``kx in [0,7]``, ``ky,kz in [-7,7]``, ``0 < kx^2+ky^2+kz^2 < 49``
+  with no factor 2 on the ``kx > 0`` half space (``repeat.py:1126-1143``);
+- the cell volume is signed (``repeat.py:1763``);
+- no Ewald self term is subtracted (it would cancel in the centring);
+- under symmetry, the stats' average Coulomb potential uses the *unreduced*
+  ``sum_phi`` scattered as if reduced (``repeat.py:2319-2320, 2491-2509``);
+- a RESP/QEq parameter line with >= 3 tokens that fails ``float()`` drops the
+  whole file (``repeat.py:1892-1918``).
+"""
+from __future__ import annotations
+
+import ctypes
+import itertools
+import os
+import subprocess
+from dataclasses import dataclass, field
+
+import numpy as np
+import scipy.linalg
+
+from pyrepeat_amd.tables import ATOM_SYMBOLS, DEFAULT_QEQ_PARAMS, vdw_radius_lookup, element_symbol
+
+PI = np.pi
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liboracle.so")
+_lib = None
+
+
+# --------------------------------------------------------------------------- C kernels
+
+def build(force: bool = False) -> str:
+    """Compile ``oracle_kernels.c`` into ``oracle/liboracle.so`` (gcc, OpenMP, strict fp64)."""
+    src = os.path.join(_HERE, "oracle_kernels.c")
+    if force or not os.path.exists(_LIB_PATH) or os.path.getmtime(_LIB_PATH) < os.path.getmtime(src):
+        subprocess.check_call(["make", "-s", "-C", _HERE, "liboracle.so"])
+    return _LIB_PATH
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = ctypes.CDLL(_LIB_PATH)
+        dp = ctypes.POINTER(ctypes.c_double)
+        u8p = ctypes.POINTER(ctypes.c_uint8)
+        i64 = ctypes.c_int64
+        L.oracle_filter_mask.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, dp, i64, dp, dp, dp, u8p,
+                                         ctypes.c_int]
+        L.oracle_filter_mask.restype = None
+        L.oracle_cdist.argtypes = [i64, dp, i64, dp, dp]
+        L.oracle_grid_point.argtypes = [dp, i64, i64, i64, dp]
+        L.oracle_phi.argtypes = [i64, dp, i64, dp, i64, dp, dp, i64, dp, ctypes.c_double, ctypes.c_double,
+                                 ctypes.c_int, ctypes.c_int, dp, ctypes.c_int]
+        L.oracle_phi.restype = None
+        L.oracle_phi_matvec.argtypes = [i64, dp, i64, dp, i64, dp, dp, i64, dp, ctypes.c_double,
+                                        ctypes.c_double, ctypes.c_int, ctypes.c_int, dp, dp, ctypes.c_int]
+        _lib = L
+    return _lib
+
+
+def _dp(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def _threads(n):
+    if n:
+        return int(n)
+    return len(os.sched_getaffinity(0))
+
+
+# --------------------------------------------------------------------------- geometry
+
+def box_and_volume(n_grid, axis):
+    """Cell vectors ``box[i] = n_i * axis[i]`` and the *signed* volume (``repeat.py:1759-1763``)."""
+    axis = np.asarray(axis, dtype=np.float64)
+    box = np.array([n_grid[i] * axis[i] for i in range(3)], dtype=np.float64)
+    volume = float(np.dot(box[0], np.cross(box[1], box[2])))
+    return box, volume
+
+
+def neighbor_shifts_27(box):
+    """27 image shifts, loop order kz, ky, kx (``repeat.py:807-813``)."""
+    out = []
+    for kz in (-1, 0, 1):
+        for ky in (-1, 0, 1):
+            for kx in (-1, 0, 1):
+                out.append(kx * box[0] + ky * box[1] + kz * box[2])
+    return np.asarray(out, dtype=np.float64)
+
+
+def effective_atoms(atom_pos, atomic_numbers, axis, box, vdw_factor, vdw_max):
+    """Effective atoms of the filter (``repeat.py:1934-1964``).
+
+    Returns positions (E,3), r_min (E,), r_max (E,) and max(r_max).  The order
+    is atom-major, image-minor; only the set matters to the mask.
+    """
+    raw = vdw_radius_lookup(atomic_numbers)
+    rmin = vdw_factor * raw
+    rmax = vdw_max * raw
+    atom_pos = np.asarray(atom_pos, dtype=np.float64)
+    shifts = neighbor_shifts_27(box)
+    corners = np.dot(np.array(list(itertools.product([0, 1], repeat=3)), dtype=np.float64), np.asarray(axis))
+    box_min, box_max = corners.min(axis=0), corners.max(axis=0)
+    global_max_r = float(np.max(rmax)) if len(rmax) > 0 else 5.0
+    margin = global_max_r + 0.1
+    lo, hi = box_min - margin, box_max + margin
+    allp = (atom_pos[:, None, :] + shifts[None, :, :]).reshape(-1, 3)
+    armin = np.repeat(rmin, len(shifts))
+    armax = np.repeat(rmax, len(shifts))
+    inb = ((allp[:, 0] >= lo[0]) & (allp[:, 0] <= hi[0]) & (allp[:, 1] >= lo[1]) & (allp[:, 1] <= hi[1])
+           & (allp[:, 2] >= lo[2]) & (allp[:, 2] <= hi[2]))
+    return allp[inb], armin[inb], armax[inb], global_max_r
+
+
+def filter_mask(n_grid, axis, eff_pos, eff_rmin, eff_rmax, nthreads=None):
+    """Keep mask (uint8, flat order ``i + j*nx + k*nx*ny``) by brute force (``repeat.py:2115-2180``)."""
+    nx, ny, nz = (int(v) for v in n_grid)
+    axis = np.ascontiguousarray(axis, dtype=np.float64)
+    eff_pos = np.ascontiguousarray(eff_pos, dtype=np.float64)
+    rmin = np.ascontiguousarray(eff_rmin, dtype=np.float64)
+    rmax = np.ascontiguousarray(eff_rmax, dtype=np.float64)
+    keep = np.zeros(nx * ny * nz, dtype=np.uint8)
+    lib().oracle_filter_mask(nx, ny, nz, _dp(axis), len(eff_pos), _dp(eff_pos), _dp(rmin), _dp(rmax),
+                             keep.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)), _threads(nthreads))
+    return keep
+
+
+def filter_mask_cdist(n_grid, axis, eff_pos, eff_rmin, eff_rmax):
+    """Same mask through ``scipy.spatial.distance.cdist`` exactly as the reference calls it.
+
+    Pure NumPy/SciPy, for small cubes only (``repeat.py:2139-2178`` with one chunk).
+    """
+    from scipy.spatial.distance import cdist
+    nx, ny, nz = (int(v) for v in n_grid)
+    axis = np.asarray(axis, dtype=np.float64)
+    j_grid, i_grid = np.meshgrid(np.arange(ny, dtype=np.int32), np.arange(nx, dtype=np.int32), indexing="ij")
+    ig, jg = i_grid.ravel(), j_grid.ravel()
+    base_xy = np.zeros((nx * ny, 3), dtype=np.float64)
+    for c in range(3):
+        base_xy[:, c] = ig * axis[0][c] + jg * axis[1][c]
+    idx = np.arange(nx * ny * nz)
+    k = idx // (nx * ny)
+    xy = idx % (nx * ny)
+    coords = base_xy[xy] + (k[:, None] * axis[2])
+    if len(eff_pos) == 0:
+        return np.zeros(nx * ny * nz, dtype=np.uint8)
+    d = cdist(coords, eff_pos)
+    valid = np.all(d > eff_rmin, axis=1)
+    near = np.any(d <= eff_rmax, axis=1)
+    return (valid & near).astype(np.uint8)
+
+
+def compact(keep, esp_flat, n_grid, axis):
+    """Kept positions in ascending flat index and mean-subtracted ESP (``repeat.py:2081-2113``).
+
+    ``esp_flat`` is the ESP in flat order ``i + j*nx + k*nx*ny``.
+    """
+    nx, ny, nz = (int(v) for v in n_grid)
+    axis = np.asarray(axis, dtype=np.float64)
+    flat = np.flatnonzero(keep)
+    v = np.asarray(esp_flat)[flat].astype(np.float64, copy=False)
+    layer = nx * ny
+    k = flat // layer
+    rem = flat - k * layer
+    j = rem // nx
+    i = rem - j * nx
+    pos = np.empty((len(flat), 3), dtype=np.float64)
+    pos[:] = axis[0]
+    pos *= i[:, None]
+    pos += axis[1] * j[:, None]
+    pos += axis[2] * k[:, None]
+    return pos, v - np.mean(v), flat
+
+
+def esp_cube_to_flat(esp_cube_order, n_grid):
+    """Cube file order (x outer, z inner) -> flat order ``i + j*nx + k*nx*ny`` (``repeat.py:788-796, 2086``)."""
+    nx, ny, nz = (int(v) for v in n_grid)
+    return np.asarray(esp_cube_order).reshape(nx, ny, nz).transpose(2, 1, 0).ravel()
+
+
+# --------------------------------------------------------------------------- Ewald setup
+
+@dataclass
+class Ewald:
+    box: np.ndarray
+    volume: float
+    r_cutoff: float
+    fit_flag: int
+    alpha: float = 0.0
+    sqrt_alpha: float = 0.0
+    recip: np.ndarray | None = None
+    nmax: np.ndarray | None = None
+    kvecs: np.ndarray = field(default_factory=lambda: np.empty((0, 3)))
+    kcoefs: np.ndarray = field(default_factory=lambda: np.empty((0,)))
+    kidx: np.ndarray = field(default_factory=lambda: np.empty((0, 3), dtype=np.int64))
+    shifts: np.ndarray | None = None
+    shifts_pruned: np.ndarray | None = None
+
+
+def ewald_setup(box, volume, r_cutoff=20.0, fit_flag=1):
+    """``EwaldCalculator.__init__`` + ``_setup_kspace`` (``repeat.py:1077-1143``)."""
+    box = np.asarray(box, dtype=np.float64)
+    e = Ewald(box=box, volume=float(volume), r_cutoff=float(r_cutoff), fit_flag=int(fit_flag))
+    e.alpha = (PI / e.r_cutoff) ** 2
+    e.sqrt_alpha = float(np.sqrt(e.alpha))
+    if fit_flag == 1:
+        a, b, c = box
+        e.recip = np.array([2 * PI * np.cross(b, c) / e.volume,
+                            2 * PI * np.cross(c, a) / e.volume,
+                            2 * PI * np.cross(a, b) / e.volume], dtype=np.float64)
+        e.nmax = np.array([int(np.floor(e.r_cutoff / np.linalg.norm(box[i]))) + 1 for i in range(3)],
+                          dtype=np.int64)
+        beta = 1.0 / (4.0 * e.alpha)
+        kv, kc, ki = [], [], []
+        for kx in range(8):
+            for ky in range(-7, 8):
+                for kz in range(-7, 8):
+                    ksq = kx * kx + ky * ky + kz * kz
+                    if ksq >= 49 or ksq == 0:
+                        continue
+                    kvec = kx * e.recip[0] + ky * e.recip[1] + kz * e.recip[2]
+                    k2 = np.dot(kvec, kvec)
+                    kv.append(kvec)
+                    kc.append((4.0 * PI / e.volume) * np.exp(-beta * k2) / k2)
+                    ki.append((kx, ky, kz))
+        e.kvecs = np.asarray(kv, dtype=np.float64)
+        e.kcoefs = np.asarray(kc, dtype=np.float64)
+        e.kidx = np.asarray(ki, dtype=np.int64)
+        e.shifts = realspace_shifts(e.nmax, box)
+    else:
+        e.nmax = np.zeros(3, dtype=np.int64)
+        e.shifts = np.zeros((1, 3), dtype=np.float64)
+        e.shifts_pruned = e.shifts
+    return e
+
+
+def realspace_shifts(nmax, box):
+    """Shift lattice, loop order ix, iy, iz (``repeat.py:815-822``)."""
+    out = []
+    for ix in range(-int(nmax[0]), int(nmax[0]) + 1):
+        for iy in range(-int(nmax[1]), int(nmax[1]) + 1):
+            for iz in range(-int(nmax[2]), int(nmax[2]) + 1):
+                out.append(ix * box[0] + iy * box[1] + iz * box[2])
+    return np.asarray(out, dtype=np.float64)
+
+
+def _aabb_distance_sq(min1, max1, min2, max2):
+    d2 = 0.0
+    for a0, a1, b0, b1 in zip(min1, max1, min2, max2):
+        if a1 < b0:
+            d2 += (b0 - a1) ** 2
+        elif b1 < a0:
+            d2 += (a0 - b1) ** 2
+    return d2
+
+
+def prune_shifts(shifts, grid_pos, box, atom_pos, r_cutoff):
+    """AABB pruning, order preserving (``repeat.py:1031-1069``)."""
+    shifts = np.asarray(shifts, dtype=np.float64)
+    if grid_pos is not None and len(grid_pos) > 0:
+        gmin, gmax = grid_pos.min(axis=0), grid_pos.max(axis=0)
+    else:
+        corners = np.array(list(itertools.product([0.0, 1.0], repeat=3))) @ np.asarray(box)
+        gmin, gmax = corners.min(axis=0), corners.max(axis=0)
+    if atom_pos is None or len(atom_pos) == 0:
+        return shifts
+    amin, amax = atom_pos.min(axis=0), atom_pos.max(axis=0)
+    r2 = float(r_cutoff) ** 2
+    kept = [sh for sh in shifts if _aabb_distance_sq(gmin, gmax, amin + sh, amax + sh) <= r2]
+    if not kept:
+        kept = [np.zeros(3)]
+    return np.asarray(kept, dtype=np.float64)
+
+
+# --------------------------------------------------------------------------- basis
+
+def phi_matrix(grid_pos, atom_pos, ew: Ewald, shifts=None, block_k=64, nthreads=None):
+    """Basis Phi (M x N) as ``compute_batch_parallel`` (``repeat.py:1145``, per-atom columns)."""
+    grid_pos = np.ascontiguousarray(grid_pos, dtype=np.float64)
+    atom_pos = np.ascontiguousarray(atom_pos, dtype=np.float64)
+    if shifts is None:
+        shifts = ew.shifts
+    shifts = np.ascontiguousarray(shifts, dtype=np.float64)
+    kv = np.ascontiguousarray(ew.kvecs, dtype=np.float64).reshape(-1, 3)
+    kc = np.ascontiguousarray(ew.kcoefs, dtype=np.float64)
+    M, N = len(grid_pos), len(atom_pos)
+    out = np.zeros((M, N), dtype=np.float64)
+    if M == 0 or N == 0:
+        return out
+    lib().oracle_phi(M, _dp(grid_pos), N, _dp(atom_pos), len(kv), _dp(kv), _dp(kc), len(shifts), _dp(shifts),
+                     ew.r_cutoff, ew.sqrt_alpha, ew.fit_flag, int(block_k), _dp(out), _threads(nthreads))
+    return out
+
+
+def phi_matvec(grid_pos, atom_pos, ew: Ewald, q, shifts=None, block_k=64, nthreads=None):
+    grid_pos = np.ascontiguousarray(grid_pos, dtype=np.float64)
+    atom_pos = np.ascontiguousarray(atom_pos, dtype=np.float64)
+    shifts = np.ascontiguousarray(ew.shifts if shifts is None else shifts, dtype=np.float64)
+    kv = np.ascontiguousarray(ew.kvecs, dtype=np.float64).reshape(-1, 3)
+    kc = np.ascontiguousarray(ew.kcoefs, dtype=np.float64)
+    q = np.ascontiguousarray(q, dtype=np.float64)
+    out = np.zeros(len(grid_pos), dtype=np.float64)
+    lib().oracle_phi_matvec(len(grid_pos), _dp(grid_pos), len(atom_pos), _dp(atom_pos), len(kv), _dp(kv),
+                            _dp(kc), len(shifts), _dp(shifts), ew.r_cutoff, ew.sqrt_alpha, ew.fit_flag,
+                            int(block_k), _dp(q), _dp(out), _threads(nthreads))
+    return out
+
+
+# --------------------------------------------------------------------------- normal equations
+
+def accumulate(grid_pos, V, atom_pos, ew: Ewald, chunk=200000, nthreads=None, keep_phi=False):
+    """Streamed ``ATA, ATb, sum_phi, sum_V`` (``repeat.py:2235-2250``) with NumPy BLAS."""
+    N = len(atom_pos)
+    ATA = np.zeros((N, N))
+    ATb = np.zeros(N)
+    sum_phi = np.zeros(N)
+    sum_V = 0.0
+    phis = []
+    for s in range(0, len(grid_pos), chunk):
+        e = min(s + chunk, len(grid_pos))
+        ph = phi_matrix(grid_pos[s:e], atom_pos, ew, nthreads=nthreads)
+        ATA += ph.T @ ph
+        ATb += ph.T @ V[s:e]
+        sum_phi += np.sum(ph, axis=0)
+        sum_V += np.sum(V[s:e])
+        if keep_phi:
+            phis.append(ph)
+    return ATA, ATb, sum_phi, sum_V, (np.concatenate(phis) if keep_phi and phis else None)
+
+
+def centre(ATA, ATb, sum_phi, sum_V, n_grid_total):
+    """REPEAT offset: ``A = ATA - N phibar phibar^T``, ``b = ATb - N phibar Vbar`` (``repeat.py:2266-2276``)."""
+    phi_bar = sum_phi / n_grid_total
+    v_bar = sum_V / n_grid_total
+    A = ATA - n_grid_total * np.outer(phi_bar, phi_bar)
+    b = ATb - n_grid_total * phi_bar * v_bar
+    return A, b
+
+
+# --------------------------------------------------------------------------- parameter files
+
+def parse_atom_ranges(range_str):
+    """'1-5,9' -> 0-based indices, first occurrence kept (``repeat.py:1814-1841``)."""
+    idx = []
+    for part in range_str.split(","):
+        part = part.strip()
+        if not part:
+            continue
+        if "-" in part:
+            a, b = part.split("-", 1)
+            idx.extend(range(int(a.strip()), int(b.strip()) + 1))
+        else:
+            idx.append(int(part))
+    seen, out = set(), []
+    for i in idx:
+        if i - 1 not in seen:
+            seen.add(i - 1)
+            out.append(i - 1)
+    return out
+
+
+def read_symmetry(path):
+    """Symmetry groups (``repeat.py:1843-1890``): one group per line, first atom is the base."""
+    try:
+        with open(path) as f:
+            lines = [ln.strip() for ln in f if ln.strip() and not ln.startswith("#")]
+        trees = []
+        for line in lines:
+            if "#" in line:
+                line = line.split("#")[0].strip()
+            if not line:
+                continue
+            ids = parse_atom_ranges(line)
+            if len(ids) < 1:
+                continue
+            trees.append({"n_equiv": len(ids), "base": ids[0], "linked": ids[1:]})
+        return {"trees": trees, "n_independent": len(trees)}
+    except Exception:
+        return None
+
+
+def _read_param_file(path, key1, key2):
+    out = {}
+    try:
+        with open(path) as f:
+            for line in f:
+                parts = line.split()
+                if len(parts) >= 3:
+                    out[parts[0]] = {key1: float(parts[1]), key2: float(parts[2])}
+        return out
+    except Exception:
+        return {}
+
+
+def read_resp_params(path):
+    """``repeat.py:1892-1904`` (any unparsable 3-token line empties the dict)."""
+    return _read_param_file(path, "strength", "target")
+
+
+def read_qeq_params(path):
+    """``repeat.py:1906-1918``."""
+    return _read_param_file(path, "electroneg", "hardness")
+
+
+# --------------------------------------------------------------------------- constraints
+
+def _independent(trees, n_atoms):
+    linked = set()
+    for t in trees:
+        linked.update(t["linked"])
+    return [i for i in range(n_atoms) if i not in linked]
+
+
+def apply_symmetry(A, b, weights, atom_types, trees, n_atoms):
+    """Column merge for all trees, then rows, then drop linked atoms (``repeat.py:2445-2470``)."""
+    A = A.copy()
+    b = b.copy()
+    weights = weights.copy()
+    for t in trees:
+        base = t["base"]
+        for ln in t["linked"]:
+            b[base] += b[ln]
+            weights[base] += 1.0
+            A[:, base] += A[:, ln]
+    for t in trees:
+        base = t["base"]
+        for ln in t["linked"]:
+            A[base, :] += A[ln, :]
+    ind = _independent(trees, n_atoms)
+    return A[np.ix_(ind, ind)], b[ind], weights[ind], np.asarray(atom_types)[ind]
+
+
+def expand_charges(q_red, trees, n_atoms):
+    """``repeat.py:2472-2489``."""
+    ind = _independent(trees, n_atoms)
+    q = np.zeros(n_atoms)
+    for i, idx in enumerate(ind):
+        q[idx] = q_red[i]
+    for t in trees:
+        for ln in t["linked"]:
+            q[ln] = q[t["base"]]
+    return q
+
+
+def expand_sum_phi(sum_phi_given, trees, n_atoms):
+    """``repeat.py:2491-2509``; called with the *unreduced* sum_phi by ``fit_charges`` (quirk)."""
+    ind = _independent(trees, n_atoms)
+    out = np.zeros(n_atoms)
+    for i, idx in enumerate(ind):
+        out[idx] = sum_phi_given[i]
+    for t in trees:
+        for ln in t["linked"]:
+            out[ln] = out[t["base"]]
+    return out
+
+
+def apply_resp(A, b, atom_types, resp):
+    """``repeat.py:2511-2520``."""
+    A = A.copy()
+    b = b.copy()
+    if not resp:
+        return A, b
+    for i, z in enumerate(atom_types):
+        sym = element_symbol(z)
+        if sym in resp:
+            A[i, i] += resp[sym]["strength"]
+            b[i] += resp[sym]["strength"] * resp[sym]["target"]
+    return A, b
+
+
+def apply_qeq(A, b, atom_types, qeq):
+    """``repeat.py:2522-2535``."""
+    A = A.copy()
+    b = b.copy()
+    for i, z in enumerate(atom_types):
+        sym = element_symbol(z)
+        if qeq and sym in qeq:
+            chi, hard = qeq[sym]["electroneg"], qeq[sym]["hardness"]
+        elif sym in DEFAULT_QEQ_PARAMS:
+            chi, hard = DEFAULT_QEQ_PARAMS[sym]
+        else:
+            continue
+        b[i] += chi
+        A[i, i] += hard
+    return A, b
+
+
+def solve_kkt(A, b, weights, q_tot):
+    """Bordered system ``[[A, w], [w^T, 0]]`` solved by LAPACK LU, lstsq on LinAlgError (``repeat.py:2293-2310``)."""
+    n_atoms = len(b)
+    n = n_atoms + 1
+    K = np.zeros((n, n))
+    K[:n_atoms, :n_atoms] = A
+    K[:n_atoms, n_atoms] = weights
+    K[n_atoms, :n_atoms] = weights
+    rhs = np.zeros(n)
+    rhs[:n_atoms] = b
+    rhs[n_atoms] = q_tot
+    try:
+        sol = scipy.linalg.solve(K, rhs)
+        fallback = False
+    except np.linalg.LinAlgError:
+        sol = np.linalg.lstsq(K, rhs, rcond=None)[0]
+        fallback = True
+    return sol, fallback
+
+
+# --------------------------------------------------------------------------- end to end
+
+@dataclass
+class OracleResult:
+    charges: np.ndarray
+    mask: np.ndarray
+    grid_pos: np.ndarray
+    V: np.ndarray
+    ATA: np.ndarray
+    ATb: np.ndarray
+    sum_phi: np.ndarray
+    sum_V: float
+    A: np.ndarray
+    b: np.ndarray
+    kkt_solution: np.ndarray
+    used_fallback: bool
+    ewald: Ewald
+    stats_sum_phi: np.ndarray
+    rms: float | None = None
+    rms_corrected: float | None = None
+    v_coul: np.ndarray | None = None
+
+
+def fit(n_grid, axis, atomic_numbers, atom_pos, esp_flat, *, fit_flag=1, vdw_factor=1.0, vdw_max=1000.0,
+        r_cutoff=20.0, q_tot=0.0, symm=None, resp=None, qeq=None, stats=False, nthreads=None,
+        chunk=200000):
+    """The whole fit path: ``ChargeCalculator.__init__`` + ``fit_charges`` (+ ``compute_stats``).
+
+    ``symm`` is the dict of :func:`read_symmetry`, ``resp``/``qeq`` the dicts of the parameter readers.
+    """
+    axis = np.asarray(axis, dtype=np.float64)
+    atom_pos = np.asarray(atom_pos, dtype=np.float64)
+    atomic_numbers = np.asarray(atomic_numbers, dtype=np.int64)
+    n_atoms = len(atom_pos)
+    box, volume = box_and_volume(n_grid, axis)
+    ew = ewald_setup(box, volume, r_cutoff, fit_flag)
+    eff, rmin, rmax, _ = effective_atoms(atom_pos, atomic_numbers, axis, box, vdw_factor, vdw_max)
+    keep = filter_mask(n_grid, axis, eff, rmin, rmax, nthreads)
+    grid_pos, V, _ = compact(keep, esp_flat, n_grid, axis)
+    if fit_flag == 1:
+        ew.shifts_pruned = prune_shifts(ew.shifts, grid_pos, box, atom_pos, r_cutoff)
+    Mg = len(grid_pos)
+    ATA, ATb, sum_phi, sum_V, phi_full = accumulate(grid_pos, V, atom_pos, ew, chunk=chunk, nthreads=nthreads,
+                                                    keep_phi=stats)
+    A, b = centre(ATA, ATb, sum_phi, sum_V, Mg)
+    weights = np.ones(n_atoms)
+    types = atomic_numbers.copy()
+    trees = symm["trees"] if symm else None
+    if trees:
+        A, b, weights, types = apply_symmetry(A, b, weights, types, trees, n_atoms)
+    if resp:
+        A, b = apply_resp(A, b, types, resp)
+    elif qeq:
+        A, b = apply_qeq(A, b, types, qeq)
+    sol, fb = solve_kkt(A, b, weights, q_tot)
+    nr = len(b)
+    q = expand_charges(sol[:nr], trees, n_atoms) if trees else sol[:nr]
+    s_phi = expand_sum_phi(sum_phi, trees, n_atoms) if trees else sum_phi
+    res = OracleResult(charges=q, mask=keep, grid_pos=grid_pos, V=V, ATA=ATA, ATb=ATb, sum_phi=sum_phi,
+                       sum_V=sum_V, A=A, b=b, kkt_solution=sol, used_fallback=fb, ewald=ew, stats_sum_phi=s_phi)
+    if stats:
+        v_coul = phi_full @ q
+        res.v_coul = v_coul
+        res.rms = rms_error(v_coul, V, s_phi, q)
+        res.rms_corrected = rms_error(v_coul, V, sum_phi, q)
+    return res
+
+
+def rms_error(v_coul, V, sum_phi_for_mean, q):
+    """``compute_stats`` (``repeat.py:2340-2422``): mean of Coulomb from ``sum_phi . q / N``."""
+    n = len(V)
+    avrg_coul = float(sum_phi_for_mean @ q) / n
+    avrg_qm = np.mean(V)
+    diff = (v_coul - avrg_coul) - (V - avrg_qm)
+    return float(np.sqrt(np.sum(diff ** 2) / n))

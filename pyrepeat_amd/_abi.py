@@ -1,0 +1,279 @@
+"""ctypes binding of ``librepeat_hip.so`` (declarations in ``include/repeat_hip.h``).
+
+The library is the only compute path: if it is missing or cannot be loaded,
+every entry point raises ``RepeatHipError``; there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "librepeat_hip.so")
+
+RP_OK = 0
+RP_EINVAL = -1
+RP_ENOMEM = -2
+RP_EHIP = -3
+RP_ESINGULAR = -4
+RP_EBUDGET = -5
+RP_ESTATE = -6
+RP_EEMPTY = -7
+
+RP_F64, RP_F32 = 0, 1
+RP_ORDER_CUBE, RP_ORDER_FLAT = 0, 1
+RP_PREC_FP64, RP_PREC_OZAKI = 0, 1
+
+EXPORTED = (
+    "rp_ctx_create", "rp_ctx_destroy", "rp_last_error", "rp_ctx_hbm_in_use",
+    "rp_fit_create", "rp_fit_destroy", "rp_fit_info_get", "rp_fit_grid", "rp_fit_ewald",
+    "rp_fit_accumulate", "rp_fit_normal_equations", "rp_fit_solve", "rp_fit_phi_matvec",
+    "rp_fit_phi", "rp_fit_timings",
+)
+
+
+class RepeatHipError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"librepeat_hip error {code}: {msg}")
+        self.code = code
+
+
+class SingularKKTError(RepeatHipError):
+    pass
+
+
+class RpSystem(ctypes.Structure):
+    _fields_ = [("n_grid", ctypes.c_int32 * 3), ("axis", ctypes.c_double * 9), ("n_atoms", ctypes.c_int32),
+                ("atomic_number", ctypes.POINTER(ctypes.c_int32)), ("atom_xyz", ctypes.POINTER(ctypes.c_double)),
+                ("esp", ctypes.c_void_p), ("esp_dtype", ctypes.c_int32), ("esp_order", ctypes.c_int32)]
+
+
+class RpOptions(ctypes.Structure):
+    _fields_ = [("fit_type", ctypes.c_int32), ("precision", ctypes.c_int32), ("vdw_factor", ctypes.c_double),
+                ("vdw_max", ctypes.c_double), ("r_cutoff", ctypes.c_double)]
+
+
+class RpFitInfo(ctypes.Structure):
+    _fields_ = [("n_total", ctypes.c_int64), ("n_keep", ctypes.c_int64), ("n_eff", ctypes.c_int64),
+                ("n_atoms", ctypes.c_int32), ("n_k", ctypes.c_int32), ("n_shift", ctypes.c_int32),
+                ("n_shift_pruned", ctypes.c_int32), ("volume", ctypes.c_double), ("alpha", ctypes.c_double),
+                ("sqrt_alpha", ctypes.c_double), ("v_mean", ctypes.c_double)]
+
+
+class RpConstraints(ctypes.Structure):
+    _fields_ = [("q_tot", ctypes.c_double), ("n_reduced", ctypes.c_int32),
+                ("red_ptr", ctypes.POINTER(ctypes.c_int32)), ("red_src", ctypes.POINTER(ctypes.c_int32)),
+                ("red_mult", ctypes.POINTER(ctypes.c_double)), ("weights", ctypes.POINTER(ctypes.c_double)),
+                ("diag_add", ctypes.POINTER(ctypes.c_double)), ("rhs_add", ctypes.POINTER(ctypes.c_double)),
+                ("expand_src", ctypes.POINTER(ctypes.c_int32))]
+
+
+_lib = None
+
+
+def load(path: str | None = None):
+    """Load the HIP library (raises if it is absent: the product has no fallback)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    path = path or LIB_PATH
+    if not os.path.exists(path):
+        raise RepeatHipError(RP_EINVAL, f"{path} not built; run __graft_entry__.build() or pyrepeat_amd.build")
+    L = ctypes.CDLL(path)
+    vp = ctypes.c_void_p
+    dp = ctypes.POINTER(ctypes.c_double)
+    i64p = ctypes.POINTER(ctypes.c_int64)
+    L.rp_ctx_create.argtypes = [ctypes.c_int, ctypes.c_uint64, ctypes.POINTER(vp)]
+    L.rp_ctx_destroy.argtypes = [vp]
+    L.rp_last_error.argtypes = [vp]
+    L.rp_last_error.restype = ctypes.c_char_p
+    L.rp_ctx_hbm_in_use.argtypes = [vp, ctypes.POINTER(ctypes.c_uint64)]
+    L.rp_fit_create.argtypes = [vp, ctypes.POINTER(RpSystem), ctypes.POINTER(RpOptions), ctypes.POINTER(vp)]
+    L.rp_fit_destroy.argtypes = [vp]
+    L.rp_fit_info_get.argtypes = [vp, ctypes.POINTER(RpFitInfo)]
+    L.rp_fit_grid.argtypes = [vp, ctypes.POINTER(ctypes.c_uint8), i64p, dp, dp]
+    L.rp_fit_ewald.argtypes = [vp, dp, dp, dp, dp]
+    L.rp_fit_accumulate.argtypes = [vp]
+    L.rp_fit_normal_equations.argtypes = [vp, dp, dp, dp, dp]
+    L.rp_fit_solve.argtypes = [vp, ctypes.POINTER(RpConstraints), dp, dp, ctypes.POINTER(ctypes.c_int32)]
+    L.rp_fit_phi_matvec.argtypes = [vp, dp, dp]
+    L.rp_fit_phi.argtypes = [vp, ctypes.c_int64, dp, dp]
+    L.rp_fit_timings.argtypes = [vp, dp]
+    for name in EXPORTED:
+        if getattr(L, name).restype is None and name != "rp_last_error":
+            getattr(L, name).restype = ctypes.c_int
+    _lib = L
+    return L
+
+
+def ptr(a, ctype=ctypes.c_double):
+    return None if a is None else a.ctypes.data_as(ctypes.POINTER(ctype))
+
+
+def check(rc, ctx=None):
+    if rc != RP_OK:
+        msg = load().rp_last_error(ctx).decode(errors="replace")
+        if rc == RP_ESINGULAR:
+            raise SingularKKTError(rc, msg)
+        raise RepeatHipError(rc, msg)
+
+
+class Context:
+    """One device context (stream + HBM budget)."""
+
+    def __init__(self, device: int = 0, hbm_budget: int = 0):
+        L = load()
+        h = ctypes.c_void_p()
+        check(L.rp_ctx_create(int(device), int(hbm_budget), ctypes.byref(h)))
+        self.handle = h
+        self.device = device
+
+    def hbm_in_use(self):
+        v = ctypes.c_uint64()
+        check(load().rp_ctx_hbm_in_use(self.handle, ctypes.byref(v)), self.handle)
+        return int(v.value)
+
+    def close(self):
+        if getattr(self, "handle", None):
+            load().rp_ctx_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+_default_ctx = None
+
+
+def default_context():
+    global _default_ctx
+    if _default_ctx is None:
+        _default_ctx = Context(0, 0)
+    return _default_ctx
+
+
+class Fit:
+    """Device-resident fit (filter done at construction)."""
+
+    def __init__(self, ctx: Context, n_grid, axis, atomic_numbers, atom_xyz, esp, esp_order=RP_ORDER_CUBE,
+                 fit_type=1, precision=RP_PREC_FP64, vdw_factor=1.0, vdw_max=1000.0, r_cutoff=20.0):
+        L = load()
+        self.ctx = ctx
+        self._Z = np.ascontiguousarray(atomic_numbers, dtype=np.int32)
+        self._xyz = np.ascontiguousarray(atom_xyz, dtype=np.float64).reshape(-1, 3)
+        esp = np.ascontiguousarray(esp)
+        if esp.dtype not in (np.float32, np.float64):
+            esp = esp.astype(np.float64)
+        self._esp = esp
+        sysd = RpSystem()
+        for i in range(3):
+            sysd.n_grid[i] = int(n_grid[i])
+        ax = np.asarray(axis, dtype=np.float64).ravel()
+        for i in range(9):
+            sysd.axis[i] = float(ax[i])
+        sysd.n_atoms = len(self._Z)
+        sysd.atomic_number = ptr(self._Z, ctypes.c_int32)
+        sysd.atom_xyz = ptr(self._xyz)
+        sysd.esp = esp.ctypes.data_as(ctypes.c_void_p)
+        sysd.esp_dtype = RP_F32 if esp.dtype == np.float32 else RP_F64
+        sysd.esp_order = int(esp_order)
+        opt = RpOptions(int(fit_type), int(precision), float(vdw_factor), float(vdw_max), float(r_cutoff))
+        h = ctypes.c_void_p()
+        check(L.rp_fit_create(ctx.handle, ctypes.byref(sysd), ctypes.byref(opt), ctypes.byref(h)), ctx.handle)
+        self.handle = h
+        self.n_atoms = len(self._Z)
+        self.info = self._info()
+
+    def _info(self):
+        inf = RpFitInfo()
+        check(load().rp_fit_info_get(self.handle, ctypes.byref(inf)), self.ctx.handle)
+        return inf
+
+    def grid(self, mask=False):
+        L = load()
+        inf = self.info
+        idx = np.empty(inf.n_keep, dtype=np.int64)
+        v = np.empty(inf.n_keep, dtype=np.float64)
+        xyz = np.empty((inf.n_keep, 3), dtype=np.float64)
+        m = np.empty(inf.n_total, dtype=np.uint8) if mask else None
+        check(L.rp_fit_grid(self.handle, ptr(m, ctypes.c_uint8), ptr(idx, ctypes.c_int64), ptr(v), ptr(xyz)),
+              self.ctx.handle)
+        return (xyz, v, idx, m) if mask else (xyz, v, idx)
+
+    def ewald(self):
+        inf = self.info
+        kv = np.empty((inf.n_k, 3))
+        kc = np.empty(inf.n_k)
+        sh = np.empty((inf.n_shift, 3))
+        sp = np.empty((inf.n_shift_pruned, 3))
+        check(load().rp_fit_ewald(self.handle, ptr(kv), ptr(kc), ptr(sh), ptr(sp)), self.ctx.handle)
+        return kv, kc, sh, sp
+
+    def accumulate(self):
+        check(load().rp_fit_accumulate(self.handle), self.ctx.handle)
+
+    def normal_equations(self):
+        N = self.n_atoms
+        ata = np.empty((N, N))
+        atb = np.empty(N)
+        sp = np.empty(N)
+        sv = np.empty(1)
+        check(load().rp_fit_normal_equations(self.handle, ptr(ata), ptr(atb), ptr(sp), ptr(sv)), self.ctx.handle)
+        return ata, atb, sp, float(sv[0])
+
+    def solve(self, con):
+        """``con``: dict from :func:`pyrepeat_amd.params.constraint_arrays`."""
+        N = self.n_atoms
+        nr = int(con["n_reduced"])
+        c = RpConstraints()
+        c.q_tot = float(con["q_tot"])
+        c.n_reduced = nr
+        arrays = {}
+        for name, dt, ct in (("red_ptr", np.int32, ctypes.c_int32), ("red_src", np.int32, ctypes.c_int32),
+                             ("red_mult", np.float64, ctypes.c_double), ("weights", np.float64, ctypes.c_double),
+                             ("diag_add", np.float64, ctypes.c_double), ("rhs_add", np.float64, ctypes.c_double),
+                             ("expand_src", np.int32, ctypes.c_int32)):
+            a = con.get(name)
+            if a is None:
+                setattr(c, name, None)
+                continue
+            arrays[name] = np.ascontiguousarray(a, dtype=dt)
+            setattr(c, name, ptr(arrays[name], ct))
+        q = np.empty(N)
+        sol = np.empty(nr + 1)
+        fb = ctypes.c_int32(0)
+        check(load().rp_fit_solve(self.handle, ctypes.byref(c), ptr(q), ptr(sol), ctypes.byref(fb)), self.ctx.handle)
+        return q, sol
+
+    def phi_matvec(self, q):
+        q = np.ascontiguousarray(q, dtype=np.float64)
+        out = np.empty(self.info.n_keep)
+        check(load().rp_fit_phi_matvec(self.handle, ptr(q), ptr(out)), self.ctx.handle)
+        return out
+
+    def phi(self, grid_xyz):
+        g = np.ascontiguousarray(grid_xyz, dtype=np.float64).reshape(-1, 3)
+        out = np.empty((len(g), self.n_atoms))
+        check(load().rp_fit_phi(self.handle, len(g), ptr(g), ptr(out)), self.ctx.handle)
+        return out
+
+    def timings(self):
+        t = np.zeros(3)
+        check(load().rp_fit_timings(self.handle, ptr(t)), self.ctx.handle)
+        return {"filter_ms": t[0], "accumulate_ms": t[1], "solve_ms": t[2]}
+
+    def close(self):
+        if getattr(self, "handle", None):
+            load().rp_fit_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

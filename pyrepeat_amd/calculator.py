@@ -1,0 +1,150 @@
+"""``ChargeCalculator``: the reference's fit-path interface on the HIP library.
+
+Same constructor arguments, method names and results as ``repeat.py``'s
+``ChargeCalculator`` (``repeat.py:1733-2535``) for the REPEAT fit:
+
+- ``__init__`` reads the cube, runs the vdW filter and the Ewald setup on the
+  device (``repeat.py:1736-1786``);
+- ``fit_charges()`` streams the basis and solves the constrained least squares
+  on the device, returning one charge per atom (``repeat.py:2184-2324``);
+- ``compute_stats(charges, need_v_coul)`` returns ``rms_error``, ``avrg_qm``,
+  ``avrg_coul`` and optionally ``V_coul`` (``repeat.py:2326-2441``).
+
+``precision="fp64"`` is the plain fp64 path (the reference's ``--fp64``);
+``precision="split"`` is the split form (int8 fixed-point slices on MFMA with
+exact accumulation, fp32 real space, fp64 across tiles and in the solve).
+Every computation runs in ``librepeat_hip.so``; nothing falls back to the CPU.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from . import _abi
+from .cube import read_gauss_cube, v_pot_view
+from .params import constraint_arrays, read_qeq_params, read_resp_params, read_symmetry
+
+PRECISIONS = {"fp64": _abi.RP_PREC_FP64, "split": _abi.RP_PREC_OZAKI}
+
+
+class ChargeCalculator:
+    def __init__(self, cube_file=None, fit_flag=1, vdw_factor=1.0, vdw_max=1000.0, R_cutoff=20.0, q_tot=0.0,
+                 symm_file=None, resp_file=None, qeq_file=None, n_cores=None, block_k=64, use_jit=True,
+                 precision="fp64", cube=None, context=None):
+        del n_cores, block_k, use_jit  # CPU scheduling knobs of the reference; no meaning on the device
+        self.cube_file = cube_file
+        self.fit_flag = int(fit_flag)
+        self.vdw_factor = float(vdw_factor)
+        self.vdw_max = float(vdw_max)
+        self.R_cutoff = float(R_cutoff)
+        self.q_tot = float(q_tot)
+        self.precision = precision
+        self.cube = cube if cube is not None else read_gauss_cube(cube_file)
+        self.n_atoms = int(self.cube["n_atoms"])
+        self.ctx = context or _abi.default_context()
+        self._fit = _abi.Fit(self.ctx, self.cube["n_grid"], self.cube["axis_vector"], self.cube["atom_index"],
+                             self.cube["atom_pos"], self.cube["esp_cube"], _abi.RP_ORDER_CUBE, self.fit_flag,
+                             PRECISIONS[precision], self.vdw_factor, self.vdw_max, self.R_cutoff)
+        self.symm_data = read_symmetry(symm_file) if symm_file else None
+        self.resp_data = read_resp_params(resp_file) if resp_file else None
+        self.qeq_data = read_qeq_params(qeq_file) if qeq_file else None
+        self._grid = None
+        self._sum_phi = None
+        self._sum_V = None
+        self._charges = None
+
+    # ---- filter results (downloaded lazily; the fit itself keeps them on the device)
+    def _grid_arrays(self):
+        if self._grid is None:
+            self._grid = self._fit.grid()
+        return self._grid
+
+    @property
+    def grid_pos(self):
+        return self._grid_arrays()[0]
+
+    @property
+    def V_pot(self):
+        return self._grid_arrays()[1]
+
+    @property
+    def n_grid_points(self):
+        return int(self._fit.info.n_keep)
+
+    def V_pot_cube(self):
+        return v_pot_view(self.cube)
+
+    # ---- fit
+    def constraints(self):
+        return constraint_arrays(self.n_atoms, self.cube["atom_index"], self.q_tot, self.symm_data,
+                                 self.resp_data, self.qeq_data)
+
+    def fit_charges(self):
+        self._fit.accumulate()
+        q, sol = self._fit.solve(self.constraints())
+        _, _, sum_phi, sum_v = self._fit.normal_equations()
+        # The reference scatters the *unreduced* sum_phi as if it were reduced (repeat.py:2317-2320, 2491-2509).
+        self._sum_phi = _expand_sum_phi_quirk(sum_phi, self.symm_data, self.n_atoms) if self.symm_data else sum_phi
+        self._sum_phi_true = sum_phi
+        self._sum_V = sum_v
+        self._solution = sol
+        self._charges = q
+        return q
+
+    def normal_equations(self):
+        """``ATA, ATb, sum_phi, sum_V`` of the last fit (the reference's accumulators)."""
+        return self._fit.normal_equations()
+
+    def compute_stats(self, charges, need_v_coul=True):
+        n = self.n_grid_points
+        avrg_coul = float(self._sum_phi @ charges) / n if self._sum_phi is not None else None
+        v = self.V_pot
+        avrg_qm = float(np.mean(v))
+        v_coul = self._fit.phi_matvec(charges)
+        if avrg_coul is None:
+            avrg_coul = float(np.mean(v_coul))
+        diff = (v_coul - avrg_coul) - (v - avrg_qm)
+        rms = float(np.sqrt(np.sum(diff ** 2) / n))
+        out = {"rms_error": rms, "avrg_qm": avrg_qm, "avrg_coul": avrg_coul}
+        if need_v_coul:
+            out["V_coul"] = v_coul
+        return out
+
+    def rms_from_normal_equations(self, charges, corrected=False):
+        """RMS of the fit from the accumulated Gram of [Phi | V | 1], without a second basis pass.
+
+        sum((Phi q - c) - (V - vbar))^2 = q'ATA q - 2 q'ATb + V'V - 2 t (sum_phi'q - sum_V) + n t^2,
+        t = c - vbar, c = the reference's average Coulomb potential (``corrected`` uses the true sum_phi).
+        """
+        ata, atb, sum_phi, sum_v, vtv, n = self._fit_gram_extras()
+        q = np.asarray(charges, dtype=np.float64)
+        sp = sum_phi if corrected or self._sum_phi is None else self._sum_phi
+        c = float(sp @ q) / n
+        t = c - sum_v / n
+        ss = q @ ata @ q - 2 * (q @ atb) + vtv - 2 * t * (sum_phi @ q - sum_v) + n * t * t
+        return float(np.sqrt(max(ss, 0.0) / n))
+
+    def _fit_gram_extras(self):
+        ata, atb, sum_phi, sum_v = self._fit.normal_equations()
+        v = self.V_pot
+        return ata, atb, sum_phi, sum_v, float(v @ v), self.n_grid_points
+
+    def timings(self):
+        return self._fit.timings()
+
+    def close(self):
+        self._fit.close()
+
+
+def _expand_sum_phi_quirk(sum_phi_given, symm, n_atoms):
+    trees = symm["trees"]
+    linked = set()
+    for t in trees:
+        linked.update(t["linked"])
+    independent = [i for i in range(n_atoms) if i not in linked]
+    out = np.zeros(n_atoms)
+    for i, idx in enumerate(independent):
+        out[idx] = sum_phi_given[i]
+    for t in trees:
+        for ln in t["linked"]:
+            out[ln] = out[t["base"]]
+    return out

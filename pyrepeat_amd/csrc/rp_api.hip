@@ -1,0 +1,407 @@
+// C ABI entry points of librepeat_hip.so and the fit lifecycle.
+#include <algorithm>
+#include <cmath>
+#include <mutex>
+
+#include "rp_fit_common.h"
+
+namespace rp {
+
+namespace {
+std::mutex g_err_mu;
+std::string g_err;
+}  // namespace
+
+void set_global_error(const std::string &msg) {
+    std::lock_guard<std::mutex> lk(g_err_mu);
+    g_err = msg;
+}
+
+DBuf::~DBuf() { release(); }
+
+void DBuf::release() {
+    if (p) {
+        (void)hipFree(p);
+        if (ctx) ctx->in_use -= bytes;
+    }
+    p = nullptr;
+    bytes = 0;
+}
+
+int DBuf::alloc(Ctx *c, size_t n) {
+    release();
+    ctx = c;
+    if (n == 0) n = 16;
+    if (c->budget && c->in_use + n > c->budget)
+        return c->fail(RP_EBUDGET, "HBM budget exceeded: need " + std::to_string(n) + " bytes, " +
+                                       std::to_string(c->budget - c->in_use) + " left");
+    hipError_t e = hipMalloc(&p, n);
+    if (e != hipSuccess) {
+        p = nullptr;
+        return c->fail(RP_ENOMEM, std::string("hipMalloc failed: ") + hipGetErrorString(e));
+    }
+    bytes = n;
+    c->in_use += n;
+    return RP_OK;
+}
+
+namespace {
+
+// 2 pi r / n for integer-reduced r, as (cos, sin).
+void phase_table(int hmin, int hmax, int n, std::vector<double2> &out) {
+    const double two_pi = 6.283185307179586476925286766559;
+    out.resize((size_t)(hmax - hmin + 1) * n);
+    for (int h = hmin; h <= hmax; ++h)
+        for (int i = 0; i < n; ++i) {
+            long long r = ((long long)h * i) % n;
+            if (r < 0) r += n;
+            const double ang = two_pi * (double)r / (double)n;
+            out[(size_t)(h - hmin) * n + i] = make_double2(std::cos(ang), std::sin(ang));
+        }
+}
+
+// Basis at arbitrary points: direct fp64 cos(k.(g - R)) and the real-space sum (repeat.py:831-872).
+__global__ __launch_bounds__(256) void k_phi_points(int64_t m, const double *__restrict__ g, int N,
+                                                    const double *__restrict__ atoms, int K,
+                                                    const double *__restrict__ kvecs, const double *__restrict__ kcoef,
+                                                    EpiArgs e, double *__restrict__ out) {
+    const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (t >= m * N) return;
+    const int64_t r = t / N;
+    const int a = (int)(t % N);
+    const double gx = g[3 * r], gy = g[3 * r + 1], gz = g[3 * r + 2];
+    const double dx = gx - atoms[3 * a], dy = gy - atoms[3 * a + 1], dz = gz - atoms[3 * a + 2];
+    double rec = 0.0;
+    for (int k = 0; k < K; ++k) {
+        const double kr = (dx * kvecs[3 * k] + dy * kvecs[3 * k + 1]) + dz * kvecs[3 * k + 2];
+        rec += cos(kr) * kcoef[k];
+    }
+    out[t] = rec + e.real_space(gx, gy, gz, a);
+}
+
+__global__ __launch_bounds__(256) void k_zero(double *p, int64_t n) {
+    const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (t < n) p[t] = 0.0;
+}
+
+}  // namespace
+
+int prepare_device(rp_fit *fit, FitDevice &dv, int precision) {
+    Ctx *ctx = fit->ctx;
+    hipStream_t st = ctx->stream;
+    const Setup &s = fit->setup;
+    const int N = fit->n_atoms;
+    dv.K = (int)s.kcoefs.size();
+    dv.KP = dv.K > 0 ? (int)round_up(2 * dv.K, precision == RP_PREC_FP64 ? 16 : 64) : 0;
+    dv.NP = (int)round_up(N + 2, 128);
+    dv.S = (int)(fit->shifts_pruned.size() / 3);
+    dv.geom.nx = s.nx;
+    dv.geom.ny = s.ny;
+    dv.geom.nz = s.nz;
+    std::copy(s.axis, s.axis + 9, dv.geom.ax);
+    if (dv.K > 0) {
+        std::vector<double2> ex, ey, ez;
+        phase_table(0, 7, s.nx, ex);
+        phase_table(-7, 7, s.ny, ey);
+        phase_table(-7, 7, s.nz, ez);
+        RP_CHECK(dv.ex.alloc(ctx, sizeof(double2) * ex.size()));
+        RP_CHECK(dv.ey.alloc(ctx, sizeof(double2) * ey.size()));
+        RP_CHECK(dv.ez.alloc(ctx, sizeof(double2) * ez.size()));
+        RP_HIP(ctx, hipMemcpyAsync(dv.ex.p, ex.data(), sizeof(double2) * ex.size(), hipMemcpyHostToDevice, st));
+        RP_HIP(ctx, hipMemcpyAsync(dv.ey.p, ey.data(), sizeof(double2) * ey.size(), hipMemcpyHostToDevice, st));
+        RP_HIP(ctx, hipMemcpyAsync(dv.ez.p, ez.data(), sizeof(double2) * ez.size(), hipMemcpyHostToDevice, st));
+        RP_CHECK(dv.khkl.alloc(ctx, sizeof(int) * s.khkl.size()));
+        RP_CHECK(dv.kcoef.alloc(ctx, sizeof(double) * s.kcoefs.size()));
+        RP_CHECK(dv.kvecs.alloc(ctx, sizeof(double) * s.kvecs.size()));
+        RP_HIP(ctx, hipMemcpyAsync(dv.khkl.p, s.khkl.data(), sizeof(int) * s.khkl.size(), hipMemcpyHostToDevice, st));
+        RP_HIP(ctx, hipMemcpyAsync(dv.kcoef.p, s.kcoefs.data(), sizeof(double) * s.kcoefs.size(), hipMemcpyHostToDevice, st));
+        RP_HIP(ctx, hipMemcpyAsync(dv.kvecs.p, s.kvecs.data(), sizeof(double) * s.kvecs.size(), hipMemcpyHostToDevice, st));
+    }
+    RP_CHECK(dv.shifts.alloc(ctx, sizeof(double) * fit->shifts_pruned.size()));
+    RP_HIP(ctx, hipMemcpyAsync(dv.shifts.p, fit->shifts_pruned.data(), sizeof(double) * fit->shifts_pruned.size(),
+                               hipMemcpyHostToDevice, st));
+    dv.epi.geom = dv.geom;
+    dv.epi.idx = fit->d_idx.as<int>();
+    dv.epi.v = fit->d_v.as<double>();
+    dv.epi.N = N;
+    dv.epi.fit_type = s.fit_type;
+    dv.epi.atoms = fit->d_atoms.as<double>();
+    dv.epi.shifts = dv.shifts.as<double>();
+    dv.epi.S = dv.S;
+    dv.epi.R2 = s.r_cutoff * s.r_cutoff;
+    dv.epi.sqrt_alpha = s.sqrt_alpha;
+
+    // chunking: at most 16384 points per chunk, at most ~1 GiB of chunk buffers
+    const int64_t M = fit->n_keep;
+    const int64_t per_row = 8 * (int64_t)(dv.KP + dv.NP);
+    int64_t rows = std::min<int64_t>(16384, round_up(M, 128));
+    while (rows > 128 && rows * per_row > ((int64_t)1 << 30)) rows /= 2;
+    rows = std::max<int64_t>(128, round_up(rows, 128));
+    dv.rows = rows;
+    const int nt = dv.NP / 128;
+    dv.tiles.clear();
+    for (int bi = 0; bi < nt; ++bi)
+        for (int bj = bi; bj < nt; ++bj) dv.tiles.push_back(make_int2(bi, bj));
+    int splits = 1;
+    while ((int64_t)dv.tiles.size() * splits < 1024 && rows / (splits * 2) >= 256 && (rows / (splits * 2)) % 64 == 0) splits *= 2;
+    dv.splits = splits;
+    RP_CHECK(dv.d_tiles.alloc(ctx, sizeof(int2) * dv.tiles.size()));
+    RP_HIP(ctx, hipMemcpyAsync(dv.d_tiles.p, dv.tiles.data(), sizeof(int2) * dv.tiles.size(), hipMemcpyHostToDevice, st));
+    return RP_OK;
+}
+
+int run_accumulate(rp_fit *fit) {
+    Ctx *ctx = fit->ctx;
+    hipStream_t st = ctx->stream;
+    FitDevice dv;
+    RP_CHECK(prepare_device(fit, dv, fit->opt.precision));
+    const int64_t g2 = (int64_t)dv.NP * dv.NP;
+    if (fit->d_gram.bytes != sizeof(double) * (size_t)g2) RP_CHECK(fit->d_gram.alloc(ctx, sizeof(double) * g2));
+    hipEvent_t e0, e1;
+    RP_HIP(ctx, hipEventCreate(&e0));
+    RP_HIP(ctx, hipEventCreate(&e1));
+    RP_HIP(ctx, hipEventRecord(e0, st));
+    k_zero<<<ceil_div(g2, 256), 256, 0, st>>>(fit->d_gram.as<double>(), g2);
+    int rc = fit->opt.precision == RP_PREC_FP64 ? accumulate_f64(fit, dv) : accumulate_split(fit, dv);
+    if (rc != RP_OK) return rc;
+    k_symmetrize<<<ceil_div(g2, 256), 256, 0, st>>>(fit->d_gram.as<double>(), dv.NP, 128);
+    RP_HIP(ctx, hipGetLastError());
+    RP_HIP(ctx, hipEventRecord(e1, st));
+    RP_HIP(ctx, hipStreamSynchronize(st));
+    float ms = 0.f;
+    RP_HIP(ctx, hipEventElapsedTime(&ms, e0, e1));
+    fit->t_accum_ms = ms;
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    fit->na = fit->n_atoms + 2;
+    fit->accumulated = true;
+    return RP_OK;
+}
+
+int run_phi_points(rp_fit *fit, int64_t m, const double *grid_xyz, double *phi_out) {
+    Ctx *ctx = fit->ctx;
+    hipStream_t st = ctx->stream;
+    FitDevice dv;
+    RP_CHECK(prepare_device(fit, dv, RP_PREC_FP64));
+    const int N = fit->n_atoms;
+    DBuf d_g, d_out;
+    RP_CHECK(d_g.alloc(ctx, sizeof(double) * 3 * std::max<int64_t>(m, 1)));
+    RP_CHECK(d_out.alloc(ctx, sizeof(double) * std::max<int64_t>(m * N, 1)));
+    RP_HIP(ctx, hipMemcpyAsync(d_g.p, grid_xyz, sizeof(double) * 3 * m, hipMemcpyHostToDevice, st));
+    if (m * N > 0)
+        k_phi_points<<<ceil_div(m * N, 256), 256, 0, st>>>(m, d_g.as<double>(), N, fit->d_atoms.as<double>(),
+                                                           fit->setup.fit_type == 1 ? dv.K : 0, dv.kvecs.as<double>(),
+                                                           dv.kcoef.as<double>(), dv.epi, d_out.as<double>());
+    RP_HIP(ctx, hipGetLastError());
+    RP_HIP(ctx, hipMemcpyAsync(phi_out, d_out.p, sizeof(double) * m * N, hipMemcpyDeviceToHost, st));
+    RP_HIP(ctx, hipStreamSynchronize(st));
+    return RP_OK;
+}
+
+}  // namespace rp
+
+using namespace rp;
+
+extern "C" {
+
+int rp_ctx_create(int device, uint64_t hbm_budget_bytes, rp_ctx **out) {
+    if (!out) {
+        set_global_error("rp_ctx_create: out is NULL");
+        return RP_EINVAL;
+    }
+    *out = nullptr;
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess || device < 0 || device >= n) {
+        set_global_error("rp_ctx_create: no such HIP device");
+        return RP_EINVAL;
+    }
+    e = hipSetDevice(device);
+    if (e != hipSuccess) {
+        set_global_error(std::string("hipSetDevice failed: ") + hipGetErrorString(e));
+        return RP_EHIP;
+    }
+    rp_ctx *c = new rp_ctx();
+    c->device = device;
+    c->budget = hbm_budget_bytes;
+    (void)hipGetDeviceProperties(&c->prop, device);
+    e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        delete c;
+        set_global_error(std::string("hipStreamCreate failed: ") + hipGetErrorString(e));
+        return RP_EHIP;
+    }
+    *out = c;
+    return RP_OK;
+}
+
+int rp_ctx_destroy(rp_ctx *ctx) {
+    if (!ctx) return RP_OK;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return RP_OK;
+}
+
+const char *rp_last_error(const rp_ctx *ctx) {
+    if (ctx) return ctx->err.c_str();
+    return g_err.c_str();
+}
+
+int rp_ctx_hbm_in_use(const rp_ctx *ctx, uint64_t *bytes) {
+    if (!ctx || !bytes) return RP_EINVAL;
+    *bytes = ctx->in_use;
+    return RP_OK;
+}
+
+int rp_fit_create(rp_ctx *ctx, const rp_system *sys, const rp_options *opt, rp_fit **out) {
+    if (!ctx || !sys || !opt || !out) return ctx ? ctx->fail(RP_EINVAL, "rp_fit_create: NULL argument") : RP_EINVAL;
+    *out = nullptr;
+    if (sys->n_atoms < 1 || !sys->atomic_number || !sys->atom_xyz || !sys->esp)
+        return ctx->fail(RP_EINVAL, "rp_fit_create: system needs atoms, positions and ESP");
+    if (sys->esp_dtype != RP_F64 && sys->esp_dtype != RP_F32) return ctx->fail(RP_EINVAL, "bad esp_dtype");
+    if (sys->esp_order != RP_ORDER_CUBE && sys->esp_order != RP_ORDER_FLAT) return ctx->fail(RP_EINVAL, "bad esp_order");
+    if (opt->precision != RP_PREC_FP64 && opt->precision != RP_PREC_OZAKI) return ctx->fail(RP_EINVAL, "bad precision");
+    RP_HIP(ctx, hipSetDevice(ctx->device));
+    rp_fit *fit = new rp_fit();
+    fit->ctx = ctx;
+    fit->opt = *opt;
+    fit->n_atoms = sys->n_atoms;
+    fit->Z.assign(sys->atomic_number, sys->atomic_number + sys->n_atoms);
+    fit->atom_xyz.assign(sys->atom_xyz, sys->atom_xyz + 3 * sys->n_atoms);
+    int rc = build_setup(sys, opt, fit->setup, ctx->err);
+    if (rc == RP_OK) rc = fit->d_atoms.alloc(ctx, sizeof(double) * 3 * sys->n_atoms);
+    if (rc == RP_OK) {
+        hipError_t e = hipMemcpyAsync(fit->d_atoms.p, sys->atom_xyz, sizeof(double) * 3 * sys->n_atoms,
+                                      hipMemcpyHostToDevice, ctx->stream);
+        if (e != hipSuccess) rc = ctx->fail(RP_EHIP, std::string("hipMemcpyAsync failed: ") + hipGetErrorString(e));
+    }
+    if (rc == RP_OK) rc = run_filter(fit, sys);
+    if (rc != RP_OK) {
+        delete fit;
+        return rc;
+    }
+    *out = fit;
+    return RP_OK;
+}
+
+int rp_fit_destroy(rp_fit *fit) {
+    if (!fit) return RP_OK;
+    (void)hipSetDevice(fit->ctx->device);
+    (void)hipStreamSynchronize(fit->ctx->stream);
+    delete fit;
+    return RP_OK;
+}
+
+int rp_fit_info_get(const rp_fit *fit, rp_fit_info *o) {
+    if (!fit || !o) return RP_EINVAL;
+    const Setup &s = fit->setup;
+    o->n_total = fit->n_total;
+    o->n_keep = fit->n_keep;
+    o->n_eff = (int64_t)s.eff_smin.size();
+    o->n_atoms = fit->n_atoms;
+    o->n_k = (int32_t)s.kcoefs.size();
+    o->n_shift = (int32_t)(s.shifts.size() / 3);
+    o->n_shift_pruned = (int32_t)(fit->shifts_pruned.size() / 3);
+    o->volume = s.volume;
+    o->alpha = s.alpha;
+    o->sqrt_alpha = s.sqrt_alpha;
+    o->v_mean = fit->v_mean;
+    return RP_OK;
+}
+
+int rp_fit_grid(const rp_fit *fit, uint8_t *mask, int64_t *flat_idx, double *v, double *grid_xyz) {
+    if (!fit) return RP_EINVAL;
+    Ctx *ctx = fit->ctx;
+    hipStream_t st = ctx->stream;
+    RP_HIP(ctx, hipSetDevice(ctx->device));
+    if (mask) RP_HIP(ctx, hipMemcpyAsync(mask, fit->d_mask.p, (size_t)fit->n_total, hipMemcpyDeviceToHost, st));
+    std::vector<int> idx;
+    if (flat_idx || grid_xyz) {
+        idx.resize(fit->n_keep);
+        RP_HIP(ctx, hipMemcpyAsync(idx.data(), fit->d_idx.p, sizeof(int) * fit->n_keep, hipMemcpyDeviceToHost, st));
+    }
+    if (v) RP_HIP(ctx, hipMemcpyAsync(v, fit->d_v.p, sizeof(double) * fit->n_keep, hipMemcpyDeviceToHost, st));
+    RP_HIP(ctx, hipStreamSynchronize(st));
+    const Setup &s = fit->setup;
+    for (int64_t t = 0; t < (int64_t)idx.size(); ++t) {
+        if (flat_idx) flat_idx[t] = idx[t];
+        if (grid_xyz) {
+            const int64_t f = idx[t], layer = (int64_t)s.nx * s.ny;
+            const int64_t k = f / layer, rem = f - k * layer, j = rem / s.nx, i = rem - j * s.nx;
+            for (int c = 0; c < 3; ++c) {
+                double u = (double)i * s.axis[c];
+                u = u + (double)j * s.axis[3 + c];
+                grid_xyz[3 * t + c] = u + (double)k * s.axis[6 + c];
+            }
+        }
+    }
+    return RP_OK;
+}
+
+int rp_fit_ewald(const rp_fit *fit, double *kvecs, double *kcoefs, double *shifts, double *shifts_pruned) {
+    if (!fit) return RP_EINVAL;
+    const Setup &s = fit->setup;
+    if (kvecs) std::copy(s.kvecs.begin(), s.kvecs.end(), kvecs);
+    if (kcoefs) std::copy(s.kcoefs.begin(), s.kcoefs.end(), kcoefs);
+    if (shifts) std::copy(s.shifts.begin(), s.shifts.end(), shifts);
+    if (shifts_pruned) std::copy(fit->shifts_pruned.begin(), fit->shifts_pruned.end(), shifts_pruned);
+    return RP_OK;
+}
+
+int rp_fit_accumulate(rp_fit *fit) {
+    if (!fit) return RP_EINVAL;
+    RP_HIP(fit->ctx, hipSetDevice(fit->ctx->device));
+    return run_accumulate(fit);
+}
+
+int rp_fit_normal_equations(const rp_fit *fit, double *ata, double *atb, double *sum_phi, double *sum_v) {
+    if (!fit) return RP_EINVAL;
+    Ctx *ctx = fit->ctx;
+    if (!fit->accumulated) return ctx->fail(RP_ESTATE, "rp_fit_normal_equations before rp_fit_accumulate");
+    RP_HIP(ctx, hipSetDevice(ctx->device));
+    const int N = fit->n_atoms;
+    const int NP = (int)round_up(N + 2, 128);
+    std::vector<double> g((size_t)NP * NP);
+    RP_HIP(ctx, hipMemcpyAsync(g.data(), fit->d_gram.p, sizeof(double) * g.size(), hipMemcpyDeviceToHost, ctx->stream));
+    RP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    for (int a = 0; a < N; ++a) {
+        if (ata)
+            for (int b = 0; b < N; ++b) ata[(int64_t)a * N + b] = g[(size_t)a * NP + b];
+        if (atb) atb[a] = g[(size_t)a * NP + N];
+        if (sum_phi) sum_phi[a] = g[(size_t)a * NP + N + 1];
+    }
+    if (sum_v) {
+        sum_v[0] = g[(size_t)N * NP + N + 1];
+    }
+    return RP_OK;
+}
+
+int rp_fit_solve(rp_fit *fit, const rp_constraints *con, double *q_out, double *solution_out, int32_t *used_fallback) {
+    if (!fit || !con || !q_out) return fit ? fit->ctx->fail(RP_EINVAL, "rp_fit_solve: NULL argument") : RP_EINVAL;
+    RP_HIP(fit->ctx, hipSetDevice(fit->ctx->device));
+    return run_solve(fit, con, q_out, solution_out, used_fallback);
+}
+
+int rp_fit_phi_matvec(rp_fit *fit, const double *q, double *v_coul) {
+    if (!fit || !q || !v_coul) return RP_EINVAL;
+    RP_HIP(fit->ctx, hipSetDevice(fit->ctx->device));
+    return run_phi_matvec(fit, q, v_coul);
+}
+
+int rp_fit_phi(rp_fit *fit, int64_t m, const double *grid_xyz, double *phi_out) {
+    if (!fit || m < 0 || (m > 0 && (!grid_xyz || !phi_out))) return RP_EINVAL;
+    RP_HIP(fit->ctx, hipSetDevice(fit->ctx->device));
+    return run_phi_points(fit, m, grid_xyz, phi_out);
+}
+
+int rp_fit_timings(const rp_fit *fit, double *t3) {
+    if (!fit || !t3) return RP_EINVAL;
+    t3[0] = fit->t_filter_ms;
+    t3[1] = fit->t_accum_ms;
+    t3[2] = fit->t_solve_ms;
+    return RP_OK;
+}
+
+}  // extern "C"

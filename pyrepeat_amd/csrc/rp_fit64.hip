@@ -1,0 +1,355 @@
+// Normal-equation accumulation, plain fp64 form (fit_charges steps 1-3, repeat.py:2195-2262).
+//
+// The kept points are streamed in chunks of `rows` points.  Per chunk:
+//   1. k_grid_phase  G[m][2k], G[m][2k+1] = w_k (cos, sin)(k . g_m)      (rows x KP, fp64)
+//      k . g = 2 pi (h i/nx + l j/ny + n k/nz) exactly, so the phase is the
+//      product of three 1-D tables built on the host from integer-reduced angles.
+//   2. k_phi_gemm    Phi = G . B,  B[2k][a], B[2k+1][a] = (cos, sin)(k . R_a)   (fp64 MFMA)
+//      Its epilogue adds the real-space sum over the pruned shifts,
+//      erfc(sqrt(alpha) r)/r for r^2 <= Rc^2 (self image skipped at r^2 <= 1e-20),
+//      and fills the two augmented columns V and 1, so that one Gram of
+//      [Phi | V | 1] carries Phi^T Phi, Phi^T V, sum(Phi), sum(V) and the count.
+//      cos(k.(g - R)) = cos(k.g)cos(k.R) + sin(k.g)sin(k.R) is the identity used
+//      (reference: direct cos of k.(g-R), repeat.py:831-846; real space repeat.py:849-872).
+//   3. k_syrk        upper-triangular 128x128 tiles of [Phi|V|1]^T [Phi|V|1], split over
+//      rows for occupancy, partials reduced in a fixed order (deterministic).
+#include <algorithm>
+#include <cmath>
+
+#include "rp_fit_common.h"
+#include "rp_mfma.h"
+
+namespace rp {
+
+namespace {
+
+constexpr int BM = 128, BN = 128, BK = 16;
+
+// A tile (BM x BK) from row-major src (ld), transposed into As[BK][BM]; B tile (BK x BN) row-major into Bs[BK][BN].
+struct TileLoader {
+    // A: thread t loads row t/2, k-segment (t%2)*8 .. +8
+    static __device__ __forceinline__ void load_a(const double *src, int64_t ld, int64_t row0, int k0, double (&r)[8]) {
+        const int t = threadIdx.x;
+        const double2 *p = reinterpret_cast<const double2 *>(src + (row0 + (t >> 1)) * ld + k0 + (t & 1) * 8);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            double2 v = p[q];
+            r[2 * q] = v.x;
+            r[2 * q + 1] = v.y;
+        }
+    }
+    static __device__ __forceinline__ void store_a(double (*As)[BM], const double (&r)[8]) {
+        const int t = threadIdx.x;
+#pragma unroll
+        for (int q = 0; q < 8; ++q) As[(t & 1) * 8 + q][t >> 1] = r[q];
+    }
+    // row-major (BK x BN) block: thread t loads k-row t/16, columns (t%16)*8 .. +8
+    static __device__ __forceinline__ void load_b(const double *src, int64_t ld, int64_t krow0, int64_t col0, double (&r)[8]) {
+        const int t = threadIdx.x;
+        const double2 *p = reinterpret_cast<const double2 *>(src + (krow0 + (t >> 4)) * ld + col0 + (t & 15) * 8);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            double2 v = p[q];
+            r[2 * q] = v.x;
+            r[2 * q + 1] = v.y;
+        }
+    }
+    static __device__ __forceinline__ void store_b(double (*Bs)[BN], const double (&r)[8]) {
+        const int t = threadIdx.x;
+#pragma unroll
+        for (int q = 0; q < 8; ++q) Bs[t >> 4][(t & 15) * 8 + q] = r[q];
+    }
+};
+
+// 128x128 block of C = A . B accumulated in acc (4 waves, 2x2, each 64x64 = 4x4 MFMA tiles).
+__device__ __forceinline__ void mma_tile(const double (*As)[BM], const double (*Bs)[BN], dvec4 (&acc)[4][4]) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int wm = wave >> 1, wn = wave & 1;
+#pragma unroll
+    for (int kk = 0; kk < BK / 4; ++kk) {
+        double af[4], bf[4];
+#pragma unroll
+        for (int mt = 0; mt < 4; ++mt) af[mt] = As[kk * 4 + (lane >> 4)][wm * 64 + mt * 16 + (lane & 15)];
+#pragma unroll
+        for (int nt = 0; nt < 4; ++nt) bf[nt] = Bs[kk * 4 + (lane >> 4)][wn * 64 + nt * 16 + (lane & 15)];
+#pragma unroll
+        for (int mt = 0; mt < 4; ++mt)
+#pragma unroll
+            for (int nt = 0; nt < 4; ++nt) acc[mt][nt] = F64x16::mma(af[mt], bf[nt], acc[mt][nt]);
+    }
+}
+
+__global__ __launch_bounds__(256) void k_grid_phase_f64(FitGeom g, const int *__restrict__ idx, int64_t row0,
+                                                        int64_t rows_valid, const int *__restrict__ khkl,
+                                                        const double *__restrict__ kcoef, int K,
+                                                        const double2 *__restrict__ ex, const double2 *__restrict__ ey,
+                                                        const double2 *__restrict__ ez, double *__restrict__ G, int KP) {
+    const int p = blockIdx.x * 256 + threadIdx.x;
+    const int64_t r = blockIdx.y;
+    if (2 * p >= KP) return;
+    double c = 0.0, s = 0.0;
+    if (r < rows_valid && p < K) {
+        int i, j, k;
+        g.ijk(idx[row0 + r], i, j, k);
+        const double2 a = ex[khkl[3 * p] * g.nx + i];
+        const double2 b = ey[(khkl[3 * p + 1] + 7) * g.ny + j];
+        const double2 d = ez[(khkl[3 * p + 2] + 7) * g.nz + k];
+        const double re = a.x * b.x - a.y * b.y, im = a.x * b.y + a.y * b.x;
+        const double w = kcoef[p];
+        c = (re * d.x - im * d.y) * w;
+        s = (re * d.y + im * d.x) * w;
+    }
+    reinterpret_cast<double2 *>(G + r * KP)[p] = make_double2(c, s);
+}
+
+__global__ __launch_bounds__(256) void k_atom_table_f64(int K, const double *__restrict__ kvecs, int N,
+                                                        const double *__restrict__ atoms, double *__restrict__ B,
+                                                        int NP) {
+    const int a = blockIdx.x * 256 + threadIdx.x;
+    const int p = blockIdx.y;
+    if (a >= NP) return;
+    double c = 0.0, s = 0.0;
+    if (p < K && a < N) {
+        const double th = (kvecs[3 * p] * atoms[3 * a] + kvecs[3 * p + 1] * atoms[3 * a + 1]) + kvecs[3 * p + 2] * atoms[3 * a + 2];
+        sincos(th, &s, &c);
+    }
+    B[(int64_t)(2 * p) * NP + a] = c;
+    B[(int64_t)(2 * p + 1) * NP + a] = s;
+}
+
+__global__ __launch_bounds__(256) void k_phi_gemm_f64(const double *__restrict__ G, int KP, const double *__restrict__ B,
+                                                      int NP, double *__restrict__ Phi, EpiArgs e) {
+    __shared__ double As[2][BK][BM];
+    __shared__ double Bs[2][BK][BN];
+    const int64_t m0 = (int64_t)blockIdx.y * BM;
+    const int n0 = blockIdx.x * BN;
+    dvec4 acc[4][4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) acc[a][b] = dvec4{0.0, 0.0, 0.0, 0.0};
+    const int nk = KP / BK;
+    if (nk > 0) {
+        double ra[8], rb[8];
+        TileLoader::load_a(G, KP, m0, 0, ra);
+        TileLoader::load_b(B, NP, 0, n0, rb);
+        TileLoader::store_a(As[0], ra);
+        TileLoader::store_b(Bs[0], rb);
+        __syncthreads();
+        for (int kt = 0; kt < nk; ++kt) {
+            const bool more = kt + 1 < nk;
+            if (more) {
+                TileLoader::load_a(G, KP, m0, (kt + 1) * BK, ra);
+                TileLoader::load_b(B, NP, (int64_t)(kt + 1) * BK, n0, rb);
+            }
+            mma_tile(As[kt & 1], Bs[kt & 1], acc);
+            if (more) {
+                TileLoader::store_a(As[(kt + 1) & 1], ra);
+                TileLoader::store_b(Bs[(kt + 1) & 1], rb);
+            }
+            __syncthreads();
+        }
+    }
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int wm = wave >> 1, wn = wave & 1;
+#pragma unroll
+    for (int mt = 0; mt < 4; ++mt)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int64_t row = m0 + wm * 64 + mt * 16 + F64x16::d_row(lane, v);
+            double gx = 0, gy = 0, gz = 0;
+            const bool valid_row = row < e.rows_valid;
+            if (valid_row) e.geom.xyz(e.idx[e.row0 + row], gx, gy, gz);
+#pragma unroll
+            for (int nt = 0; nt < 4; ++nt) {
+                const int col = n0 + wn * 64 + nt * 16 + F64x16::d_col(lane);
+                double val = 0.0;
+                if (valid_row) {
+                    if (col < e.N)
+                        val = acc[mt][nt][v] + e.real_space(gx, gy, gz, col);
+                    else if (col == e.N)
+                        val = e.v[e.row0 + row];
+                    else if (col == e.N + 1)
+                        val = 1.0;
+                }
+                Phi[row * NP + col] = val;
+            }
+        }
+}
+
+// Upper-triangular tiles of Phi^T Phi over rows [split*rps, ...), one 128x128 tile per block.
+__global__ __launch_bounds__(256) void k_syrk_f64(const double *__restrict__ Phi, int NP, int64_t rows, int rps,
+                                                  const int2 *__restrict__ tiles, int n_tiles, double *__restrict__ out,
+                                                  int direct) {
+    __shared__ double As[2][BK][BM];
+    __shared__ double Bs[2][BK][BN];
+    const int2 tile = tiles[blockIdx.x];
+    const int64_t r0 = (int64_t)blockIdx.y * rps;
+    const int64_t r1 = min((int64_t)rows, r0 + rps);
+    dvec4 acc[4][4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) acc[a][b] = dvec4{0.0, 0.0, 0.0, 0.0};
+    const int nk = (int)((r1 - r0) / BK);
+    if (nk > 0) {
+        double ra[8], rb[8];
+        TileLoader::load_b(Phi, NP, r0, tile.x * BM, ra);
+        TileLoader::load_b(Phi, NP, r0, tile.y * BN, rb);
+        TileLoader::store_b(reinterpret_cast<double(*)[BN]>(As[0]), ra);
+        TileLoader::store_b(Bs[0], rb);
+        __syncthreads();
+        for (int kt = 0; kt < nk; ++kt) {
+            const bool more = kt + 1 < nk;
+            if (more) {
+                TileLoader::load_b(Phi, NP, r0 + (int64_t)(kt + 1) * BK, tile.x * BM, ra);
+                TileLoader::load_b(Phi, NP, r0 + (int64_t)(kt + 1) * BK, tile.y * BN, rb);
+            }
+            mma_tile(As[kt & 1], Bs[kt & 1], acc);
+            if (more) {
+                TileLoader::store_b(reinterpret_cast<double(*)[BN]>(As[(kt + 1) & 1]), ra);
+                TileLoader::store_b(Bs[(kt + 1) & 1], rb);
+            }
+            __syncthreads();
+        }
+    }
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int wm = wave >> 1, wn = wave & 1;
+#pragma unroll
+    for (int mt = 0; mt < 4; ++mt)
+#pragma unroll
+        for (int nt = 0; nt < 4; ++nt)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int r = wm * 64 + mt * 16 + F64x16::d_row(lane, v);
+                const int c = wn * 64 + nt * 16 + F64x16::d_col(lane);
+                if (direct) {
+                    double *dst = out + (int64_t)(tile.x * BM + r) * NP + tile.y * BN + c;
+                    *dst += acc[mt][nt][v];
+                } else {
+                    out[((int64_t)blockIdx.y * n_tiles + blockIdx.x) * (BM * BN) + r * BN + c] = acc[mt][nt][v];
+                }
+            }
+}
+
+__global__ __launch_bounds__(256) void k_reduce_splits(const double *__restrict__ part, int n_splits, int n_tiles,
+                                                       const int2 *__restrict__ tiles, double *__restrict__ gram, int NP) {
+    const int t = blockIdx.x;
+    const int2 tile = tiles[t];
+    for (int e = threadIdx.x; e < BM * BN; e += 256) {
+        double s = 0.0;
+        for (int sp = 0; sp < n_splits; ++sp) s += part[((int64_t)sp * n_tiles + t) * (BM * BN) + e];
+        gram[(int64_t)(tile.x * BM + e / BN) * NP + tile.y * BN + (e % BN)] += s;
+    }
+}
+
+// v[r] = sum_a Phi[r][a] q[a] over the N basis columns; one wave per row.
+__global__ __launch_bounds__(256) void k_rowdot(const double *__restrict__ Phi, int NP, int N, const double *__restrict__ q,
+                                                int64_t rows_valid, double *__restrict__ out) {
+    const int64_t r = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (r >= rows_valid) return;
+    double s = 0.0;
+    for (int a = lane; a < N; a += 64) s += Phi[r * NP + a] * q[a];
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+    if (lane == 0) out[r] = s;
+}
+
+}  // namespace
+
+// Lower triangle from the upper tiles (diagonal tiles are full).
+__global__ void k_symmetrize(double *gram, int NP, int tile) {
+    const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (e >= (int64_t)NP * NP) return;
+    const int i = (int)(e / NP), j = (int)(e % NP);
+    if (i / tile > j / tile) gram[e] = gram[(int64_t)j * NP + i];
+}
+
+int accumulate_f64(rp_fit *fit, FitDevice &dv) {
+    Ctx *ctx = fit->ctx;
+    hipStream_t st = ctx->stream;
+    const int K = dv.K;
+    const int KP = dv.KP;
+    const int NP = dv.NP;
+    const int64_t M = fit->n_keep;
+    const int64_t rows = dv.rows;     // chunk rows, multiple of BM
+    DBuf d_G, d_B, d_phi, d_part;
+    if (KP > 0) {
+        RP_CHECK(d_G.alloc(ctx, sizeof(double) * rows * KP));
+        RP_CHECK(d_B.alloc(ctx, sizeof(double) * (size_t)KP * NP));
+        dim3 gb(ceil_div(NP, 256), KP / 2);
+        k_atom_table_f64<<<gb, 256, 0, st>>>(K, dv.kvecs.as<double>(), fit->n_atoms, fit->d_atoms.as<double>(), d_B.as<double>(), NP);
+    }
+    RP_CHECK(d_phi.alloc(ctx, sizeof(double) * rows * NP));
+    const int n_tiles = (int)dv.tiles.size();
+    const int n_splits = dv.splits;
+    if (n_splits > 1) RP_CHECK(d_part.alloc(ctx, sizeof(double) * (size_t)n_splits * n_tiles * BM * BN));
+    const int rps = (int)(rows / n_splits);
+    for (int64_t r0 = 0; r0 < M; r0 += rows) {
+        const int64_t valid = std::min<int64_t>(rows, M - r0);
+        if (KP > 0) {
+            dim3 gg(ceil_div(KP / 2, 256), rows);
+            k_grid_phase_f64<<<gg, 256, 0, st>>>(dv.geom, fit->d_idx.as<int>(), r0, valid, dv.khkl.as<int>(), dv.kcoef.as<double>(), K,
+                                                 dv.ex.as<double2>(), dv.ey.as<double2>(), dv.ez.as<double2>(), d_G.as<double>(), KP);
+        }
+        EpiArgs e = dv.epi;
+        e.row0 = r0;
+        e.rows_valid = valid;
+        dim3 gp(NP / BN, rows / BM);
+        k_phi_gemm_f64<<<gp, 256, 0, st>>>(d_G.as<double>(), KP, d_B.as<double>(), NP, d_phi.as<double>(), e);
+        dim3 gs(n_tiles, n_splits);
+        k_syrk_f64<<<gs, 256, 0, st>>>(d_phi.as<double>(), NP, rows, rps, dv.d_tiles.as<int2>(), n_tiles,
+                                       n_splits > 1 ? d_part.as<double>() : fit->d_gram.as<double>(), n_splits > 1 ? 0 : 1);
+        if (n_splits > 1)
+            k_reduce_splits<<<n_tiles, 256, 0, st>>>(d_part.as<double>(), n_splits, n_tiles, dv.d_tiles.as<int2>(),
+                                                     fit->d_gram.as<double>(), NP);
+    }
+    RP_HIP(ctx, hipGetLastError());
+    RP_HIP(ctx, hipStreamSynchronize(st));  // chunk buffers are released on return
+    return RP_OK;
+}
+
+int run_phi_matvec(rp_fit *fit, const double *q, double *v_coul) {
+    Ctx *ctx = fit->ctx;
+    hipStream_t st = ctx->stream;
+    FitDevice dv;
+    RP_CHECK(prepare_device(fit, dv, RP_PREC_FP64));
+    const int K = dv.K, KP = dv.KP, NP = dv.NP, N = fit->n_atoms;
+    const int64_t M = fit->n_keep, rows = dv.rows;
+    DBuf d_G, d_B, d_phi, d_q, d_out;
+    if (KP > 0) {
+        RP_CHECK(d_G.alloc(ctx, sizeof(double) * rows * KP));
+        RP_CHECK(d_B.alloc(ctx, sizeof(double) * (size_t)KP * NP));
+        dim3 gb(ceil_div(NP, 256), KP / 2);
+        k_atom_table_f64<<<gb, 256, 0, st>>>(K, dv.kvecs.as<double>(), N, fit->d_atoms.as<double>(), d_B.as<double>(), NP);
+    }
+    RP_CHECK(d_phi.alloc(ctx, sizeof(double) * rows * NP));
+    RP_CHECK(d_q.alloc(ctx, sizeof(double) * N));
+    RP_CHECK(d_out.alloc(ctx, sizeof(double) * M));
+    RP_HIP(ctx, hipMemcpyAsync(d_q.p, q, sizeof(double) * N, hipMemcpyHostToDevice, st));
+    for (int64_t r0 = 0; r0 < M; r0 += rows) {
+        const int64_t valid = std::min<int64_t>(rows, M - r0);
+        if (KP > 0) {
+            dim3 gg(ceil_div(KP / 2, 256), rows);
+            k_grid_phase_f64<<<gg, 256, 0, st>>>(dv.geom, fit->d_idx.as<int>(), r0, valid, dv.khkl.as<int>(), dv.kcoef.as<double>(), K,
+                                                 dv.ex.as<double2>(), dv.ey.as<double2>(), dv.ez.as<double2>(), d_G.as<double>(), KP);
+        }
+        EpiArgs e = dv.epi;
+        e.row0 = r0;
+        e.rows_valid = valid;
+        dim3 gp(NP / BN, rows / BM);
+        k_phi_gemm_f64<<<gp, 256, 0, st>>>(d_G.as<double>(), KP, d_B.as<double>(), NP, d_phi.as<double>(), e);
+        k_rowdot<<<ceil_div(valid, 4), 256, 0, st>>>(d_phi.as<double>(), NP, N, d_q.as<double>(), valid, d_out.as<double>() + r0);
+    }
+    RP_HIP(ctx, hipGetLastError());
+    RP_HIP(ctx, hipMemcpyAsync(v_coul, d_out.p, sizeof(double) * M, hipMemcpyDeviceToHost, st));
+    RP_HIP(ctx, hipStreamSynchronize(st));
+    return RP_OK;
+}
+
+int accumulate_split(rp_fit *fit, FitDevice &dv) {
+    (void)dv;
+    return fit->ctx->fail(RP_EINVAL, "split precision accumulation is not available in this build");
+}
+
+}  // namespace rp

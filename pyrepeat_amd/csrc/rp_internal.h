@@ -1,0 +1,133 @@
+// Internal declarations shared by the librepeat_hip.so translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/repeat_hip.h"
+
+namespace rp {
+
+// Last error of calls made without a context (e.g. rp_ctx_create failures).
+void set_global_error(const std::string &msg);
+
+struct Ctx;
+
+// Device allocation charged against the owning context's HBM budget.
+struct DBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+    Ctx *ctx = nullptr;
+    DBuf() = default;
+    DBuf(const DBuf &) = delete;
+    DBuf &operator=(const DBuf &) = delete;
+    ~DBuf();
+    int alloc(Ctx *c, size_t n);   // frees any previous allocation first
+    void release();
+    template <class T> T *as() const { return static_cast<T *>(p); }
+};
+
+struct Ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    uint64_t budget = 0;
+    uint64_t in_use = 0;
+    std::string err;
+    hipDeviceProp_t prop{};
+    int fail(int code, const std::string &msg) {
+        err = msg;
+        return code;
+    }
+};
+
+// Checks a HIP call; records the error on ctx and returns RP_EHIP from the caller.
+#define RP_HIP(ctx, call)                                                                        \
+    do {                                                                                          \
+        hipError_t e_ = (call);                                                                   \
+        if (e_ != hipSuccess)                                                                     \
+            return (ctx)->fail(RP_EHIP, std::string(#call " failed: ") + hipGetErrorString(e_)); \
+    } while (0)
+
+#define RP_CHECK(expr)            \
+    do {                          \
+        int rc_ = (expr);         \
+        if (rc_ != RP_OK) return rc_; \
+    } while (0)
+
+inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
+inline int64_t round_up(int64_t a, int64_t b) { return ceil_div(a, b) * b; }
+
+// ---------------------------------------------------------------- geometry / Ewald (host)
+
+struct Vec3 {
+    double x, y, z;
+};
+
+// Everything the host derives from the cube and the options before the kernels run.
+struct Setup {
+    int nx = 0, ny = 0, nz = 0;
+    double axis[9] = {};
+    double box[9] = {};
+    double volume = 0.0;        // signed, repeat.py:1763
+    double recip[9] = {};
+    double alpha = 0.0, sqrt_alpha = 0.0, r_cutoff = 20.0;
+    int fit_type = 1;
+    int nmax[3] = {0, 0, 0};
+    // k-set in loop order (repeat.py:1127-1140)
+    std::vector<int32_t> khkl;  // K x 3 integer indices
+    std::vector<double> kvecs;  // K x 3
+    std::vector<double> kcoefs; // K
+    std::vector<double> shifts; // S x 3, full list (repeat.py:815-822)
+    // filter
+    std::vector<double> eff_xyz;    // E x 3
+    std::vector<double> eff_smin;   // squared-distance thresholds (see filter_thresholds)
+    std::vector<double> eff_smax;
+    double max_rmin = 0.0, max_rmax = 0.0;
+};
+
+int build_setup(const rp_system *sys, const rp_options *opt, Setup &s, std::string &err);
+double sqrt_threshold(double r);  // largest double s with fl(sqrt(s)) <= r
+std::vector<double> prune_shifts(const std::vector<double> &shifts, const double gmin[3], const double gmax[3],
+                                 const std::vector<double> &atom_xyz, double r_cutoff);
+
+}  // namespace rp
+
+// The fit object behind the opaque rp_fit handle.
+struct rp_fit {
+    rp::Ctx *ctx = nullptr;
+    rp::Setup setup;
+    int32_t n_atoms = 0;
+    std::vector<int32_t> Z;
+    std::vector<double> atom_xyz;
+    rp_options opt{};
+    int64_t n_total = 0;
+    int64_t n_keep = 0;
+    double v_mean = 0.0;
+    std::vector<double> shifts_pruned;
+    bool accumulated = false;
+    double t_filter_ms = 0, t_accum_ms = 0, t_solve_ms = 0;
+
+    // device state
+    rp::DBuf d_mask;      // n_total bytes
+    rp::DBuf d_idx;       // n_keep int32 flat indices
+    rp::DBuf d_v;         // n_keep doubles (mean subtracted)
+    rp::DBuf d_atoms;     // N x 3 doubles
+    rp::DBuf d_gram;      // NA x NA doubles, NA = N + 2 (augmented with V and 1)
+    int32_t na = 0;       // augmented width N + 2
+};
+
+struct rp_ctx : rp::Ctx {};
+
+namespace rp {
+// Phases implemented in the .hip files.
+int run_filter(rp_fit *fit, const rp_system *sys);
+int run_accumulate(rp_fit *fit);
+int run_solve(rp_fit *fit, const rp_constraints *con, double *q_out, double *solution_out, int32_t *used_fallback);
+int run_phi_matvec(rp_fit *fit, const double *q, double *v_coul);
+int run_phi_points(rp_fit *fit, int64_t m, const double *grid_xyz, double *phi_out);
+}  // namespace rp

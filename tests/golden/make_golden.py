@@ -1,0 +1,64 @@
+"""Writes oracle fixtures for the seeded workloads (run from the repo root, CPU only).
+
+    python tests/golden/make_golden.py water crystal200 mof1000
+
+Each ``<name>.json`` holds the fit settings, the kept-point count, a checksum of
+the keep mask, the oracle's charges (full precision) and its RMS errors.  The
+inputs are regenerated from their seeds by ``pyrepeat_amd.synth``; the oracle
+is ``oracle/repeat_oracle.py`` (CPU restatement of the reference).
+"""
+import hashlib
+import json
+import os
+import sys
+import tempfile
+import time
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+
+import numpy as np  # noqa: E402
+
+from oracle import repeat_oracle as O  # noqa: E402
+from pyrepeat_amd import synth  # noqa: E402
+
+
+def mask_digest(mask):
+    return hashlib.sha256(np.packbits(np.asarray(mask, dtype=np.uint8)).tobytes()).hexdigest()
+
+
+def run(name):
+    s = getattr(synth, name)()
+    symm = resp = None
+    with tempfile.TemporaryDirectory() as td:
+        if s.symm_groups:
+            p = os.path.join(td, "symm.txt")
+            open(p, "w").write(s.symm_text())
+            symm = O.read_symmetry(p)
+        if s.resp_text:
+            p = os.path.join(td, "resp.txt")
+            open(p, "w").write(s.resp_text)
+            resp = O.read_resp_params(p)
+    stats = s.n_atoms * 1.0 * np.prod(s.n_grid) < 5e8
+    t0 = time.time()
+    r = O.fit(s.n_grid, s.axis, s.atomic_numbers, s.atom_pos, s.esp_flat(), q_tot=s.q_tot, symm=symm, resp=resp,
+              stats=stats)
+    out = {
+        "workload": name, "n_grid": [int(v) for v in s.n_grid], "n_atoms": int(s.n_atoms),
+        "q_tot": s.q_tot, "symmetry_groups": len(s.symm_groups), "resp": s.resp_text,
+        "n_keep": int(len(r.V)), "mask_sha256": mask_digest(r.mask),
+        "n_k": int(len(r.ewald.kvecs)), "n_shift": int(len(r.ewald.shifts)),
+        "n_shift_pruned": int(len(r.ewald.shifts_pruned)),
+        "charges": [float(q) for q in r.charges], "used_fallback": bool(r.used_fallback),
+        "rms": r.rms, "rms_corrected": r.rms_corrected, "sum_V": float(r.sum_V),
+        "oracle_seconds": round(time.time() - t0, 1),
+    }
+    with open(os.path.join(HERE, f"{name}.json"), "w") as f:
+        json.dump(out, f, indent=1)
+    print(name, "done in", out["oracle_seconds"], "s; M =", out["n_keep"], flush=True)
+
+
+if __name__ == "__main__":
+    for n in sys.argv[1:]:
+        run(n)

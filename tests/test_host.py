@@ -1,0 +1,114 @@
+"""CPU tests of the host side: C ABI exports, parameter files, constraint maps, cube I/O."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+from pyrepeat_amd import _abi, params, synth
+from pyrepeat_amd.cube import read_gauss_cube, v_pot_view
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = _abi.load()
+    header = open(os.path.join(ROOT, "include", "repeat_hip.h")).read()
+    declared = set(re.findall(r"^\s*(?:int|const char \*)\s*\**(rp_\w+)\s*\(", header, re.M))
+    assert declared, "no declarations parsed"
+    assert declared == set(_abi.EXPORTED)
+    for name in declared:
+        assert isinstance(getattr(lib, name), ctypes._CFuncPtr)
+
+
+def test_product_has_no_cpu_fallback(tmp_path):
+    with pytest.raises(_abi.RepeatHipError):
+        _abi._lib_saved = _abi._lib
+        try:
+            _abi._lib = None
+            _abi.load(str(tmp_path / "missing.so"))
+        finally:
+            _abi._lib = _abi._lib_saved
+
+
+def _random_trees(rng, n, k):
+    trees = []
+    for _ in range(k):
+        size = int(rng.integers(2, 5))
+        ids = [int(x) for x in rng.choice(n, size=size, replace=False)]
+        trees.append({"base": ids[0], "linked": ids[1:]})
+    return trees
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_merge_sets_reproduce_sequential_symmetry(oracle, seed):
+    """Merge sets + weights == the reference's in-place column/row merges, overlapping groups included."""
+    rng = np.random.default_rng(seed)
+    n = 12
+    X = rng.normal(size=(40, n))
+    A = X.T @ X
+    b = rng.normal(size=n)
+    trees = _random_trees(rng, n, 4)
+    Ar, br, wr, _ = oracle.apply_symmetry(A, b, np.ones(n), np.arange(n), trees, n)
+    con = params.constraint_arrays(n, np.ones(n, dtype=int), 0.0, {"trees": trees})
+    nr = con["n_reduced"]
+    M = np.zeros((nr, n))
+    for I in range(nr):
+        for p in range(con["red_ptr"][I], con["red_ptr"][I + 1]):
+            M[I, con["red_src"][p]] += con["red_mult"][p]
+    np.testing.assert_allclose(M @ A @ M.T, Ar, rtol=1e-12, atol=1e-12)
+    np.testing.assert_allclose(M @ b, br, rtol=1e-12, atol=1e-12)
+    np.testing.assert_array_equal(con["weights"], wr)
+    x = rng.normal(size=nr)
+    q_ref = oracle.expand_charges(x, trees, n)
+    q = np.where(con["expand_src"] >= 0, x[np.maximum(con["expand_src"], 0)], 0.0)
+    np.testing.assert_array_equal(q, q_ref)
+
+
+def test_restraint_terms_match_oracle(oracle):
+    types = np.array([1, 8, 6, 1, 7, 30])
+    A = np.eye(6) * 3.0
+    b = np.zeros(6)
+    resp = {"H": {"strength": 0.1, "target": 0.05}, "N": {"strength": 0.3, "target": -0.2}}
+    con = params.constraint_arrays(6, types, 0.0, None, resp, None)
+    A2, b2 = oracle.apply_resp(A, b, types, resp)
+    np.testing.assert_allclose(np.diag(A2) - 3.0, con["diag_add"])
+    np.testing.assert_allclose(b2, con["rhs_add"])
+    qeq = {"O": {"electroneg": 0.5, "hardness": 0.6}}
+    con = params.constraint_arrays(6, types, 0.0, None, None, qeq)
+    A3, b3 = oracle.apply_qeq(A, b, types, qeq)
+    np.testing.assert_allclose(np.diag(A3) - 3.0, con["diag_add"])
+    np.testing.assert_allclose(b3, con["rhs_add"])
+
+
+def test_parameter_readers_follow_reference(tmp_path, oracle):
+    p = tmp_path / "symm.txt"
+    p.write_text("# groups\n1-3,5  # first\n\n7,9\n")
+    assert params.read_symmetry(str(p)) == oracle.read_symmetry(str(p))
+    r = tmp_path / "resp.txt"
+    r.write_text("# Element Strength Target\nH 0.1 0.0\n")
+    assert params.read_resp_params(str(r)) == {} == oracle.read_resp_params(str(r))
+
+
+def test_cube_roundtrip_and_reference_layout(tmp_path):
+    s = synth.tiny(seed=11, n_atoms=3, n=6, L=7.0, triclinic=True)
+    path = tmp_path / "t.cube"
+    s.write_cube(str(path))
+    c = read_gauss_cube(str(path))
+    np.testing.assert_array_equal(c["n_grid"], s.n_grid)
+    np.testing.assert_array_equal(c["axis_vector"], s.axis)
+    np.testing.assert_array_equal(c["atom_pos"], s.atom_pos)
+    np.testing.assert_allclose(c["esp_cube"], s.esp_cube, rtol=1e-10)
+    # the reference's V_pot[i + j*nx, k] filled x outer, z inner (repeat.py:797-803)
+    nx, ny, nz = s.n_grid
+    vp = np.zeros((nx * ny, nz))
+    idx = 0
+    for i in range(nx):
+        for j in range(ny):
+            for k in range(nz):
+                vp[i + j * nx, k] = c["esp_cube"][idx]
+                idx += 1
+    np.testing.assert_array_equal(v_pot_view(c), vp)
+    # F-order ravel of V_pot is the flat order i + j*nx + k*nx*ny (repeat.py:2086)
+    np.testing.assert_allclose(vp.ravel(order="F"), s.esp_flat(), rtol=1e-10)
