@@ -112,6 +112,12 @@ int rp_ctx_destroy(rp_ctx *ctx);
 const char *rp_last_error(const rp_ctx *ctx);
 int rp_ctx_hbm_in_use(const rp_ctx *ctx, uint64_t *bytes);
 
+/* Kernel timing with HIP events on the context's stream (off by default; enabling resets the
+ * counters).  kind: 0 k-space contraction, 1 Gram, 2 filter mask, 3 KKT solve, 4 grid phases,
+ * 5 Phi slicing.  total_ms sums the launches of that kind since the last reset. */
+int rp_ctx_profile(rp_ctx *ctx, int enable);
+int rp_ctx_kernel_stats(const rp_ctx *ctx, int kind, double *total_ms, int64_t *launches);
+
 /* ---------------------------------------------------------------- fit path */
 /*
  * ChargeCalculator.__init__ (repeat.py:1736-1786): upload the cube, build the
@@ -152,6 +158,13 @@ int rp_fit_normal_equations(const rp_fit *fit, double *ata, double *atb, double 
  */
 int rp_fit_solve(rp_fit *fit, const rp_constraints *con, double *q_out, double *solution_out,
                  int32_t *used_fallback);
+
+/*
+ * One whole fit from the device-resident cube of rp_fit_create: vdW filter, Phi streaming and
+ * accumulation, constrained solve (the reference's _filter_grid + fit_charges,
+ * repeat.py:1922-2113, 2184-2313), with no host-to-device transfer of the grid data.
+ */
+int rp_fit_refit(rp_fit *fit, const rp_constraints *con, double *q_out);
 
 /* compute_stats (repeat.py:2326-2441): v_coul[m] = (Phi q)[m] for every kept point, streamed. */
 int rp_fit_phi_matvec(rp_fit *fit, const double *q, double *v_coul);

@@ -30,8 +30,9 @@ EXPORTED = (
     "rp_ctx_create", "rp_ctx_destroy", "rp_last_error", "rp_ctx_hbm_in_use",
     "rp_fit_create", "rp_fit_destroy", "rp_fit_info_get", "rp_fit_grid", "rp_fit_ewald",
     "rp_fit_accumulate", "rp_fit_normal_equations", "rp_fit_solve", "rp_fit_phi_matvec",
-    "rp_fit_phi", "rp_fit_timings",
+    "rp_fit_phi", "rp_fit_timings", "rp_ctx_profile", "rp_ctx_kernel_stats", "rp_fit_refit",
 )
+KERNEL_KINDS = {"kspace": 0, "gram": 1, "filter": 2, "solve": 3, "phase": 4, "slice": 5}
 
 
 class RepeatHipError(RuntimeError):
@@ -101,6 +102,9 @@ def load(path: str | None = None):
     L.rp_fit_phi_matvec.argtypes = [vp, dp, dp]
     L.rp_fit_phi.argtypes = [vp, ctypes.c_int64, dp, dp]
     L.rp_fit_timings.argtypes = [vp, dp]
+    L.rp_ctx_profile.argtypes = [vp, ctypes.c_int]
+    L.rp_ctx_kernel_stats.argtypes = [vp, ctypes.c_int, dp, i64p]
+    L.rp_fit_refit.argtypes = [vp, ctypes.POINTER(RpConstraints), dp]
     for name in EXPORTED:
         if getattr(L, name).restype is None and name != "rp_last_error":
             getattr(L, name).restype = ctypes.c_int
@@ -134,6 +138,18 @@ class Context:
         v = ctypes.c_uint64()
         check(load().rp_ctx_hbm_in_use(self.handle, ctypes.byref(v)), self.handle)
         return int(v.value)
+
+    def profile(self, enable=True):
+        check(load().rp_ctx_profile(self.handle, int(enable)), self.handle)
+
+    def kernel_stats(self):
+        out = {}
+        for name, kind in KERNEL_KINDS.items():
+            ms = ctypes.c_double()
+            n = ctypes.c_int64()
+            check(load().rp_ctx_kernel_stats(self.handle, kind, ctypes.byref(ms), ctypes.byref(n)), self.handle)
+            out[name] = (ms.value, n.value)
+        return out
 
     def close(self):
         if getattr(self, "handle", None):
@@ -226,9 +242,8 @@ class Fit:
         check(load().rp_fit_normal_equations(self.handle, ptr(ata), ptr(atb), ptr(sp), ptr(sv)), self.ctx.handle)
         return ata, atb, sp, float(sv[0])
 
-    def solve(self, con):
-        """``con``: dict from :func:`pyrepeat_amd.params.constraint_arrays`."""
-        N = self.n_atoms
+    @staticmethod
+    def _constraints(con):
         nr = int(con["n_reduced"])
         c = RpConstraints()
         c.q_tot = float(con["q_tot"])
@@ -244,11 +259,26 @@ class Fit:
                 continue
             arrays[name] = np.ascontiguousarray(a, dtype=dt)
             setattr(c, name, ptr(arrays[name], ct))
+        return c, arrays
+
+    def solve(self, con):
+        """``con``: dict from :func:`pyrepeat_amd.params.constraint_arrays`."""
+        N = self.n_atoms
+        nr = int(con["n_reduced"])
+        c, keep = self._constraints(con)
         q = np.empty(N)
         sol = np.empty(nr + 1)
         fb = ctypes.c_int32(0)
         check(load().rp_fit_solve(self.handle, ctypes.byref(c), ptr(q), ptr(sol), ctypes.byref(fb)), self.ctx.handle)
         return q, sol
+
+    def refit(self, con, c_prepared=None):
+        """Whole fit (filter, accumulate, solve) from the device-resident cube."""
+        c, keep = c_prepared if c_prepared is not None else self._constraints(con)
+        q = np.empty(self.n_atoms)
+        check(load().rp_fit_refit(self.handle, ctypes.byref(c), ptr(q)), self.ctx.handle)
+        self.info = self._info()
+        return q
 
     def phi_matvec(self, q):
         q = np.ascontiguousarray(q, dtype=np.float64)

@@ -95,11 +95,12 @@ int prepare_device(rp_fit *fit, FitDevice &dv, int precision) {
     dv.KP = dv.K > 0 ? (int)round_up(2 * dv.K, precision == RP_PREC_FP64 ? 16 : 64) : 0;
     dv.NP = (int)round_up(N + 2, 128);
     dv.S = (int)(fit->shifts_pruned.size() / 3);
+    const bool rebuild = !dv.ready || dv.shifts_host != fit->shifts_pruned;
     dv.geom.nx = s.nx;
     dv.geom.ny = s.ny;
     dv.geom.nz = s.nz;
     std::copy(s.axis, s.axis + 9, dv.geom.ax);
-    if (dv.K > 0) {
+    if (rebuild && dv.K > 0) {
         std::vector<double2> ex, ey, ez;
         phase_table(0, 7, s.nx, ex);
         phase_table(-7, 7, s.ny, ey);
@@ -117,25 +118,65 @@ int prepare_device(rp_fit *fit, FitDevice &dv, int precision) {
         RP_HIP(ctx, hipMemcpyAsync(dv.kcoef.p, s.kcoefs.data(), sizeof(double) * s.kcoefs.size(), hipMemcpyHostToDevice, st));
         RP_HIP(ctx, hipMemcpyAsync(dv.kvecs.p, s.kvecs.data(), sizeof(double) * s.kvecs.size(), hipMemcpyHostToDevice, st));
     }
-    RP_CHECK(dv.shifts.alloc(ctx, sizeof(double) * fit->shifts_pruned.size()));
-    RP_HIP(ctx, hipMemcpyAsync(dv.shifts.p, fit->shifts_pruned.data(), sizeof(double) * fit->shifts_pruned.size(),
-                               hipMemcpyHostToDevice, st));
+    if (rebuild) {
+        // fractional atom coordinates R . inv(box) and interplanar spacings, for the nearest-image test
+        const double *B = s.box;
+        double cof[9];
+        cof[0] = B[4] * B[8] - B[5] * B[7];
+        cof[1] = B[5] * B[6] - B[3] * B[8];
+        cof[2] = B[3] * B[7] - B[4] * B[6];
+        cof[3] = B[2] * B[7] - B[1] * B[8];
+        cof[4] = B[0] * B[8] - B[2] * B[6];
+        cof[5] = B[1] * B[6] - B[0] * B[7];
+        cof[6] = B[1] * B[5] - B[2] * B[4];
+        cof[7] = B[2] * B[3] - B[0] * B[5];
+        cof[8] = B[0] * B[4] - B[1] * B[3];
+        const double det = B[0] * cof[0] + B[1] * cof[1] + B[2] * cof[2];
+        // inv(box)[c][i] = cof[3*i + c] / det ; frac_i = sum_c R_c inv[c][i]
+        std::vector<double> af(3 * (size_t)N);
+        for (int a = 0; a < N; ++a)
+            for (int i = 0; i < 3; ++i) {
+                double acc = 0.0;
+                for (int c = 0; c < 3; ++c) acc += fit->atom_xyz[3 * a + c] * cof[3 * i + c] / det;
+                af[3 * a + i] = acc;
+            }
+        RP_CHECK(dv.afrac.alloc(ctx, sizeof(double) * af.size()));
+        RP_HIP(ctx, hipMemcpyAsync(dv.afrac.p, af.data(), sizeof(double) * af.size(), hipMemcpyHostToDevice, st));
+        RP_HIP(ctx, hipStreamSynchronize(st));
+        for (int i = 0; i < 3; ++i) {
+            const double *cr = cof + 3 * i;  // |a_j x a_k| for the plane family of axis i
+            // shrink by a relative 1e-12 so rounding can only make the test more conservative
+            dv.epi.h[i] = std::fabs(det) / std::sqrt(cr[0] * cr[0] + cr[1] * cr[1] + cr[2] * cr[2]) * (1.0 - 1e-12);
+        }
+        std::copy(B, B + 9, dv.epi.box);
+        dv.epi.inv_n[0] = 1.0 / s.nx;
+        dv.epi.inv_n[1] = 1.0 / s.ny;
+        dv.epi.inv_n[2] = 1.0 / s.nz;
+    }
+    if (rebuild) {
+        RP_CHECK(dv.shifts.alloc(ctx, sizeof(double) * fit->shifts_pruned.size()));
+        RP_HIP(ctx, hipMemcpyAsync(dv.shifts.p, fit->shifts_pruned.data(), sizeof(double) * fit->shifts_pruned.size(),
+                                   hipMemcpyHostToDevice, st));
+        dv.shifts_host = fit->shifts_pruned;
+    }
     dv.epi.geom = dv.geom;
     dv.epi.idx = fit->d_idx.as<int>();
     dv.epi.v = fit->d_v.as<double>();
     dv.epi.N = N;
     dv.epi.fit_type = s.fit_type;
     dv.epi.atoms = fit->d_atoms.as<double>();
+    dv.epi.afrac = dv.afrac.as<double>();
+    dv.epi.r_cut = s.r_cutoff;
     dv.epi.shifts = dv.shifts.as<double>();
     dv.epi.S = dv.S;
     dv.epi.R2 = s.r_cutoff * s.r_cutoff;
     dv.epi.sqrt_alpha = s.sqrt_alpha;
 
-    // chunking: at most 16384 points per chunk, at most ~1 GiB of chunk buffers
+    // chunking: at most 65536 points per chunk, at most ~2 GiB of chunk buffers
     const int64_t M = fit->n_keep;
     const int64_t per_row = 8 * (int64_t)(dv.KP + dv.NP);
-    int64_t rows = std::min<int64_t>(16384, round_up(M, 128));
-    while (rows > 128 && rows * per_row > ((int64_t)1 << 30)) rows /= 2;
+    int64_t rows = std::min<int64_t>(65536, round_up(M, 128));
+    while (rows > 128 && rows * per_row > ((int64_t)1 << 31)) rows /= 2;
     rows = std::max<int64_t>(128, round_up(rows, 128));
     dv.rows = rows;
     const int nt = dv.NP / 128;
@@ -143,18 +184,28 @@ int prepare_device(rp_fit *fit, FitDevice &dv, int precision) {
     for (int bi = 0; bi < nt; ++bi)
         for (int bj = bi; bj < nt; ++bj) dv.tiles.push_back(make_int2(bi, bj));
     int splits = 1;
-    while ((int64_t)dv.tiles.size() * splits < 1024 && rows / (splits * 2) >= 256 && (rows / (splits * 2)) % 64 == 0) splits *= 2;
+    while ((int64_t)dv.tiles.size() * splits < 512 && rows / (splits * 2) >= 1024 && (rows / (splits * 2)) % 64 == 0) splits *= 2;
     dv.splits = splits;
-    RP_CHECK(dv.d_tiles.alloc(ctx, sizeof(int2) * dv.tiles.size()));
-    RP_HIP(ctx, hipMemcpyAsync(dv.d_tiles.p, dv.tiles.data(), sizeof(int2) * dv.tiles.size(), hipMemcpyHostToDevice, st));
+    if (rebuild || dv.d_tiles.bytes < sizeof(int2) * dv.tiles.size()) {
+        RP_CHECK(dv.d_tiles.grow(ctx, sizeof(int2) * dv.tiles.size()));
+        RP_HIP(ctx, hipMemcpyAsync(dv.d_tiles.p, dv.tiles.data(), sizeof(int2) * dv.tiles.size(), hipMemcpyHostToDevice, st));
+    }
+    dv.ready = true;
     return RP_OK;
+}
+
+int device_setup(rp_fit *fit, int precision, FitDevice *&dv) {
+    if (!fit->dev) fit->dev = new FitDevice();
+    dv = fit->dev;
+    return prepare_device(fit, *dv, precision);
 }
 
 int run_accumulate(rp_fit *fit) {
     Ctx *ctx = fit->ctx;
     hipStream_t st = ctx->stream;
-    FitDevice dv;
-    RP_CHECK(prepare_device(fit, dv, fit->opt.precision));
+    FitDevice *dvp = nullptr;
+    RP_CHECK(device_setup(fit, fit->opt.precision, dvp));
+    FitDevice &dv = *dvp;
     const int64_t g2 = (int64_t)dv.NP * dv.NP;
     if (fit->d_gram.bytes != sizeof(double) * (size_t)g2) RP_CHECK(fit->d_gram.alloc(ctx, sizeof(double) * g2));
     hipEvent_t e0, e1;
@@ -181,12 +232,13 @@ int run_accumulate(rp_fit *fit) {
 int run_phi_points(rp_fit *fit, int64_t m, const double *grid_xyz, double *phi_out) {
     Ctx *ctx = fit->ctx;
     hipStream_t st = ctx->stream;
-    FitDevice dv;
-    RP_CHECK(prepare_device(fit, dv, RP_PREC_FP64));
+    FitDevice *dvp = nullptr;
+    RP_CHECK(device_setup(fit, fit->opt.precision, dvp));
+    FitDevice &dv = *dvp;
     const int N = fit->n_atoms;
-    DBuf d_g, d_out;
-    RP_CHECK(d_g.alloc(ctx, sizeof(double) * 3 * std::max<int64_t>(m, 1)));
-    RP_CHECK(d_out.alloc(ctx, sizeof(double) * std::max<int64_t>(m * N, 1)));
+    DBuf &d_g = fit->ws[20], &d_out = fit->ws[21];
+    RP_CHECK(d_g.grow(ctx, sizeof(double) * 3 * std::max<int64_t>(m, 1)));
+    RP_CHECK(d_out.grow(ctx, sizeof(double) * std::max<int64_t>(m * N, 1)));
     RP_HIP(ctx, hipMemcpyAsync(d_g.p, grid_xyz, sizeof(double) * 3 * m, hipMemcpyHostToDevice, st));
     if (m * N > 0)
         k_phi_points<<<ceil_div(m * N, 256), 256, 0, st>>>(m, d_g.as<double>(), N, fit->d_atoms.as<double>(),
@@ -201,6 +253,8 @@ int run_phi_points(rp_fit *fit, int64_t m, const double *grid_xyz, double *phi_o
 }  // namespace rp
 
 using namespace rp;
+
+rp_fit::~rp_fit() { delete dev; }
 
 extern "C" {
 
@@ -255,6 +309,31 @@ int rp_ctx_hbm_in_use(const rp_ctx *ctx, uint64_t *bytes) {
     return RP_OK;
 }
 
+int rp_ctx_profile(rp_ctx *ctx, int enable) {
+    if (!ctx) return RP_EINVAL;
+    ctx->profile = enable != 0;
+    for (int k = 0; k < KT_N; ++k) {
+        ctx->kt_ms[k] = 0.0;
+        ctx->kt_n[k] = 0;
+    }
+    return RP_OK;
+}
+
+int rp_ctx_kernel_stats(const rp_ctx *ctx, int kind, double *total_ms, int64_t *launches) {
+    if (!ctx || kind < 0 || kind >= KT_N || !total_ms || !launches) return RP_EINVAL;
+    *total_ms = ctx->kt_ms[kind];
+    *launches = ctx->kt_n[kind];
+    return RP_OK;
+}
+
+int rp_fit_refit(rp_fit *fit, const rp_constraints *con, double *q_out) {
+    if (!fit || !con || !q_out) return fit ? fit->ctx->fail(RP_EINVAL, "rp_fit_refit: NULL argument") : RP_EINVAL;
+    RP_HIP(fit->ctx, hipSetDevice(fit->ctx->device));
+    RP_CHECK(run_filter(fit));
+    RP_CHECK(run_accumulate(fit));
+    return run_solve(fit, con, q_out, nullptr, nullptr);
+}
+
 int rp_fit_create(rp_ctx *ctx, const rp_system *sys, const rp_options *opt, rp_fit **out) {
     if (!ctx || !sys || !opt || !out) return ctx ? ctx->fail(RP_EINVAL, "rp_fit_create: NULL argument") : RP_EINVAL;
     *out = nullptr;
@@ -277,7 +356,8 @@ int rp_fit_create(rp_ctx *ctx, const rp_system *sys, const rp_options *opt, rp_f
                                       hipMemcpyHostToDevice, ctx->stream);
         if (e != hipSuccess) rc = ctx->fail(RP_EHIP, std::string("hipMemcpyAsync failed: ") + hipGetErrorString(e));
     }
-    if (rc == RP_OK) rc = run_filter(fit, sys);
+    if (rc == RP_OK) rc = upload_filter(fit, sys);
+    if (rc == RP_OK) rc = run_filter(fit);
     if (rc != RP_OK) {
         delete fit;
         return rc;

@@ -31,16 +31,6 @@ namespace {
 
 constexpr int kBlock = 256;
 
-struct FilterArgs {
-    int nx, ny, nz;
-    double ax[9];
-    double bmin[3];
-    double binv;
-    int bn[3];
-    int fallback;          // 1 if some r_max exceeds the bin edge
-    int64_t n_all;
-};
-
 __device__ __forceinline__ void grid_xyz(const double *ax, int i, int j, int k, double &x, double &y, double &z) {
     x = ((double)i * ax[0] + (double)j * ax[3]) + (double)k * ax[6];
     y = ((double)i * ax[1] + (double)j * ax[4]) + (double)k * ax[7];
@@ -259,12 +249,15 @@ __global__ __launch_bounds__(kBlock) void k_center_and_bounds(int nx, int ny, co
 
 }  // namespace
 
-int run_filter(rp_fit *fit, const rp_system *sys) {
+int upload_filter(rp_fit *fit, const rp_system *sys) {
     Ctx *ctx = fit->ctx;
     const Setup &s = fit->setup;
+    FilterState &fs = fit->filt;
     const int64_t n_total = (int64_t)s.nx * s.ny * s.nz;
     fit->n_total = n_total;
-    const int64_t n_blocks = ceil_div(n_total, kBlock);
+    fs.n_blocks = ceil_div(n_total, kBlock);
+    fs.esp_dtype = sys->esp_dtype;
+    fs.esp_order = sys->esp_order;
 
     // ---- cell list over the grid's bounding box (host)
     double glo[3], ghi[3];
@@ -332,7 +325,7 @@ int run_filter(rp_fit *fit, const rp_system *sys) {
             bin_smax[pos] = s.eff_smax[t];
         }
     }
-    FilterArgs fa{};
+    FilterArgs &fa = fs.fa;
     fa.nx = s.nx;
     fa.ny = s.ny;
     fa.nz = s.nz;
@@ -343,64 +336,83 @@ int run_filter(rp_fit *fit, const rp_system *sys) {
     fa.fallback = s.max_rmax > h ? 1 : 0;
     fa.n_all = n_eff;
 
-    // ---- device buffers
-    DBuf d_bin_start, d_bin_atoms, d_bin_smax, d_all, d_all_smax, d_cnt, d_off, d_vsum, d_mean, d_esp, d_axis, d_bounds;
-    RP_CHECK(d_bin_start.alloc(ctx, sizeof(int) * (nbins + 1)));
-    RP_CHECK(d_bin_atoms.alloc(ctx, sizeof(double4) * bin_atoms.size()));
-    RP_CHECK(d_bin_smax.alloc(ctx, sizeof(double) * bin_smax.size()));
-    RP_CHECK(d_all.alloc(ctx, sizeof(double4) * all_atoms.size()));
-    RP_CHECK(d_all_smax.alloc(ctx, sizeof(double) * all_smax.size()));
-    RP_CHECK(d_cnt.alloc(ctx, sizeof(int) * n_blocks));
-    RP_CHECK(d_off.alloc(ctx, sizeof(int64_t) * (n_blocks + 1)));
-    RP_CHECK(d_vsum.alloc(ctx, sizeof(double) * n_blocks));
-    RP_CHECK(d_mean.alloc(ctx, sizeof(double)));
-    RP_CHECK(d_axis.alloc(ctx, sizeof(double) * 9));
+    const int64_t nb = fs.n_blocks;
+    RP_CHECK(fs.bin_start.alloc(ctx, sizeof(int) * (nbins + 1)));
+    RP_CHECK(fs.bin_atoms.alloc(ctx, sizeof(double4) * bin_atoms.size()));
+    RP_CHECK(fs.bin_smax.alloc(ctx, sizeof(double) * bin_smax.size()));
+    RP_CHECK(fs.all.alloc(ctx, sizeof(double4) * all_atoms.size()));
+    RP_CHECK(fs.all_smax.alloc(ctx, sizeof(double) * all_smax.size()));
+    RP_CHECK(fs.cnt.alloc(ctx, sizeof(int) * nb));
+    RP_CHECK(fs.off.alloc(ctx, sizeof(int64_t) * (nb + 1)));
+    RP_CHECK(fs.vsum.alloc(ctx, sizeof(double) * nb));
+    RP_CHECK(fs.mean.alloc(ctx, sizeof(double)));
+    RP_CHECK(fs.axis.alloc(ctx, sizeof(double) * 9));
     RP_CHECK(fit->d_mask.alloc(ctx, (size_t)n_total));
     const size_t esp_bytes = (size_t)n_total * (sys->esp_dtype == RP_F32 ? 4 : 8);
-    RP_CHECK(d_esp.alloc(ctx, esp_bytes));
+    RP_CHECK(fs.esp.alloc(ctx, esp_bytes));
     hipStream_t st = ctx->stream;
-    RP_HIP(ctx, hipMemcpyAsync(d_bin_start.p, bin_start.data(), sizeof(int) * (nbins + 1), hipMemcpyHostToDevice, st));
-    RP_HIP(ctx, hipMemcpyAsync(d_bin_atoms.p, bin_atoms.data(), sizeof(double4) * bin_atoms.size(), hipMemcpyHostToDevice, st));
-    RP_HIP(ctx, hipMemcpyAsync(d_bin_smax.p, bin_smax.data(), sizeof(double) * bin_smax.size(), hipMemcpyHostToDevice, st));
-    RP_HIP(ctx, hipMemcpyAsync(d_all.p, all_atoms.data(), sizeof(double4) * all_atoms.size(), hipMemcpyHostToDevice, st));
-    RP_HIP(ctx, hipMemcpyAsync(d_all_smax.p, all_smax.data(), sizeof(double) * all_smax.size(), hipMemcpyHostToDevice, st));
-    RP_HIP(ctx, hipMemcpyAsync(d_axis.p, s.axis, sizeof(double) * 9, hipMemcpyHostToDevice, st));
-    RP_HIP(ctx, hipMemcpyAsync(d_esp.p, sys->esp, esp_bytes, hipMemcpyHostToDevice, st));
+    RP_HIP(ctx, hipMemcpyAsync(fs.bin_start.p, bin_start.data(), sizeof(int) * (nbins + 1), hipMemcpyHostToDevice, st));
+    RP_HIP(ctx, hipMemcpyAsync(fs.bin_atoms.p, bin_atoms.data(), sizeof(double4) * bin_atoms.size(), hipMemcpyHostToDevice, st));
+    RP_HIP(ctx, hipMemcpyAsync(fs.bin_smax.p, bin_smax.data(), sizeof(double) * bin_smax.size(), hipMemcpyHostToDevice, st));
+    RP_HIP(ctx, hipMemcpyAsync(fs.all.p, all_atoms.data(), sizeof(double4) * all_atoms.size(), hipMemcpyHostToDevice, st));
+    RP_HIP(ctx, hipMemcpyAsync(fs.all_smax.p, all_smax.data(), sizeof(double) * all_smax.size(), hipMemcpyHostToDevice, st));
+    RP_HIP(ctx, hipMemcpyAsync(fs.axis.p, s.axis, sizeof(double) * 9, hipMemcpyHostToDevice, st));
+    RP_HIP(ctx, hipMemcpyAsync(fs.esp.p, sys->esp, esp_bytes, hipMemcpyHostToDevice, st));
+    RP_HIP(ctx, hipStreamSynchronize(st));  // host staging vectors go out of scope
+    return RP_OK;
+}
 
+int run_filter(rp_fit *fit) {
+    Ctx *ctx = fit->ctx;
+    const Setup &s = fit->setup;
+    FilterState &fs = fit->filt;
+    hipStream_t st = ctx->stream;
+    const int64_t n_blocks = fs.n_blocks;
     hipEvent_t e0, e1;
     RP_HIP(ctx, hipEventCreate(&e0));
     RP_HIP(ctx, hipEventCreate(&e1));
     RP_HIP(ctx, hipEventRecord(e0, st));
-    k_filter_mask<<<n_blocks, kBlock, 0, st>>>(fa, d_bin_start.as<int>(), d_bin_atoms.as<double4>(), d_bin_smax.as<double>(),
-                                               d_all.as<double4>(), d_all_smax.as<double>(), fit->d_mask.as<uint8_t>(),
-                                               d_cnt.as<int>());
-    k_scan_counts<<<1, 1024, 0, st>>>(d_cnt.as<int>(), n_blocks, d_off.as<int64_t>());
+    {
+        KTimer kt(ctx, KT_FILTER);
+        k_filter_mask<<<n_blocks, kBlock, 0, st>>>(fs.fa, fs.bin_start.as<int>(), fs.bin_atoms.as<double4>(),
+                                                   fs.bin_smax.as<double>(), fs.all.as<double4>(), fs.all_smax.as<double>(),
+                                                   fit->d_mask.as<uint8_t>(), fs.cnt.as<int>());
+    }
+    k_scan_counts<<<1, 1024, 0, st>>>(fs.cnt.as<int>(), n_blocks, fs.off.as<int64_t>());
     int64_t n_keep = 0;
-    RP_HIP(ctx, hipMemcpyAsync(&n_keep, d_off.as<int64_t>() + n_blocks, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+    RP_HIP(ctx, hipMemcpyAsync(&n_keep, fs.off.as<int64_t>() + n_blocks, sizeof(int64_t), hipMemcpyDeviceToHost, st));
     RP_HIP(ctx, hipStreamSynchronize(st));
-    fit->n_keep = n_keep;
-    if (n_keep == 0) return ctx->fail(RP_EEMPTY, "the vdW filter kept no grid point");
-    RP_CHECK(fit->d_idx.alloc(ctx, sizeof(int) * n_keep));
-    RP_CHECK(fit->d_v.alloc(ctx, sizeof(double) * n_keep));
-    if (sys->esp_dtype == RP_F32)
-        k_compact<float><<<n_blocks, kBlock, 0, st>>>(s.nx, s.ny, s.nz, sys->esp_order, fit->d_mask.as<uint8_t>(),
-                                                      d_off.as<int64_t>(), d_esp.as<float>(), fit->d_idx.as<int>(),
-                                                      fit->d_v.as<double>(), d_vsum.as<double>());
+    if (n_keep == 0) {
+        fit->n_keep = 0;
+        hipEventDestroy(e0);
+        hipEventDestroy(e1);
+        return ctx->fail(RP_EEMPTY, "the vdW filter kept no grid point");
+    }
+    if (n_keep != fit->n_keep || !fit->d_idx.p) {
+        fit->n_keep = n_keep;
+        RP_CHECK(fit->d_idx.alloc(ctx, sizeof(int) * n_keep));
+        RP_CHECK(fit->d_v.alloc(ctx, sizeof(double) * n_keep));
+    }
+    if (fs.esp_dtype == RP_F32)
+        k_compact<float><<<n_blocks, kBlock, 0, st>>>(s.nx, s.ny, s.nz, fs.esp_order, fit->d_mask.as<uint8_t>(),
+                                                      fs.off.as<int64_t>(), fs.esp.as<float>(), fit->d_idx.as<int>(),
+                                                      fit->d_v.as<double>(), fs.vsum.as<double>());
     else
-        k_compact<double><<<n_blocks, kBlock, 0, st>>>(s.nx, s.ny, s.nz, sys->esp_order, fit->d_mask.as<uint8_t>(),
-                                                       d_off.as<int64_t>(), d_esp.as<double>(), fit->d_idx.as<int>(),
-                                                       fit->d_v.as<double>(), d_vsum.as<double>());
-    k_mean<<<1, 1024, 0, st>>>(d_vsum.as<double>(), n_blocks, d_off.as<int64_t>() + n_blocks, d_mean.as<double>());
+        k_compact<double><<<n_blocks, kBlock, 0, st>>>(s.nx, s.ny, s.nz, fs.esp_order, fit->d_mask.as<uint8_t>(),
+                                                       fs.off.as<int64_t>(), fs.esp.as<double>(), fit->d_idx.as<int>(),
+                                                       fit->d_v.as<double>(), fs.vsum.as<double>());
+    k_mean<<<1, 1024, 0, st>>>(fs.vsum.as<double>(), n_blocks, fs.off.as<int64_t>() + n_blocks, fs.mean.as<double>());
     const int64_t cb = ceil_div(n_keep, kBlock);
-    RP_CHECK(d_bounds.alloc(ctx, sizeof(double) * 6 * cb));
-    k_center_and_bounds<<<cb, kBlock, 0, st>>>(s.nx, s.ny, d_axis.as<double>(), fit->d_idx.as<int>(), fit->d_v.as<double>(),
-                                                n_keep, d_mean.as<double>(), d_bounds.as<double>());
+    if (fs.bounds.bytes < sizeof(double) * 6 * cb) RP_CHECK(fs.bounds.alloc(ctx, sizeof(double) * 6 * cb));
+    k_center_and_bounds<<<cb, kBlock, 0, st>>>(s.nx, s.ny, fs.axis.as<double>(), fit->d_idx.as<int>(), fit->d_v.as<double>(),
+                                                n_keep, fs.mean.as<double>(), fs.bounds.as<double>());
     RP_HIP(ctx, hipGetLastError());
     RP_HIP(ctx, hipEventRecord(e1, st));
     std::vector<double> bounds(6 * cb);
-    RP_HIP(ctx, hipMemcpyAsync(bounds.data(), d_bounds.p, sizeof(double) * 6 * cb, hipMemcpyDeviceToHost, st));
-    RP_HIP(ctx, hipMemcpyAsync(&fit->v_mean, d_mean.p, sizeof(double), hipMemcpyDeviceToHost, st));
+    RP_HIP(ctx, hipMemcpyAsync(bounds.data(), fs.bounds.p, sizeof(double) * 6 * cb, hipMemcpyDeviceToHost, st));
+    RP_HIP(ctx, hipMemcpyAsync(&fit->v_mean, fs.mean.p, sizeof(double), hipMemcpyDeviceToHost, st));
     RP_HIP(ctx, hipStreamSynchronize(st));
+    ctx->collect();
     float ms = 0.f;
     RP_HIP(ctx, hipEventElapsedTime(&ms, e0, e1));
     fit->t_filter_ms = ms;
@@ -419,6 +431,7 @@ int run_filter(rp_fit *fit, const rp_system *sys) {
         fit->shifts_pruned = prune_shifts(s.shifts, gmin, gmax, fit->atom_xyz, s.r_cutoff);
     else
         fit->shifts_pruned = s.shifts;
+    fit->accumulated = false;
     return RP_OK;
 }
 

@@ -166,7 +166,7 @@ __global__ __launch_bounds__(256) void k_phi_gemm_f64(const double *__restrict__
                 double val = 0.0;
                 if (valid_row) {
                     if (col < e.N)
-                        val = acc[mt][nt][v] + e.real_space(gx, gy, gz, col);
+                        val = acc[mt][nt][v] + e.real_space(gx, gy, gz, col, e.idx[e.row0 + row]);
                     else if (col == e.N)
                         val = e.v[e.row0 + row];
                     else if (col == e.N + 1)
@@ -227,7 +227,8 @@ __global__ __launch_bounds__(256) void k_syrk_f64(const double *__restrict__ Phi
                     double *dst = out + (int64_t)(tile.x * BM + r) * NP + tile.y * BN + c;
                     *dst += acc[mt][nt][v];
                 } else {
-                    out[((int64_t)blockIdx.y * n_tiles + blockIdx.x) * (BM * BN) + r * BN + c] = acc[mt][nt][v];
+                    // each (split, tile) slot belongs to one block: partials accumulate across chunks
+                    out[((int64_t)blockIdx.y * n_tiles + blockIdx.x) * (BM * BN) + r * BN + c] += acc[mt][nt][v];
                 }
             }
 }
@@ -257,12 +258,13 @@ __global__ __launch_bounds__(256) void k_rowdot(const double *__restrict__ Phi, 
 
 }  // namespace
 
-// Lower triangle from the upper tiles (diagonal tiles are full).
+// Lower triangle from the upper one (every element with i <= j has been accumulated).
 __global__ void k_symmetrize(double *gram, int NP, int tile) {
+    (void)tile;
     const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (e >= (int64_t)NP * NP) return;
     const int i = (int)(e / NP), j = (int)(e % NP);
-    if (i / tile > j / tile) gram[e] = gram[(int64_t)j * NP + i];
+    if (i > j) gram[e] = gram[(int64_t)j * NP + i];
 }
 
 int accumulate_f64(rp_fit *fit, FitDevice &dv) {
@@ -273,22 +275,27 @@ int accumulate_f64(rp_fit *fit, FitDevice &dv) {
     const int NP = dv.NP;
     const int64_t M = fit->n_keep;
     const int64_t rows = dv.rows;     // chunk rows, multiple of BM
-    DBuf d_G, d_B, d_phi, d_part;
+    DBuf &d_G = fit->ws[0], &d_B = fit->ws[1], &d_phi = fit->ws[2], &d_part = fit->ws[3];
     if (KP > 0) {
-        RP_CHECK(d_G.alloc(ctx, sizeof(double) * rows * KP));
-        RP_CHECK(d_B.alloc(ctx, sizeof(double) * (size_t)KP * NP));
+        RP_CHECK(d_G.grow(ctx, sizeof(double) * rows * KP));
+        RP_CHECK(d_B.grow(ctx, sizeof(double) * (size_t)KP * NP));
         dim3 gb(ceil_div(NP, 256), KP / 2);
         k_atom_table_f64<<<gb, 256, 0, st>>>(K, dv.kvecs.as<double>(), fit->n_atoms, fit->d_atoms.as<double>(), d_B.as<double>(), NP);
     }
-    RP_CHECK(d_phi.alloc(ctx, sizeof(double) * rows * NP));
+    RP_CHECK(d_phi.grow(ctx, sizeof(double) * rows * NP));
     const int n_tiles = (int)dv.tiles.size();
     const int n_splits = dv.splits;
-    if (n_splits > 1) RP_CHECK(d_part.alloc(ctx, sizeof(double) * (size_t)n_splits * n_tiles * BM * BN));
+    const int64_t part_n = (int64_t)n_splits * n_tiles * BM * BN;
+    if (n_splits > 1) {
+        RP_CHECK(d_part.grow(ctx, sizeof(double) * (size_t)part_n));
+        RP_HIP(ctx, hipMemsetAsync(d_part.p, 0, sizeof(double) * (size_t)part_n, st));
+    }
     const int rps = (int)(rows / n_splits);
     for (int64_t r0 = 0; r0 < M; r0 += rows) {
         const int64_t valid = std::min<int64_t>(rows, M - r0);
         if (KP > 0) {
             dim3 gg(ceil_div(KP / 2, 256), rows);
+            KTimer kt(ctx, KT_PHASE);
             k_grid_phase_f64<<<gg, 256, 0, st>>>(dv.geom, fit->d_idx.as<int>(), r0, valid, dv.khkl.as<int>(), dv.kcoef.as<double>(), K,
                                                  dv.ex.as<double2>(), dv.ey.as<double2>(), dv.ez.as<double2>(), d_G.as<double>(), KP);
         }
@@ -296,36 +303,46 @@ int accumulate_f64(rp_fit *fit, FitDevice &dv) {
         e.row0 = r0;
         e.rows_valid = valid;
         dim3 gp(NP / BN, rows / BM);
-        k_phi_gemm_f64<<<gp, 256, 0, st>>>(d_G.as<double>(), KP, d_B.as<double>(), NP, d_phi.as<double>(), e);
+        {
+            KTimer kt(ctx, KT_KSPACE);
+            k_phi_gemm_f64<<<gp, 256, 0, st>>>(d_G.as<double>(), KP, d_B.as<double>(), NP, d_phi.as<double>(), e);
+        }
         dim3 gs(n_tiles, n_splits);
+        KTimer kt(ctx, KT_GRAM);
         k_syrk_f64<<<gs, 256, 0, st>>>(d_phi.as<double>(), NP, rows, rps, dv.d_tiles.as<int2>(), n_tiles,
                                        n_splits > 1 ? d_part.as<double>() : fit->d_gram.as<double>(), n_splits > 1 ? 0 : 1);
-        if (n_splits > 1)
-            k_reduce_splits<<<n_tiles, 256, 0, st>>>(d_part.as<double>(), n_splits, n_tiles, dv.d_tiles.as<int2>(),
-                                                     fit->d_gram.as<double>(), NP);
+    }
+    if (n_splits > 1) {
+        KTimer kt(ctx, KT_GRAM);
+        k_reduce_splits<<<n_tiles, 256, 0, st>>>(d_part.as<double>(), n_splits, n_tiles, dv.d_tiles.as<int2>(),
+                                                 fit->d_gram.as<double>(), NP);
     }
     RP_HIP(ctx, hipGetLastError());
     RP_HIP(ctx, hipStreamSynchronize(st));  // chunk buffers are released on return
+    ctx->collect();
     return RP_OK;
 }
 
 int run_phi_matvec(rp_fit *fit, const double *q, double *v_coul) {
     Ctx *ctx = fit->ctx;
     hipStream_t st = ctx->stream;
-    FitDevice dv;
-    RP_CHECK(prepare_device(fit, dv, RP_PREC_FP64));
-    const int K = dv.K, KP = dv.KP, NP = dv.NP, N = fit->n_atoms;
+    // the matvec always evaluates the basis in fp64 (KP padded for the fp64 GEMM)
+    FitDevice *dvp = nullptr;
+    RP_CHECK(device_setup(fit, RP_PREC_FP64, dvp));
+    FitDevice &dv = *dvp;
+    const int K = dv.K, NP = dv.NP, N = fit->n_atoms;
+    const int KP = K > 0 ? (int)round_up(2 * K, 16) : 0;
     const int64_t M = fit->n_keep, rows = dv.rows;
-    DBuf d_G, d_B, d_phi, d_q, d_out;
+    DBuf &d_G = fit->ws[0], &d_B = fit->ws[1], &d_phi = fit->ws[2], &d_q = fit->ws[22], &d_out = fit->ws[23];
     if (KP > 0) {
-        RP_CHECK(d_G.alloc(ctx, sizeof(double) * rows * KP));
-        RP_CHECK(d_B.alloc(ctx, sizeof(double) * (size_t)KP * NP));
+        RP_CHECK(d_G.grow(ctx, sizeof(double) * rows * KP));
+        RP_CHECK(d_B.grow(ctx, sizeof(double) * (size_t)KP * NP));
         dim3 gb(ceil_div(NP, 256), KP / 2);
         k_atom_table_f64<<<gb, 256, 0, st>>>(K, dv.kvecs.as<double>(), N, fit->d_atoms.as<double>(), d_B.as<double>(), NP);
     }
-    RP_CHECK(d_phi.alloc(ctx, sizeof(double) * rows * NP));
-    RP_CHECK(d_q.alloc(ctx, sizeof(double) * N));
-    RP_CHECK(d_out.alloc(ctx, sizeof(double) * M));
+    RP_CHECK(d_phi.grow(ctx, sizeof(double) * rows * NP));
+    RP_CHECK(d_q.grow(ctx, sizeof(double) * N));
+    RP_CHECK(d_out.grow(ctx, sizeof(double) * M));
     RP_HIP(ctx, hipMemcpyAsync(d_q.p, q, sizeof(double) * N, hipMemcpyHostToDevice, st));
     for (int64_t r0 = 0; r0 < M; r0 += rows) {
         const int64_t valid = std::min<int64_t>(rows, M - r0);
@@ -345,11 +362,6 @@ int run_phi_matvec(rp_fit *fit, const double *q, double *v_coul) {
     RP_HIP(ctx, hipMemcpyAsync(v_coul, d_out.p, sizeof(double) * M, hipMemcpyDeviceToHost, st));
     RP_HIP(ctx, hipStreamSynchronize(st));
     return RP_OK;
-}
-
-int accumulate_split(rp_fit *fit, FitDevice &dv) {
-    (void)dv;
-    return fit->ctx->fail(RP_EINVAL, "split precision accumulation is not available in this build");
 }
 
 }  // namespace rp

@@ -38,18 +38,51 @@ struct EpiArgs {
     int N;
     int fit_type;
     const double *atoms;   // N x 3
+    const double *afrac;   // N x 3 fractional coordinates of the atoms
     const double *shifts;  // S x 3 (pruned)
     int S;
-    double R2, sqrt_alpha;
+    double R2, sqrt_alpha, r_cut;
+    double box[9];         // cell rows
+    double h[3];           // interplanar spacings V / |a_j x a_k|
+    double inv_n[3];       // 1 / n_grid
+
+    // Nearest image of atom a seen from grid point (i, j, k): shift index n0 = rint(g_f - a_f) and the
+    // lower bound min_i (1 - |w_i|) h_i on the distance to every other image (w = g_f - a_f - n0).
+    // Returns true when that bound exceeds Rc, i.e. only the shift n0 . box can be within the cutoff.
+    __device__ __forceinline__ bool nearest_image(int f, int a, double &sx, double &sy, double &sz) const {
+        int i, j, k;
+        geom.ijk(f, i, j, k);
+        const double df0 = (double)i * inv_n[0] - afrac[3 * a];
+        const double df1 = (double)j * inv_n[1] - afrac[3 * a + 1];
+        const double df2 = (double)k * inv_n[2] - afrac[3 * a + 2];
+        const double n0 = rint(df0), n1 = rint(df1), n2 = rint(df2);
+        const double b = fmin(fmin((1.0 - fabs(df0 - n0)) * h[0], (1.0 - fabs(df1 - n1)) * h[1]),
+                              (1.0 - fabs(df2 - n2)) * h[2]);
+        // shift vector with the rounding of the shift list: ((n0*b0 + n1*b1) + n2*b2)
+        sx = (n0 * box[0] + n1 * box[3]) + n2 * box[6];
+        sy = (n0 * box[1] + n1 * box[4]) + n2 * box[7];
+        sz = (n0 * box[2] + n1 * box[5]) + n2 * box[8];
+        return b > r_cut;
+    }
 
     // sum_s erfc(sqrt_alpha r)/r over r^2 <= Rc^2, zero shift skipped at r^2 <= 1e-20
     // (repeat.py:849-872); fit-type 0: 1/|d| for |d| > 1e-10 (repeat.py:991-994).
-    __device__ __forceinline__ double real_space(double gx, double gy, double gz, int a) const {
+    __device__ __forceinline__ double real_space(double gx, double gy, double gz, int a, int f = -1) const {
         const double ax = atoms[3 * a], ay = atoms[3 * a + 1], az = atoms[3 * a + 2];
         if (fit_type == 0) {
             const double dx = gx - ax, dy = gy - ay, dz = gz - az;
             const double d = sqrt((dx * dx + dy * dy) + dz * dz);
             return d > 1e-10 ? 1.0 / d : 0.0;
+        }
+        double sx, sy, sz;
+        if (f >= 0 && nearest_image(f, a, sx, sy, sz)) {
+            // only this image can be within Rc: the shift-list sum reduces to (at most) this one term
+            const double dx = gx - (ax + sx), dy = gy - (ay + sy), dz = gz - (az + sz);
+            const double r2 = (dx * dx + dy * dy) + dz * dz;
+            if (!(r2 <= R2)) return 0.0;
+            if (sx == 0.0 && sy == 0.0 && sz == 0.0 && r2 <= 1e-20) return 0.0;
+            const double r = sqrt(r2);
+            return erfc(sqrt_alpha * r) / r;
         }
         double sum = 0.0;
         for (int s = 0; s < S; ++s) {
@@ -69,6 +102,9 @@ struct EpiArgs {
 // Device copies of the Ewald setup and the chunking plan of one fit.
 struct FitDevice {
     int K = 0, KP = 0, NP = 0, S = 0;
+    bool ready = false;
+    std::vector<double> shifts_host;   // pruned shifts the device copy was built from
+    DBuf afrac;
     int64_t rows = 0;      // points per chunk (multiple of 128)
     int splits = 1;        // Gram row splits per chunk
     FitGeom geom{};
@@ -78,6 +114,8 @@ struct FitDevice {
 };
 
 int prepare_device(rp_fit *fit, FitDevice &dv, int precision);
+// Persistent device setup of a fit (built on first use, rebuilt when the pruned shifts change).
+int device_setup(rp_fit *fit, int precision, FitDevice *&dv);
 int accumulate_f64(rp_fit *fit, FitDevice &dv);
 int accumulate_split(rp_fit *fit, FitDevice &dv);
 __global__ void k_symmetrize(double *gram, int NP, int tile);

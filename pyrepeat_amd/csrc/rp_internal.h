@@ -18,6 +18,9 @@ void set_global_error(const std::string &msg);
 
 struct Ctx;
 
+// Kernel classes timed with HIP events when profiling is on (rp_ctx_profile).
+enum { KT_KSPACE = 0, KT_GRAM = 1, KT_FILTER = 2, KT_SOLVE = 3, KT_PHASE = 4, KT_SLICE = 5, KT_N = 6 };
+
 // Device allocation charged against the owning context's HBM budget.
 struct DBuf {
     void *p = nullptr;
@@ -28,6 +31,8 @@ struct DBuf {
     DBuf &operator=(const DBuf &) = delete;
     ~DBuf();
     int alloc(Ctx *c, size_t n);   // frees any previous allocation first
+    // Keeps the allocation when it is already large enough (contents are not preserved on growth).
+    int grow(Ctx *c, size_t n) { return (p && bytes >= n && ctx == c) ? 0 : alloc(c, n); }
     void release();
     template <class T> T *as() const { return static_cast<T *>(p); }
 };
@@ -39,10 +44,68 @@ struct Ctx {
     uint64_t in_use = 0;
     std::string err;
     hipDeviceProp_t prop{};
+    bool profile = false;
+    double kt_ms[KT_N] = {};
+    int64_t kt_n[KT_N] = {};
+    struct Pending {
+        int kind;
+        hipEvent_t a, b;
+    };
+    std::vector<Pending> pending;
     int fail(int code, const std::string &msg) {
         err = msg;
         return code;
     }
+    // Adds the elapsed times of recorded launches (call after the stream is synchronised).
+    void collect() {
+        for (auto &p : pending) {
+            float ms = 0.f;
+            if (hipEventElapsedTime(&ms, p.a, p.b) == hipSuccess) {
+                kt_ms[p.kind] += ms;
+                kt_n[p.kind] += 1;
+            }
+            (void)hipEventDestroy(p.a);
+            (void)hipEventDestroy(p.b);
+        }
+        pending.clear();
+    }
+};
+
+// Scoped event pair around kernel launches on ctx->stream (no-op unless profiling).
+struct KTimer {
+    Ctx *c;
+    int kind;
+    hipEvent_t a = nullptr, b = nullptr;
+    KTimer(Ctx *ctx, int k) : c(ctx), kind(k) {
+        if (c->profile && hipEventCreate(&a) == hipSuccess && hipEventCreate(&b) == hipSuccess)
+            (void)hipEventRecord(a, c->stream);
+        else
+            a = b = nullptr;
+    }
+    ~KTimer() {
+        if (a && b) {
+            (void)hipEventRecord(b, c->stream);
+            c->pending.push_back({kind, a, b});
+        }
+    }
+};
+
+// Device-resident inputs of the vdW filter (kept so a fit can be re-run from HBM).
+struct FilterArgs {
+    int nx, ny, nz;
+    double ax[9];
+    double bmin[3];
+    double binv;
+    int bn[3];
+    int fallback;          // 1 if some r_max exceeds the bin edge
+    int64_t n_all;
+};
+
+struct FilterState {
+    FilterArgs fa{};
+    int64_t n_blocks = 0;
+    int esp_dtype = 0, esp_order = 0;
+    DBuf bin_start, bin_atoms, bin_smax, all, all_smax, cnt, off, vsum, mean, esp, axis, bounds;
 };
 
 // Checks a HIP call; records the error on ctx and returns RP_EHIP from the caller.
@@ -97,6 +160,10 @@ std::vector<double> prune_shifts(const std::vector<double> &shifts, const double
 
 }  // namespace rp
 
+namespace rp {
+struct FitDevice;
+}
+
 // The fit object behind the opaque rp_fit handle.
 struct rp_fit {
     rp::Ctx *ctx = nullptr;
@@ -110,22 +177,31 @@ struct rp_fit {
     double v_mean = 0.0;
     std::vector<double> shifts_pruned;
     bool accumulated = false;
+    bool lu_fallback = false;     // last solve used LU (A not positive definite)
+    int64_t kspace_products = 0;  // split form: int8 MFMA k-steps x slice products per row/col tile
     double t_filter_ms = 0, t_accum_ms = 0, t_solve_ms = 0;
 
     // device state
+    rp::FilterState filt;
     rp::DBuf d_mask;      // n_total bytes
     rp::DBuf d_idx;       // n_keep int32 flat indices
     rp::DBuf d_v;         // n_keep doubles (mean subtracted)
     rp::DBuf d_atoms;     // N x 3 doubles
     rp::DBuf d_gram;      // NA x NA doubles, NA = N + 2 (augmented with V and 1)
     int32_t na = 0;       // augmented width N + 2
+    // grow-only workspaces reused by every accumulate / solve / matvec of this fit
+    rp::DBuf ws[24];
+    rp::FitDevice *dev = nullptr;   // persistent device copy of the Ewald setup
+    ~rp_fit();
 };
 
 struct rp_ctx : rp::Ctx {};
 
 namespace rp {
+struct FitDevice;
 // Phases implemented in the .hip files.
-int run_filter(rp_fit *fit, const rp_system *sys);
+int upload_filter(rp_fit *fit, const rp_system *sys);
+int run_filter(rp_fit *fit);
 int run_accumulate(rp_fit *fit);
 int run_solve(rp_fit *fit, const rp_constraints *con, double *q_out, double *solution_out, int32_t *used_fallback);
 int run_phi_matvec(rp_fit *fit, const double *q, double *v_coul);

@@ -209,3 +209,53 @@ def test_deterministic_bitwise(gpu_ctx):
         qs.append(calc.fit_charges())
         calc.close()
     np.testing.assert_array_equal(qs[0], qs[1])
+
+
+# ---------------------------------------------------------------- split precision (int8 Ozaki) path
+
+@pytest.mark.parametrize("name", ["water", "crystal200"])
+def test_fit_split_matches_golden(gpu_ctx, tmp_path, name):
+    g = _golden(name)
+    s = getattr(synth, name)()
+    print(f"fit split {name}", flush=True)
+    calc = _calc(gpu_ctx, s, "split", tmp_path)
+    q = calc.fit_charges()
+    dq = np.max(np.abs(q - np.array(g["charges"])))
+    print(f"  max|dq| = {dq:.3e}", flush=True)
+    assert dq <= 1e-5
+    assert abs(q.sum() - s.q_tot) < 1e-9
+    calc.close()
+
+
+def test_split_normal_equations_close_to_fp64(gpu_ctx):
+    s = synth.tiny(seed=3, n_atoms=6, n=16, L=10.0, triclinic=True)
+    out = []
+    for prec in ("fp64", "split"):
+        f = _fit(gpu_ctx, s, precision=prec)
+        f.accumulate()
+        out.append(f.normal_equations())
+        f.close()
+    (a64, b64, s64, v64), (a8, b8, s8, v8) = out
+    np.testing.assert_allclose(a8, a64, rtol=2e-6, atol=1e-9 * np.abs(a64).max())
+    np.testing.assert_allclose(s8, s64, rtol=2e-6)
+    # sum(V) of the sliced V column: 4 slices hold V to 2^-29 of max|V| per point
+    xyz, v, _ = _fit(gpu_ctx, s).grid()
+    assert abs(v8 - v64) <= len(v) * np.abs(v).max() * 2.0 ** -28
+
+
+def test_split_deterministic_and_refit(gpu_ctx, tmp_path):
+    s = synth.crystal200()
+    calc = _calc(gpu_ctx, s, "split", tmp_path)
+    q1 = calc.fit_charges()
+    q2 = calc._fit.refit(calc.constraints())
+    np.testing.assert_array_equal(q1, q2)
+    calc.close()
+
+
+def test_refit_fp64_equals_fit_charges(gpu_ctx, tmp_path):
+    s = synth.water()
+    calc = _calc(gpu_ctx, s, "fp64", tmp_path)
+    q1 = calc.fit_charges()
+    q2 = calc._fit.refit(calc.constraints())
+    np.testing.assert_array_equal(q1, q2)
+    calc.close()
