@@ -32,12 +32,13 @@ namespace {
 
 constexpr int PMAX = 4;
 constexpr int NG = 2 * PMAX - 1;   // groups s = p+q, p,q in [0, P)
-constexpr int TM = 64, TN = 128, TK = 32;
+constexpr int TM = 64, TN = 128, TK = 64;
+constexpr int TKP = TK + 16;   // padded LDS row (bytes)
 constexpr int NTHR = 512;
 
 struct OzBlock {
-    int k0, k1, P;
-    double scale;   // S_A * S_B for the block (k-space); 1 for the Gram
+    int k0, k1, P, PB;   // slices of A and of B used in this block
+    double scale;        // S_A * S_B for the block (k-space); 1 for the Gram
 };
 
 struct OzOperands {
@@ -66,10 +67,13 @@ __device__ __forceinline__ void slice_value(double x, double inv_scale, int P, i
 
 // Main loop: accumulates the block tile (rows m0.., cols n0..) of sum_b scale_b sum_s 2^{-7(s+2)} I_s into facc.
 // K ranges of the blocks are multiples of TK; rows/cols are in range by construction.
+// LDS rows are padded to TKP bytes so the 16-byte fragment reads of 16 consecutive lanes
+// fall on distinct banks.  Each stage holds TK = 64 k (two MFMA k-steps); the next stage is
+// loaded into registers while the current one is consumed.
 __device__ __forceinline__ void oz_mainloop(const OzOperands &op, int64_t m0, int64_t n0, double (&facc)[16],
                                             int8_t *lds) {
-    int8_t(*As)[PMAX][TM][TK] = reinterpret_cast<int8_t(*)[PMAX][TM][TK]>(lds);
-    int8_t(*Bs)[PMAX][TN][TK] = reinterpret_cast<int8_t(*)[PMAX][TN][TK]>(lds + 2 * PMAX * TM * TK);
+    int8_t(*As)[PMAX][TM][TKP] = reinterpret_cast<int8_t(*)[PMAX][TM][TKP]>(lds);
+    int8_t(*Bs)[PMAX][TN][TKP] = reinterpret_cast<int8_t(*)[PMAX][TN][TKP]>(lds + 2 * PMAX * TM * TKP);
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
     const int wm = wave >> 2, wn = wave & 3;
 #pragma unroll
@@ -77,48 +81,49 @@ __device__ __forceinline__ void oz_mainloop(const OzOperands &op, int64_t m0, in
 
     for (int b = 0; b < op.nblocks; ++b) {
         const OzBlock blk = op.blk[b];
-        const int P = blk.P;
+        const int P = blk.P, PB = blk.PB;
         ivec16 acc[NG];
 #pragma unroll
         for (int s = 0; s < NG; ++s)
 #pragma unroll
             for (int r = 0; r < 16; ++r) acc[s][r] = 0;
         const int nk = (blk.k1 - blk.k0) / TK;
-        // staging: A chunks c in [0, 128P): slice c/128, row (c%128)/2, half c%2; B chunks c in [0, 256P)
-        ivec4 ra[1], rb[2];
+        // staging, 16-byte chunks: A c in [0, 256P): slice c/256, row (c%256)/4, quarter c%4;
+        //                          B c in [0, 512PB): slice c/512, row (c%512)/4, quarter c%4
+        ivec4 ra[2], rb[4];
         auto gload = [&](int kbase) {
-#pragma unroll
-            for (int u = 0; u < 1; ++u) {
-                const int c = t + NTHR * u;
-                if (c < 128 * P) {
-                    const int p = c >> 7, row = (c & 127) >> 1, half = c & 1;
-                    ra[u] = *reinterpret_cast<const ivec4 *>(op.A + p * op.a_plane + (m0 + row) * op.lda + kbase + 16 * half);
-                }
-            }
 #pragma unroll
             for (int u = 0; u < 2; ++u) {
                 const int c = t + NTHR * u;
                 if (c < 256 * P) {
-                    const int p = c >> 8, col = (c & 255) >> 1, half = c & 1;
-                    rb[u] = *reinterpret_cast<const ivec4 *>(op.B + p * op.b_plane + (n0 + col) * op.ldb + kbase + 16 * half);
+                    const int p = c >> 8, row = (c & 255) >> 2, qq = c & 3;
+                    ra[u] = *reinterpret_cast<const ivec4 *>(op.A + p * op.a_plane + (m0 + row) * op.lda + kbase + 16 * qq);
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int c = t + NTHR * u;
+                if (c < 512 * PB) {
+                    const int p = c >> 9, row = (c & 511) >> 2, qq = c & 3;
+                    rb[u] = *reinterpret_cast<const ivec4 *>(op.B + p * op.b_plane + (n0 + row) * op.ldb + kbase + 16 * qq);
                 }
             }
         };
         auto lstore = [&](int buf) {
 #pragma unroll
-            for (int u = 0; u < 1; ++u) {
-                const int c = t + NTHR * u;
-                if (c < 128 * P) {
-                    const int p = c >> 7, row = (c & 127) >> 1, half = c & 1;
-                    *reinterpret_cast<ivec4 *>(&As[buf][p][row][16 * half]) = ra[u];
-                }
-            }
-#pragma unroll
             for (int u = 0; u < 2; ++u) {
                 const int c = t + NTHR * u;
                 if (c < 256 * P) {
-                    const int p = c >> 8, col = (c & 255) >> 1, half = c & 1;
-                    *reinterpret_cast<ivec4 *>(&Bs[buf][p][col][16 * half]) = rb[u];
+                    const int p = c >> 8, row = (c & 255) >> 2, qq = c & 3;
+                    *reinterpret_cast<ivec4 *>(&As[buf][p][row][16 * qq]) = ra[u];
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int c = t + NTHR * u;
+                if (c < 512 * PB) {
+                    const int p = c >> 9, row = (c & 511) >> 2, qq = c & 3;
+                    *reinterpret_cast<ivec4 *>(&Bs[buf][p][row][16 * qq]) = rb[u];
                 }
             }
         };
@@ -130,20 +135,21 @@ __device__ __forceinline__ void oz_mainloop(const OzOperands &op, int64_t m0, in
                 const int cur = kt & 1;
                 const bool more = kt + 1 < nk;
                 if (more) gload(blk.k0 + (kt + 1) * TK);
-                ivec4 af[PMAX], bf[PMAX];
 #pragma unroll
-                for (int p = 0; p < PMAX; ++p) {
-                    if (p < P) {
-                        af[p] = *reinterpret_cast<const ivec4 *>(&As[cur][p][wm * 32 + (lane & 31)][16 * (lane >> 5)]);
-                        bf[p] = *reinterpret_cast<const ivec4 *>(&Bs[cur][p][wn * 32 + (lane & 31)][16 * (lane >> 5)]);
+                for (int h = 0; h < TK / 32; ++h) {
+                    ivec4 af[PMAX], bf[PMAX];
+#pragma unroll
+                    for (int p = 0; p < PMAX; ++p) {
+                        if (p < P) af[p] = *reinterpret_cast<const ivec4 *>(&As[cur][p][wm * 32 + (lane & 31)][32 * h + 16 * (lane >> 5)]);
+                        if (p < PB) bf[p] = *reinterpret_cast<const ivec4 *>(&Bs[cur][p][wn * 32 + (lane & 31)][32 * h + 16 * (lane >> 5)]);
                     }
+#pragma unroll
+                    for (int p = 0; p < PMAX; ++p)
+#pragma unroll
+                        for (int q = 0; q < PMAX; ++q) {
+                            if (p < P && q < PB) acc[p + q] = __builtin_amdgcn_mfma_i32_32x32x32_i8(af[p], bf[q], acc[p + q], 0, 0, 0);
+                        }
                 }
-#pragma unroll
-                for (int p = 0; p < PMAX; ++p)
-#pragma unroll
-                    for (int q = 0; q < PMAX; ++q) {
-                        if (p < P && q < P) acc[p + q] = __builtin_amdgcn_mfma_i32_32x32x32_i8(af[p], bf[q], acc[p + q], 0, 0, 0);
-                    }
                 if (more) lstore(cur ^ 1);
                 __syncthreads();
             }
@@ -273,10 +279,16 @@ __device__ __forceinline__ double real_space_f32(const EpiArgs &e, double gx, do
     return (double)sum - (double)comp;
 }
 
-__global__ __launch_bounds__(NTHR) void k_oz_kspace(OzOperands op, KspaceEpi ke) {
+__global__ __launch_bounds__(NTHR) void k_oz_kspace(OzOperands op, KspaceEpi ke, int n_col_tiles) {
     extern __shared__ int8_t lds[];
-    const int64_t m0 = (int64_t)blockIdx.y * TM;
-    const int64_t n0 = (int64_t)blockIdx.x * TN;
+    // Blocks are dispatched round-robin over the 8 XCDs.  With the column tile fastest, an XCD
+    // keeps seeing the same column tiles (all of them when there are 8), so its L2 holds their
+    // atom-side slices while the grid-side slices stream through.
+    const int L = blockIdx.x;
+    const int col_tile = L % n_col_tiles;
+    const int row_tile = L / n_col_tiles;
+    const int64_t m0 = (int64_t)row_tile * TM;
+    const int64_t n0 = (int64_t)col_tile * TN;
     double facc[16];
     oz_mainloop(op, m0, n0, facc, lds);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -325,6 +337,7 @@ __global__ __launch_bounds__(NTHR) void k_oz_gram(OzOperands op, GramEpi ge, int
     o.blk[0].k0 = blockIdx.y * rows_per_split;
     o.blk[0].k1 = o.blk[0].k0 + rows_per_split;
     o.blk[0].P = PMAX;
+    o.blk[0].PB = PMAX;
     o.blk[0].scale = 1.0;
     double facc[16];
     oz_mainloop(o, m0, n0, facc, lds);
@@ -361,9 +374,11 @@ __global__ void k_zero_u64(unsigned long long *p, int n) {
 // 16 (32 G columns, one MFMA k-step).  Every G entry must be represented to the
 // absolute precision of the uniform 4-slice form, S_max 2^-28 (S_max = scale of the
 // largest weight): a block whose weights are below S_max by 2^-e needs only
-// P = ceil((28 - e) / 7) slices of G and of B (B's slices are nested, so a block uses
-// the first P of B's 4).  Consecutive blocks with the same P form one range with its
-// own power-of-two scale; group sums stay exact in int32 over a range.
+// P = ceil((28 - e) / 7) slices of G.  B (the atom phases, shared by every grid point)
+// always keeps all 4 slices: truncating it per block perturbs each atom's basis function
+// systematically, which the ill-conditioned normal equations amplify (measured 6.7e-6 e
+// against 1.3e-6 e with 4 B slices on the 1000-atom cell).  Consecutive blocks with the
+// same P form one range with its own power-of-two scale; group sums stay exact in int32.
 static double pow2_scale(double vmax) {
     if (!(vmax > 0.0)) return 1.0;
     double S = std::ldexp(1.0, (int)std::ceil(std::log2(vmax * (128.0 / 127.0))));
@@ -388,7 +403,7 @@ static void plan_kspace(const Setup &s, int KP, KPlan &pl) {
     std::stable_sort(pl.order.begin(), pl.order.end(),
                      [&](int a, int b) { return std::fabs(s.kcoefs[a]) > std::fabs(s.kcoefs[b]); });
     const double smax = pow2_scale(K ? std::fabs(s.kcoefs[pl.order[0]]) : 0.0);
-    const int kpb = TK / 2;  // k-vectors per block
+    const int kpb = TK / 2;  // k-vectors per block (two G columns each)
     const int nblk = KP / TK;
     std::vector<int> pblk(nblk, 1);
     for (int b = 0; b < nblk; ++b) {
@@ -413,9 +428,9 @@ static void plan_kspace(const Setup &s, int KP, KPlan &pl) {
         while (e < nblk && pblk[e] == pblk[b]) ++e;
         const int k0 = b * kpb, k1 = std::min(K, e * kpb);
         const double S = k0 < K ? pow2_scale(std::fabs(s.kcoefs[pl.order[k0]])) : 1.0;
-        OzBlock r{b * TK, e * TK, pblk[b], S * 2.0};
+        OzBlock r{b * TK, e * TK, pblk[b], PMAX, S * 2.0};
         pl.ranges.push_back(r);
-        pl.products += (int64_t)(e - b) * pblk[b] * pblk[b];
+        pl.products += (int64_t)(e - b) * pblk[b] * PMAX;
         for (int p = k0; p < k1; ++p) {
             pl.kslices[p] = pblk[b];
             pl.wscaled[p] = s.kcoefs[pl.order[p]] / S;
@@ -435,10 +450,10 @@ int accumulate_split(rp_fit *fit, FitDevice &dv) {
     const int K = dv.K, NP = dv.NP, N = fit->n_atoms;
     const int KP = K > 0 ? (int)round_up(2 * K, TK) : 0;
     const int64_t M = fit->n_keep;
-    int64_t rows = std::min<int64_t>(65536, round_up(M, 128));
-    rows = std::max<int64_t>(128, round_up(rows, 128));
+    int64_t rows = std::min<int64_t>(65536, round_up(M, 512));
+    rows = std::max<int64_t>(512, round_up(rows, 512));
     // bound the chunk buffers (G slices, Phi^T, Phi slices) to about 2 GiB
-    while (rows > 128 && rows * ((int64_t)PMAX * KP + 12LL * NP) > ((int64_t)1 << 31)) rows /= 2;
+    while (rows > 512 && rows * ((int64_t)PMAX * KP + 12LL * NP) > ((int64_t)1 << 31)) rows /= 2;
     // operands
     DBuf &d_G = fit->ws[0], &d_B = fit->ws[1], &d_phiT = fit->ws[2], &d_part = fit->ws[3], &d_slices = fit->ws[4],
          &d_colmax = fit->ws[5], &d_scale = fit->ws[6], &d_wsc = fit->ws[7], &d_ksl = fit->ws[8], &d_tiles = fit->ws[9],
@@ -479,7 +494,11 @@ int accumulate_split(rp_fit *fit, FitDevice &dv) {
     RP_CHECK(d_part.grow(ctx, sizeof(double) * (size_t)splits * n_tiles * TM * TN));
     RP_HIP(ctx, hipMemsetAsync(d_part.p, 0, sizeof(double) * (size_t)splits * n_tiles * TM * TN, st));
 
-    const size_t lds_bytes = 2 * PMAX * (TM + TN) * TK;  // 48 KiB
+    const size_t lds_bytes = 2 * PMAX * (TM + TN) * TKP;  // 120 KiB
+    RP_HIP(ctx, hipFuncSetAttribute(reinterpret_cast<const void *>(&k_oz_kspace),
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes));
+    RP_HIP(ctx, hipFuncSetAttribute(reinterpret_cast<const void *>(&k_oz_gram),
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes));
     OzOperands ko{};
     ko.A = d_G.as<int8_t>();
     ko.a_plane = rows * KP;
@@ -514,10 +533,12 @@ int accumulate_split(rp_fit *fit, FitDevice &dv) {
         ke.e.rows_valid = valid;
         OzOperands kop = ko;
         if (K == 0) kop.nblocks = 0;
-        dim3 gk(NP / TN, rows / TM);
+        // rows is a multiple of 128 = 2 * TM, so the row-tile count is even; pad to a multiple of 8
+        const int row_tiles = (int)(rows / TM), col_tiles = NP / TN;
+        if (row_tiles % 8 != 0) return ctx->fail(RP_EINVAL, "chunk rows must be a multiple of 512");
         {
             KTimer kt(ctx, KT_KSPACE);
-            k_oz_kspace<<<gk, NTHR, lds_bytes, st>>>(kop, ke);
+            k_oz_kspace<<<row_tiles * col_tiles, NTHR, lds_bytes, st>>>(kop, ke, col_tiles);
         }
         {
             KTimer kts(ctx, KT_SLICE);

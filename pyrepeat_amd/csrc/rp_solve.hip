@@ -219,44 +219,55 @@ __global__ __launch_bounds__(256) void k_update(double *__restrict__ A, int ld, 
 
 constexpr int CB = 64;
 
+// Uniform broadcast of lane l's double (l a compile-time constant after unrolling: v_readlane).
+__device__ __forceinline__ double readlane_d(double x, int l) {
+    const long long v = __double_as_longlong(x);
+    const int lo = __builtin_amdgcn_readlane((int)(v & 0xffffffffll), l);
+    const int hi = __builtin_amdgcn_readlane((int)(v >> 32), l);
+    return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+
+// Cholesky of one 64x64 diagonal block by a single wave, lane t holding row t in registers.
 // A pivot at or below 1e-13 of the original diagonal entry counts as a failure: the system is then
 // left to the pivoted LU, which reports exact singularity as the reference's LAPACK call does.
 __global__ __launch_bounds__(64) void k_potrf_diag(double *__restrict__ A, int ld, int n, int k0,
                                                    const double *__restrict__ orig, int *__restrict__ info) {
-    // one wave: its barriers are cheap and the whole 64x64 block sits in LDS
-    __shared__ double L[CB][CB + 1];
     const int nb = min(CB, n - k0);
     const int t = threadIdx.x;
-    for (int e = t; e < nb * nb; e += 64) {
-        const int i = e / nb, j = e % nb;
-        L[i][j] = j <= i ? A[(int64_t)(k0 + i) * ld + k0 + j] : 0.0;
-    }
-    __syncthreads();
-    for (int j = 0; j < nb; ++j) {
-        if (t == 0) {
-            const double d = L[j][j];
-            if (!(d > 1e-13 * fabs(orig[(int64_t)(k0 + j) * ld + k0 + j]))) {
-                if (info[0] == 0) info[0] = k0 + j + 1;
-                L[j][j] = 1.0;
+    double r[CB];
+#pragma unroll
+    for (int c = 0; c < CB; ++c) r[c] = (t < nb && c <= t) ? A[(int64_t)(k0 + t) * ld + k0 + c] : 0.0;
+    int bad = 0;
+#pragma unroll
+    for (int j = 0; j < CB; ++j) {
+        if (j < nb) {
+            const double d = readlane_d(r[j], j);
+            const double od = orig[(int64_t)(k0 + j) * ld + k0 + j];
+            double piv;
+            if (!(d > 1e-13 * fabs(od))) {
+                if (!bad) bad = k0 + j + 1;
+                piv = 1.0;
             } else {
-                L[j][j] = sqrt(d);
+                piv = sqrt(d);
+            }
+            if (t == j)
+                r[j] = piv;
+            else if (t > j)
+                r[j] = r[j] / piv;
+            const double ltj = r[j];
+#pragma unroll
+            for (int c = j + 1; c < CB; ++c) {
+                if (c < nb) {
+                    const double lcj = readlane_d(r[j], c);
+                    if (t >= c) r[c] -= ltj * lcj;
+                }
             }
         }
-        __syncthreads();
-        const double piv = L[j][j];
-        if (t > j && t < nb) L[t][j] /= piv;
-        __syncthreads();
-        // lane t owns row t: L[t][c] -= L[t][j] L[c][j] for j < c <= t
-        if (t > j && t < nb) {
-            const double ltj = L[t][j];
-            for (int c = j + 1; c <= t; ++c) L[t][c] -= ltj * L[c][j];
-        }
-        __syncthreads();
     }
-    for (int e = t; e < nb * nb; e += 64) {
-        const int i = e / nb, j = e % nb;
-        if (j <= i) A[(int64_t)(k0 + i) * ld + k0 + j] = L[i][j];
-    }
+#pragma unroll
+    for (int c = 0; c < CB; ++c)
+        if (t < nb && c <= t) A[(int64_t)(k0 + t) * ld + k0 + c] = r[c];
+    if (t == 0 && bad && info[0] == 0) info[0] = bad;
 }
 
 // L21 = A21 L11^{-T}: one thread per row below the block.
@@ -347,8 +358,8 @@ __global__ __launch_bounds__(256) void k_chol_solve(const double *__restrict__ A
 #pragma unroll
             for (int j = 0; j < CB; ++j) {
                 if (j < nb) {
-                    const double d = __shfl(lr[j], j, 64);
-                    const double xw = __shfl(zw, j, 64) / d, xb = __shfl(zb, j, 64) / d;
+                    const double d = readlane_d(lr[j], j);
+                    const double xw = readlane_d(zw, j) / d, xb = readlane_d(zb, j) / d;
                     if (lane == j) {
                         zw = xw;
                         zb = xb;
@@ -389,8 +400,8 @@ __global__ __launch_bounds__(256) void k_chol_solve(const double *__restrict__ A
             for (int jj = 0; jj < CB; ++jj) {
                 const int j = CB - 1 - jj;
                 if (j < nb) {
-                    const double d = __shfl(lc[j], j, 64);
-                    const double xw = __shfl(zw, j, 64) / d, xb = __shfl(zb, j, 64) / d;
+                    const double d = readlane_d(lc[j], j);
+                    const double xw = readlane_d(zw, j) / d, xb = readlane_d(zb, j) / d;
                     if (lane == j) {
                         zw = xw;
                         zb = xb;
