@@ -108,6 +108,7 @@ typedef struct rp_constraints {
 
 /* ---------------------------------------------------------------- context */
 int rp_ctx_create(int device, uint64_t hbm_budget_bytes, rp_ctx **out);
+/* Destroys the context; with fits still alive on it, the last rp_fit_destroy completes it. */
 int rp_ctx_destroy(rp_ctx *ctx);
 const char *rp_last_error(const rp_ctx *ctx);
 int rp_ctx_hbm_in_use(const rp_ctx *ctx, uint64_t *bytes);
@@ -179,6 +180,69 @@ int rp_fit_phi(rp_fit *fit, int64_t m, const double *grid_xyz, double *phi_out);
 /* Device time (ms) of the phases of the last accumulate / solve on this fit:
  * t[0] filter, t[1] accumulate, t[2] solve. */
 int rp_fit_timings(const rp_fit *fit, double *t3);
+
+/*
+ * Reload a fit with another system and options (ChargeCalculator.__init__ on a new
+ * cube, repeat.py:1736-1786), keeping its device buffers: they only grow, so a
+ * fit object can serve a stream of systems without device allocation once it
+ * has seen the largest.  Runs the vdW filter like rp_fit_create.
+ */
+int rp_fit_reset(rp_fit *fit, const rp_system *sys, const rp_options *opt);
+
+/* ---------------------------------------------------------------- serving
+ * The reference fits one cube per process (repeat.py:2636-2789).  The serving
+ * loop runs many such fits concurrently on one device: submitted requests wait
+ * in arrival order and are admitted to a slot as soon as a slot is idle and the
+ * HBM budget holds the request's device bound.  Each slot owns a worker thread,
+ * a HIP stream and a grow-only fit object, so admitted fits overlap on the
+ * device.  A request's charges are bitwise the same as those of rp_fit_create +
+ * rp_fit_accumulate + rp_fit_solve on the same inputs.
+ */
+#define RP_EAGAIN -8      /* rp_poll: the request has not finished yet */
+
+typedef struct rp_server rp_server;
+
+/* One fit request: the system, the options and the constraints of the solve.
+ * rp_submit copies everything it points to, so the caller may free it on return. */
+typedef struct rp_request {
+    rp_system system;
+    rp_options options;
+    rp_constraints constraints;
+} rp_request;
+
+typedef struct rp_result {
+    int32_t status;              /* RP_OK or the fit's error code */
+    int32_t slot;                /* slot that ran it (-1 if never admitted) */
+    int32_t n_atoms;
+    int32_t reserved_;
+    int64_t n_keep;
+    double t_submit, t_admit, t_done;  /* seconds on the server clock (rp_server_now) */
+    double device_ms[3];         /* filter, accumulate, solve (HIP events) */
+    uint64_t bytes_bound;        /* device bytes the admission reserved for it */
+    char error[256];
+} rp_result;
+
+typedef struct rp_server_stats {
+    int64_t submitted, completed, failed;
+    int32_t queued, running, max_running, n_slots;
+    uint64_t budget, reserved, peak_reserved;  /* bytes */
+} rp_server_stats;
+
+/* n_slots concurrent fits (1..16) on `device` within hbm_budget_bytes of device memory. */
+int rp_server_create(int device, int n_slots, uint64_t hbm_budget_bytes, rp_server **out);
+/* Finishes every submitted request, then stops the workers and frees the device memory. */
+int rp_server_destroy(rp_server *srv);
+const char *rp_server_error(const rp_server *srv);
+double rp_server_now(const rp_server *srv);
+/* Device bytes the admission would reserve for this request (an upper bound of what its fit allocates). */
+int rp_request_bytes(const rp_request *req, uint64_t *bytes);
+/* Queues a request; RP_EBUDGET if its bound exceeds the whole budget. */
+int rp_submit(rp_server *srv, const rp_request *req, uint64_t *ticket);
+/* Waits up to wait_ms (negative: no limit) for a request.  On completion fills *res and q_out
+ * (n_atoms charges; may be NULL), releases the ticket and returns RP_OK; RP_EAGAIN if it is still
+ * queued or running; RP_EINVAL for an unknown ticket. */
+int rp_poll(rp_server *srv, uint64_t ticket, int32_t wait_ms, rp_result *res, double *q_out);
+int rp_server_stats_get(const rp_server *srv, rp_server_stats *out);
 
 #ifdef __cplusplus
 }

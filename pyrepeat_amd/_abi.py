@@ -7,6 +7,7 @@ from __future__ import annotations
 
 import ctypes
 import os
+import weakref
 
 import numpy as np
 
@@ -21,6 +22,7 @@ RP_ESINGULAR = -4
 RP_EBUDGET = -5
 RP_ESTATE = -6
 RP_EEMPTY = -7
+RP_EAGAIN = -8
 
 RP_F64, RP_F32 = 0, 1
 RP_ORDER_CUBE, RP_ORDER_FLAT = 0, 1
@@ -31,6 +33,8 @@ EXPORTED = (
     "rp_fit_create", "rp_fit_destroy", "rp_fit_info_get", "rp_fit_grid", "rp_fit_ewald",
     "rp_fit_accumulate", "rp_fit_normal_equations", "rp_fit_solve", "rp_fit_phi_matvec",
     "rp_fit_phi", "rp_fit_timings", "rp_ctx_profile", "rp_ctx_kernel_stats", "rp_fit_refit",
+    "rp_fit_reset", "rp_server_create", "rp_server_destroy", "rp_server_error", "rp_server_now",
+    "rp_request_bytes", "rp_submit", "rp_poll", "rp_server_stats_get",
 )
 KERNEL_KINDS = {"kspace": 0, "gram": 1, "filter": 2, "solve": 3, "phase": 4, "slice": 5}
 
@@ -71,6 +75,24 @@ class RpConstraints(ctypes.Structure):
                 ("expand_src", ctypes.POINTER(ctypes.c_int32))]
 
 
+class RpRequest(ctypes.Structure):
+    _fields_ = [("system", RpSystem), ("options", RpOptions), ("constraints", RpConstraints)]
+
+
+class RpResult(ctypes.Structure):
+    _fields_ = [("status", ctypes.c_int32), ("slot", ctypes.c_int32), ("n_atoms", ctypes.c_int32),
+                ("reserved_", ctypes.c_int32), ("n_keep", ctypes.c_int64), ("t_submit", ctypes.c_double),
+                ("t_admit", ctypes.c_double), ("t_done", ctypes.c_double), ("device_ms", ctypes.c_double * 3),
+                ("bytes_bound", ctypes.c_uint64), ("error", ctypes.c_char * 256)]
+
+
+class RpServerStats(ctypes.Structure):
+    _fields_ = [("submitted", ctypes.c_int64), ("completed", ctypes.c_int64), ("failed", ctypes.c_int64),
+                ("queued", ctypes.c_int32), ("running", ctypes.c_int32), ("max_running", ctypes.c_int32),
+                ("n_slots", ctypes.c_int32), ("budget", ctypes.c_uint64), ("reserved", ctypes.c_uint64),
+                ("peak_reserved", ctypes.c_uint64)]
+
+
 _lib = None
 
 
@@ -105,8 +127,19 @@ def load(path: str | None = None):
     L.rp_ctx_profile.argtypes = [vp, ctypes.c_int]
     L.rp_ctx_kernel_stats.argtypes = [vp, ctypes.c_int, dp, i64p]
     L.rp_fit_refit.argtypes = [vp, ctypes.POINTER(RpConstraints), dp]
+    L.rp_fit_reset.argtypes = [vp, ctypes.POINTER(RpSystem), ctypes.POINTER(RpOptions)]
+    L.rp_server_create.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_uint64, ctypes.POINTER(vp)]
+    L.rp_server_destroy.argtypes = [vp]
+    L.rp_server_error.argtypes = [vp]
+    L.rp_server_error.restype = ctypes.c_char_p
+    L.rp_server_now.argtypes = [vp]
+    L.rp_server_now.restype = ctypes.c_double
+    L.rp_request_bytes.argtypes = [ctypes.POINTER(RpRequest), ctypes.POINTER(ctypes.c_uint64)]
+    L.rp_submit.argtypes = [vp, ctypes.POINTER(RpRequest), ctypes.POINTER(ctypes.c_uint64)]
+    L.rp_poll.argtypes = [vp, ctypes.c_uint64, ctypes.c_int32, ctypes.POINTER(RpResult), dp]
+    L.rp_server_stats_get.argtypes = [vp, ctypes.POINTER(RpServerStats)]
     for name in EXPORTED:
-        if getattr(L, name).restype is None and name != "rp_last_error":
+        if getattr(L, name).restype is None and name not in ("rp_last_error", "rp_server_error", "rp_server_now"):
             getattr(L, name).restype = ctypes.c_int
     _lib = L
     return L
@@ -133,6 +166,7 @@ class Context:
         check(L.rp_ctx_create(int(device), int(hbm_budget), ctypes.byref(h)))
         self.handle = h
         self.device = device
+        self._fits = weakref.WeakSet()
 
     def hbm_in_use(self):
         v = ctypes.c_uint64()
@@ -152,7 +186,10 @@ class Context:
         return out
 
     def close(self):
+        """Destroys the context after its fits (the library defers it while any fit is alive)."""
         if getattr(self, "handle", None):
+            for f in list(getattr(self, "_fits", ())):
+                f.close()
             load().rp_ctx_destroy(self.handle)
             self.handle = None
 
@@ -173,6 +210,32 @@ def default_context():
     return _default_ctx
 
 
+def make_system(n_grid, axis, atomic_numbers, atom_xyz, esp, esp_order=RP_ORDER_CUBE):
+    """An ``rp_system`` over arrays; returns ``(struct, arrays)`` (keep ``arrays`` alive while it is used)."""
+    Z = np.ascontiguousarray(atomic_numbers, dtype=np.int32)
+    xyz = np.ascontiguousarray(atom_xyz, dtype=np.float64).reshape(-1, 3)
+    esp = np.ascontiguousarray(esp)
+    if esp.dtype not in (np.float32, np.float64):
+        esp = esp.astype(np.float64)
+    sysd = RpSystem()
+    for i in range(3):
+        sysd.n_grid[i] = int(n_grid[i])
+    ax = np.asarray(axis, dtype=np.float64).ravel()
+    for i in range(9):
+        sysd.axis[i] = float(ax[i])
+    sysd.n_atoms = len(Z)
+    sysd.atomic_number = ptr(Z, ctypes.c_int32)
+    sysd.atom_xyz = ptr(xyz)
+    sysd.esp = esp.ctypes.data_as(ctypes.c_void_p)
+    sysd.esp_dtype = RP_F32 if esp.dtype == np.float32 else RP_F64
+    sysd.esp_order = int(esp_order)
+    return sysd, (Z, xyz, esp)
+
+
+def make_options(fit_type=1, precision=RP_PREC_FP64, vdw_factor=1.0, vdw_max=1000.0, r_cutoff=20.0):
+    return RpOptions(int(fit_type), int(precision), float(vdw_factor), float(vdw_max), float(r_cutoff))
+
+
 class Fit:
     """Device-resident fit (filter done at construction)."""
 
@@ -180,29 +243,23 @@ class Fit:
                  fit_type=1, precision=RP_PREC_FP64, vdw_factor=1.0, vdw_max=1000.0, r_cutoff=20.0):
         L = load()
         self.ctx = ctx
-        self._Z = np.ascontiguousarray(atomic_numbers, dtype=np.int32)
-        self._xyz = np.ascontiguousarray(atom_xyz, dtype=np.float64).reshape(-1, 3)
-        esp = np.ascontiguousarray(esp)
-        if esp.dtype not in (np.float32, np.float64):
-            esp = esp.astype(np.float64)
-        self._esp = esp
-        sysd = RpSystem()
-        for i in range(3):
-            sysd.n_grid[i] = int(n_grid[i])
-        ax = np.asarray(axis, dtype=np.float64).ravel()
-        for i in range(9):
-            sysd.axis[i] = float(ax[i])
-        sysd.n_atoms = len(self._Z)
-        sysd.atomic_number = ptr(self._Z, ctypes.c_int32)
-        sysd.atom_xyz = ptr(self._xyz)
-        sysd.esp = esp.ctypes.data_as(ctypes.c_void_p)
-        sysd.esp_dtype = RP_F32 if esp.dtype == np.float32 else RP_F64
-        sysd.esp_order = int(esp_order)
-        opt = RpOptions(int(fit_type), int(precision), float(vdw_factor), float(vdw_max), float(r_cutoff))
+        sysd, self._arrays = make_system(n_grid, axis, atomic_numbers, atom_xyz, esp, esp_order)
+        opt = make_options(fit_type, precision, vdw_factor, vdw_max, r_cutoff)
         h = ctypes.c_void_p()
         check(L.rp_fit_create(ctx.handle, ctypes.byref(sysd), ctypes.byref(opt), ctypes.byref(h)), ctx.handle)
         self.handle = h
-        self.n_atoms = len(self._Z)
+        ctx._fits.add(self)
+        self.n_atoms = int(sysd.n_atoms)
+        self.info = self._info()
+
+    def reset(self, n_grid, axis, atomic_numbers, atom_xyz, esp, esp_order=RP_ORDER_CUBE, fit_type=1,
+              precision=RP_PREC_FP64, vdw_factor=1.0, vdw_max=1000.0, r_cutoff=20.0):
+        """Reload this fit with another system, reusing its device buffers."""
+        sysd, arrays = make_system(n_grid, axis, atomic_numbers, atom_xyz, esp, esp_order)
+        opt = make_options(fit_type, precision, vdw_factor, vdw_max, r_cutoff)
+        check(load().rp_fit_reset(self.handle, ctypes.byref(sysd), ctypes.byref(opt)), self.ctx.handle)
+        self._arrays = arrays
+        self.n_atoms = int(sysd.n_atoms)
         self.info = self._info()
 
     def _info(self):
@@ -300,6 +357,85 @@ class Fit:
     def close(self):
         if getattr(self, "handle", None):
             load().rp_fit_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Server:
+    """Serving loop: concurrent fits in ``slots`` worker slots on one device within ``hbm_budget`` bytes.
+
+    ``submit`` copies the request and returns a ticket; ``poll`` returns ``None`` while it is queued or
+    running, else ``(charges, result)`` with ``result`` a dict of the server-clock times and status.
+    """
+
+    def __init__(self, device=0, slots=4, hbm_budget=200 * 10**9):
+        L = load()
+        h = ctypes.c_void_p()
+        rc = L.rp_server_create(int(device), int(slots), int(hbm_budget), ctypes.byref(h))
+        if rc != RP_OK:
+            raise RepeatHipError(rc, L.rp_last_error(None).decode(errors="replace"))
+        self.handle = h
+        self._n_atoms = {}
+
+    @staticmethod
+    def request(system_args, con, options):
+        """``(struct, keepalive)`` for ``system_args`` = (n_grid, axis, Z, xyz, esp[, esp_order])."""
+        sysd, arrays = make_system(*system_args)
+        c, carr = Fit._constraints(con)
+        req = RpRequest()
+        req.system = sysd
+        req.options = options
+        req.constraints = c
+        return req, (arrays, carr)
+
+    @staticmethod
+    def request_bytes(system_args, con, options):
+        req, keep = Server.request(system_args, con, options)
+        v = ctypes.c_uint64()
+        rc = load().rp_request_bytes(ctypes.byref(req), ctypes.byref(v))
+        if rc != RP_OK:
+            raise RepeatHipError(rc, load().rp_last_error(None).decode(errors="replace"))
+        return int(v.value)
+
+    def now(self):
+        return float(load().rp_server_now(self.handle))
+
+    def submit(self, system_args, con, options):
+        req, keep = self.request(system_args, con, options)
+        t = ctypes.c_uint64()
+        rc = load().rp_submit(self.handle, ctypes.byref(req), ctypes.byref(t))
+        if rc != RP_OK:
+            raise RepeatHipError(rc, load().rp_server_error(self.handle).decode(errors="replace"))
+        self._n_atoms[t.value] = int(req.system.n_atoms)
+        return int(t.value)
+
+    def poll(self, ticket, wait_ms=-1):
+        res = RpResult()
+        q = np.empty(self._n_atoms[ticket])
+        rc = load().rp_poll(self.handle, ctypes.c_uint64(ticket), int(wait_ms), ctypes.byref(res), ptr(q))
+        if rc == RP_EAGAIN:
+            return None
+        if rc != RP_OK:
+            raise RepeatHipError(rc, "rp_poll: unknown ticket")
+        del self._n_atoms[ticket]
+        out = {"status": res.status, "slot": res.slot, "n_keep": res.n_keep, "t_submit": res.t_submit,
+               "t_admit": res.t_admit, "t_done": res.t_done, "device_ms": list(res.device_ms),
+               "bytes_bound": int(res.bytes_bound), "error": res.error.decode(errors="replace")}
+        return q, out
+
+    def stats(self):
+        st = RpServerStats()
+        check(load().rp_server_stats_get(self.handle, ctypes.byref(st)))
+        return {k: getattr(st, k) for k, _ in RpServerStats._fields_}
+
+    def close(self):
+        if getattr(self, "handle", None):
+            load().rp_server_destroy(self.handle)
             self.handle = None
 
     def __del__(self):

@@ -105,15 +105,15 @@ int prepare_device(rp_fit *fit, FitDevice &dv, int precision) {
         phase_table(0, 7, s.nx, ex);
         phase_table(-7, 7, s.ny, ey);
         phase_table(-7, 7, s.nz, ez);
-        RP_CHECK(dv.ex.alloc(ctx, sizeof(double2) * ex.size()));
-        RP_CHECK(dv.ey.alloc(ctx, sizeof(double2) * ey.size()));
-        RP_CHECK(dv.ez.alloc(ctx, sizeof(double2) * ez.size()));
+        RP_CHECK(dv.ex.grow(ctx, sizeof(double2) * ex.size()));
+        RP_CHECK(dv.ey.grow(ctx, sizeof(double2) * ey.size()));
+        RP_CHECK(dv.ez.grow(ctx, sizeof(double2) * ez.size()));
         RP_HIP(ctx, hipMemcpyAsync(dv.ex.p, ex.data(), sizeof(double2) * ex.size(), hipMemcpyHostToDevice, st));
         RP_HIP(ctx, hipMemcpyAsync(dv.ey.p, ey.data(), sizeof(double2) * ey.size(), hipMemcpyHostToDevice, st));
         RP_HIP(ctx, hipMemcpyAsync(dv.ez.p, ez.data(), sizeof(double2) * ez.size(), hipMemcpyHostToDevice, st));
-        RP_CHECK(dv.khkl.alloc(ctx, sizeof(int) * s.khkl.size()));
-        RP_CHECK(dv.kcoef.alloc(ctx, sizeof(double) * s.kcoefs.size()));
-        RP_CHECK(dv.kvecs.alloc(ctx, sizeof(double) * s.kvecs.size()));
+        RP_CHECK(dv.khkl.grow(ctx, sizeof(int) * s.khkl.size()));
+        RP_CHECK(dv.kcoef.grow(ctx, sizeof(double) * s.kcoefs.size()));
+        RP_CHECK(dv.kvecs.grow(ctx, sizeof(double) * s.kvecs.size()));
         RP_HIP(ctx, hipMemcpyAsync(dv.khkl.p, s.khkl.data(), sizeof(int) * s.khkl.size(), hipMemcpyHostToDevice, st));
         RP_HIP(ctx, hipMemcpyAsync(dv.kcoef.p, s.kcoefs.data(), sizeof(double) * s.kcoefs.size(), hipMemcpyHostToDevice, st));
         RP_HIP(ctx, hipMemcpyAsync(dv.kvecs.p, s.kvecs.data(), sizeof(double) * s.kvecs.size(), hipMemcpyHostToDevice, st));
@@ -140,7 +140,7 @@ int prepare_device(rp_fit *fit, FitDevice &dv, int precision) {
                 for (int c = 0; c < 3; ++c) acc += fit->atom_xyz[3 * a + c] * cof[3 * i + c] / det;
                 af[3 * a + i] = acc;
             }
-        RP_CHECK(dv.afrac.alloc(ctx, sizeof(double) * af.size()));
+        RP_CHECK(dv.afrac.grow(ctx, sizeof(double) * af.size()));
         RP_HIP(ctx, hipMemcpyAsync(dv.afrac.p, af.data(), sizeof(double) * af.size(), hipMemcpyHostToDevice, st));
         RP_HIP(ctx, hipStreamSynchronize(st));
         for (int i = 0; i < 3; ++i) {
@@ -154,7 +154,7 @@ int prepare_device(rp_fit *fit, FitDevice &dv, int precision) {
         dv.epi.inv_n[2] = 1.0 / s.nz;
     }
     if (rebuild) {
-        RP_CHECK(dv.shifts.alloc(ctx, sizeof(double) * fit->shifts_pruned.size()));
+        RP_CHECK(dv.shifts.grow(ctx, sizeof(double) * fit->shifts_pruned.size()));
         RP_HIP(ctx, hipMemcpyAsync(dv.shifts.p, fit->shifts_pruned.data(), sizeof(double) * fit->shifts_pruned.size(),
                                    hipMemcpyHostToDevice, st));
         dv.shifts_host = fit->shifts_pruned;
@@ -172,20 +172,15 @@ int prepare_device(rp_fit *fit, FitDevice &dv, int precision) {
     dv.epi.R2 = s.r_cutoff * s.r_cutoff;
     dv.epi.sqrt_alpha = s.sqrt_alpha;
 
-    // chunking: at most 65536 points per chunk, at most ~2 GiB of chunk buffers
-    const int64_t M = fit->n_keep;
-    const int64_t per_row = 8 * (int64_t)(dv.KP + dv.NP);
-    int64_t rows = std::min<int64_t>(65536, round_up(M, 128));
-    while (rows > 128 && rows * per_row > ((int64_t)1 << 31)) rows /= 2;
-    rows = std::max<int64_t>(128, round_up(rows, 128));
-    dv.rows = rows;
+    // fp64 chunking (also used by the fp64 matvec of a split-precision fit): at most 65536 points
+    // per chunk, at most ~2 GiB of chunk buffers
+    const ChunkPlan cp = plan_chunks(fit->n_keep, N, dv.K, RP_PREC_FP64);
+    dv.rows = cp.rows;
     const int nt = dv.NP / 128;
     dv.tiles.clear();
     for (int bi = 0; bi < nt; ++bi)
         for (int bj = bi; bj < nt; ++bj) dv.tiles.push_back(make_int2(bi, bj));
-    int splits = 1;
-    while ((int64_t)dv.tiles.size() * splits < 512 && rows / (splits * 2) >= 1024 && (rows / (splits * 2)) % 64 == 0) splits *= 2;
-    dv.splits = splits;
+    dv.splits = cp.splits;
     if (rebuild || dv.d_tiles.bytes < sizeof(int2) * dv.tiles.size()) {
         RP_CHECK(dv.d_tiles.grow(ctx, sizeof(int2) * dv.tiles.size()));
         RP_HIP(ctx, hipMemcpyAsync(dv.d_tiles.p, dv.tiles.data(), sizeof(int2) * dv.tiles.size(), hipMemcpyHostToDevice, st));
@@ -207,7 +202,7 @@ int run_accumulate(rp_fit *fit) {
     RP_CHECK(device_setup(fit, fit->opt.precision, dvp));
     FitDevice &dv = *dvp;
     const int64_t g2 = (int64_t)dv.NP * dv.NP;
-    if (fit->d_gram.bytes != sizeof(double) * (size_t)g2) RP_CHECK(fit->d_gram.alloc(ctx, sizeof(double) * g2));
+    RP_CHECK(fit->d_gram.grow(ctx, sizeof(double) * g2));
     hipEvent_t e0, e1;
     RP_HIP(ctx, hipEventCreate(&e0));
     RP_HIP(ctx, hipEventCreate(&e1));
@@ -250,11 +245,48 @@ int run_phi_points(rp_fit *fit, int64_t m, const double *grid_xyz, double *phi_o
     return RP_OK;
 }
 
+// Validates a system, derives the host setup, uploads atoms and the cube, and runs the vdW filter.
+// Reuses the fit's device buffers (grow-only), so reloading a fit with a system of equal or smaller
+// size allocates nothing.
+int load_system(rp_fit *fit, const rp_system *sys, const rp_options *opt) {
+    Ctx *ctx = fit->ctx;
+    if (sys->n_atoms < 1 || !sys->atomic_number || !sys->atom_xyz || !sys->esp)
+        return ctx->fail(RP_EINVAL, "rp_fit_create: system needs atoms, positions and ESP");
+    if (sys->esp_dtype != RP_F64 && sys->esp_dtype != RP_F32) return ctx->fail(RP_EINVAL, "bad esp_dtype");
+    if (sys->esp_order != RP_ORDER_CUBE && sys->esp_order != RP_ORDER_FLAT) return ctx->fail(RP_EINVAL, "bad esp_order");
+    if (opt->precision != RP_PREC_FP64 && opt->precision != RP_PREC_OZAKI) return ctx->fail(RP_EINVAL, "bad precision");
+    fit->opt = *opt;
+    fit->n_atoms = sys->n_atoms;
+    fit->Z.assign(sys->atomic_number, sys->atomic_number + sys->n_atoms);
+    fit->atom_xyz.assign(sys->atom_xyz, sys->atom_xyz + 3 * sys->n_atoms);
+    fit->setup = Setup();
+    fit->n_keep = 0;
+    fit->v_mean = 0.0;
+    fit->shifts_pruned.clear();
+    fit->accumulated = false;
+    fit->lu_fallback = false;
+    fit->na = 0;
+    if (fit->dev) fit->dev->ready = false;
+    RP_CHECK(build_setup(sys, opt, fit->setup, ctx->err));
+    RP_CHECK(fit->d_atoms.grow(ctx, sizeof(double) * 3 * sys->n_atoms));
+    RP_HIP(ctx, hipMemcpyAsync(fit->d_atoms.p, sys->atom_xyz, sizeof(double) * 3 * sys->n_atoms, hipMemcpyHostToDevice,
+                               ctx->stream));
+    RP_CHECK(upload_filter(fit, sys));
+    return run_filter(fit);
+}
+
 }  // namespace rp
 
 using namespace rp;
 
 rp_fit::~rp_fit() { delete dev; }
+
+static void destroy_ctx(rp_ctx *ctx) {
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
 
 extern "C" {
 
@@ -291,10 +323,11 @@ int rp_ctx_create(int device, uint64_t hbm_budget_bytes, rp_ctx **out) {
 
 int rp_ctx_destroy(rp_ctx *ctx) {
     if (!ctx) return RP_OK;
-    (void)hipSetDevice(ctx->device);
-    (void)hipStreamSynchronize(ctx->stream);
-    (void)hipStreamDestroy(ctx->stream);
-    delete ctx;
+    if (ctx->n_fits > 0) {
+        ctx->closing = true;  // the last rp_fit_destroy finishes the job
+        return RP_OK;
+    }
+    destroy_ctx(ctx);
     return RP_OK;
 }
 
@@ -337,40 +370,33 @@ int rp_fit_refit(rp_fit *fit, const rp_constraints *con, double *q_out) {
 int rp_fit_create(rp_ctx *ctx, const rp_system *sys, const rp_options *opt, rp_fit **out) {
     if (!ctx || !sys || !opt || !out) return ctx ? ctx->fail(RP_EINVAL, "rp_fit_create: NULL argument") : RP_EINVAL;
     *out = nullptr;
-    if (sys->n_atoms < 1 || !sys->atomic_number || !sys->atom_xyz || !sys->esp)
-        return ctx->fail(RP_EINVAL, "rp_fit_create: system needs atoms, positions and ESP");
-    if (sys->esp_dtype != RP_F64 && sys->esp_dtype != RP_F32) return ctx->fail(RP_EINVAL, "bad esp_dtype");
-    if (sys->esp_order != RP_ORDER_CUBE && sys->esp_order != RP_ORDER_FLAT) return ctx->fail(RP_EINVAL, "bad esp_order");
-    if (opt->precision != RP_PREC_FP64 && opt->precision != RP_PREC_OZAKI) return ctx->fail(RP_EINVAL, "bad precision");
     RP_HIP(ctx, hipSetDevice(ctx->device));
     rp_fit *fit = new rp_fit();
     fit->ctx = ctx;
-    fit->opt = *opt;
-    fit->n_atoms = sys->n_atoms;
-    fit->Z.assign(sys->atomic_number, sys->atomic_number + sys->n_atoms);
-    fit->atom_xyz.assign(sys->atom_xyz, sys->atom_xyz + 3 * sys->n_atoms);
-    int rc = build_setup(sys, opt, fit->setup, ctx->err);
-    if (rc == RP_OK) rc = fit->d_atoms.alloc(ctx, sizeof(double) * 3 * sys->n_atoms);
-    if (rc == RP_OK) {
-        hipError_t e = hipMemcpyAsync(fit->d_atoms.p, sys->atom_xyz, sizeof(double) * 3 * sys->n_atoms,
-                                      hipMemcpyHostToDevice, ctx->stream);
-        if (e != hipSuccess) rc = ctx->fail(RP_EHIP, std::string("hipMemcpyAsync failed: ") + hipGetErrorString(e));
-    }
-    if (rc == RP_OK) rc = upload_filter(fit, sys);
-    if (rc == RP_OK) rc = run_filter(fit);
+    const int rc = load_system(fit, sys, opt);
     if (rc != RP_OK) {
         delete fit;
         return rc;
     }
+    ++ctx->n_fits;
     *out = fit;
     return RP_OK;
 }
 
+int rp_fit_reset(rp_fit *fit, const rp_system *sys, const rp_options *opt) {
+    if (!fit || !sys || !opt) return fit ? fit->ctx->fail(RP_EINVAL, "rp_fit_reset: NULL argument") : RP_EINVAL;
+    RP_HIP(fit->ctx, hipSetDevice(fit->ctx->device));
+    RP_HIP(fit->ctx, hipStreamSynchronize(fit->ctx->stream));
+    return load_system(fit, sys, opt);
+}
+
 int rp_fit_destroy(rp_fit *fit) {
     if (!fit) return RP_OK;
-    (void)hipSetDevice(fit->ctx->device);
-    (void)hipStreamSynchronize(fit->ctx->stream);
+    rp_ctx *ctx = static_cast<rp_ctx *>(fit->ctx);
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
     delete fit;
+    if (--ctx->n_fits == 0 && ctx->closing) destroy_ctx(ctx);
     return RP_OK;
 }
 

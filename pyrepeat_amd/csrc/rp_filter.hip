@@ -249,18 +249,9 @@ __global__ __launch_bounds__(kBlock) void k_center_and_bounds(int nx, int ny, co
 
 }  // namespace
 
-int upload_filter(rp_fit *fit, const rp_system *sys) {
-    Ctx *ctx = fit->ctx;
-    const Setup &s = fit->setup;
-    FilterState &fs = fit->filt;
-    const int64_t n_total = (int64_t)s.nx * s.ny * s.nz;
-    fit->n_total = n_total;
-    fs.n_blocks = ceil_div(n_total, kBlock);
-    fs.esp_dtype = sys->esp_dtype;
-    fs.esp_order = sys->esp_order;
-
-    // ---- cell list over the grid's bounding box (host)
-    double glo[3], ghi[3];
+// Cell-list geometry over the grid's bounding box: bin edge h >= max r_min, bn bins per axis
+// starting at bmin, at most 2^22 bins.
+void filter_bins(const Setup &s, double glo[3], double ghi[3], double &h, int bn[3], double bmin[3]) {
     for (int c = 0; c < 3; ++c) {
         glo[c] = INFINITY;
         ghi[c] = -INFINITY;
@@ -275,10 +266,7 @@ int upload_filter(rp_fit *fit, const rp_system *sys) {
                     ghi[c] = std::max(ghi[c], g);
                 }
             }
-    const int64_t n_eff = (int64_t)s.eff_smin.size();
-    double h = std::max(s.max_rmin * 1.001 + 1e-9, 1e-3);
-    int bn[3];
-    double bmin[3];
+    h = std::max(s.max_rmin * 1.001 + 1e-9, 1e-3);
     for (int c = 0; c < 3; ++c) {
         bmin[c] = glo[c] - h;
         bn[c] = (int)std::ceil((ghi[c] - glo[c] + 2.0 * h) / h) + 1;
@@ -291,6 +279,39 @@ int upload_filter(rp_fit *fit, const rp_system *sys) {
             bn[c] = (int)std::ceil((ghi[c] - glo[c] + 2.0 * h) / h) + 1;
         }
     }
+}
+
+uint64_t filter_bytes_bound(const Setup &s, int esp_dtype) {
+    double glo[3], ghi[3], h, bmin[3];
+    int bn[3];
+    filter_bins(s, glo, ghi, h, bn, bmin);
+    const int64_t n_total = (int64_t)s.nx * s.ny * s.nz;
+    const int64_t nb = ceil_div(n_total, kBlock);
+    const int64_t E = std::max<int64_t>((int64_t)s.eff_smin.size(), 1);
+    const int64_t nbins = (int64_t)bn[0] * bn[1] * bn[2];
+    uint64_t b = 0;
+    b += 4 * (nbins + 1) + 2 * (32 + 8) * E;                 // bins, binned and unbinned images
+    b += 4 * nb + 8 * (nb + 1) + 8 * nb + 8 + 72;             // block counts, offsets, V sums, mean, axis
+    b += n_total * (1 + (esp_dtype == RP_F32 ? 4 : 8));       // mask, cube
+    b += 12 * n_total + 48 * ceil_div(n_total, kBlock);       // kept indices and V (M <= n_total), bounds
+    return b + 16 * 16;                                      // 16-byte minimum per buffer
+}
+
+int upload_filter(rp_fit *fit, const rp_system *sys) {
+    Ctx *ctx = fit->ctx;
+    const Setup &s = fit->setup;
+    FilterState &fs = fit->filt;
+    const int64_t n_total = (int64_t)s.nx * s.ny * s.nz;
+    fit->n_total = n_total;
+    fs.n_blocks = ceil_div(n_total, kBlock);
+    fs.esp_dtype = sys->esp_dtype;
+    fs.esp_order = sys->esp_order;
+
+    // ---- cell list over the grid's bounding box (host)
+    double glo[3], ghi[3], h, bmin[3];
+    int bn[3];
+    filter_bins(s, glo, ghi, h, bn, bmin);
+    const int64_t n_eff = (int64_t)s.eff_smin.size();
     const double binv = 1.0 / h;
     const int64_t nbins = (int64_t)bn[0] * bn[1] * bn[2];
     std::vector<int> bin_of(n_eff, -1);
@@ -337,19 +358,19 @@ int upload_filter(rp_fit *fit, const rp_system *sys) {
     fa.n_all = n_eff;
 
     const int64_t nb = fs.n_blocks;
-    RP_CHECK(fs.bin_start.alloc(ctx, sizeof(int) * (nbins + 1)));
-    RP_CHECK(fs.bin_atoms.alloc(ctx, sizeof(double4) * bin_atoms.size()));
-    RP_CHECK(fs.bin_smax.alloc(ctx, sizeof(double) * bin_smax.size()));
-    RP_CHECK(fs.all.alloc(ctx, sizeof(double4) * all_atoms.size()));
-    RP_CHECK(fs.all_smax.alloc(ctx, sizeof(double) * all_smax.size()));
-    RP_CHECK(fs.cnt.alloc(ctx, sizeof(int) * nb));
-    RP_CHECK(fs.off.alloc(ctx, sizeof(int64_t) * (nb + 1)));
-    RP_CHECK(fs.vsum.alloc(ctx, sizeof(double) * nb));
-    RP_CHECK(fs.mean.alloc(ctx, sizeof(double)));
-    RP_CHECK(fs.axis.alloc(ctx, sizeof(double) * 9));
-    RP_CHECK(fit->d_mask.alloc(ctx, (size_t)n_total));
+    RP_CHECK(fs.bin_start.grow(ctx, sizeof(int) * (nbins + 1)));
+    RP_CHECK(fs.bin_atoms.grow(ctx, sizeof(double4) * bin_atoms.size()));
+    RP_CHECK(fs.bin_smax.grow(ctx, sizeof(double) * bin_smax.size()));
+    RP_CHECK(fs.all.grow(ctx, sizeof(double4) * all_atoms.size()));
+    RP_CHECK(fs.all_smax.grow(ctx, sizeof(double) * all_smax.size()));
+    RP_CHECK(fs.cnt.grow(ctx, sizeof(int) * nb));
+    RP_CHECK(fs.off.grow(ctx, sizeof(int64_t) * (nb + 1)));
+    RP_CHECK(fs.vsum.grow(ctx, sizeof(double) * nb));
+    RP_CHECK(fs.mean.grow(ctx, sizeof(double)));
+    RP_CHECK(fs.axis.grow(ctx, sizeof(double) * 9));
+    RP_CHECK(fit->d_mask.grow(ctx, (size_t)n_total));
     const size_t esp_bytes = (size_t)n_total * (sys->esp_dtype == RP_F32 ? 4 : 8);
-    RP_CHECK(fs.esp.alloc(ctx, esp_bytes));
+    RP_CHECK(fs.esp.grow(ctx, esp_bytes));
     hipStream_t st = ctx->stream;
     RP_HIP(ctx, hipMemcpyAsync(fs.bin_start.p, bin_start.data(), sizeof(int) * (nbins + 1), hipMemcpyHostToDevice, st));
     RP_HIP(ctx, hipMemcpyAsync(fs.bin_atoms.p, bin_atoms.data(), sizeof(double4) * bin_atoms.size(), hipMemcpyHostToDevice, st));
@@ -388,11 +409,9 @@ int run_filter(rp_fit *fit) {
         hipEventDestroy(e1);
         return ctx->fail(RP_EEMPTY, "the vdW filter kept no grid point");
     }
-    if (n_keep != fit->n_keep || !fit->d_idx.p) {
-        fit->n_keep = n_keep;
-        RP_CHECK(fit->d_idx.alloc(ctx, sizeof(int) * n_keep));
-        RP_CHECK(fit->d_v.alloc(ctx, sizeof(double) * n_keep));
-    }
+    fit->n_keep = n_keep;
+    RP_CHECK(fit->d_idx.grow(ctx, sizeof(int) * n_keep));
+    RP_CHECK(fit->d_v.grow(ctx, sizeof(double) * n_keep));
     if (fs.esp_dtype == RP_F32)
         k_compact<float><<<n_blocks, kBlock, 0, st>>>(s.nx, s.ny, s.nz, fs.esp_order, fit->d_mask.as<uint8_t>(),
                                                       fs.off.as<int64_t>(), fs.esp.as<float>(), fit->d_idx.as<int>(),
@@ -403,7 +422,7 @@ int run_filter(rp_fit *fit) {
                                                        fit->d_v.as<double>(), fs.vsum.as<double>());
     k_mean<<<1, 1024, 0, st>>>(fs.vsum.as<double>(), n_blocks, fs.off.as<int64_t>() + n_blocks, fs.mean.as<double>());
     const int64_t cb = ceil_div(n_keep, kBlock);
-    if (fs.bounds.bytes < sizeof(double) * 6 * cb) RP_CHECK(fs.bounds.alloc(ctx, sizeof(double) * 6 * cb));
+    RP_CHECK(fs.bounds.grow(ctx, sizeof(double) * 6 * cb));
     k_center_and_bounds<<<cb, kBlock, 0, st>>>(s.nx, s.ny, fs.axis.as<double>(), fit->d_idx.as<int>(), fit->d_v.as<double>(),
                                                 n_keep, fs.mean.as<double>(), fs.bounds.as<double>());
     RP_HIP(ctx, hipGetLastError());

@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
@@ -45,6 +46,8 @@ struct Ctx {
     std::string err;
     hipDeviceProp_t prop{};
     bool profile = false;
+    int n_fits = 0;        // live fits on this context
+    bool closing = false;  // rp_ctx_destroy was called while fits were alive
     double kt_ms[KT_N] = {};
     int64_t kt_n[KT_N] = {};
     struct Pending {
@@ -125,6 +128,46 @@ struct FilterState {
 inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
 inline int64_t round_up(int64_t a, int64_t b) { return ceil_div(a, b) * b; }
 
+// Chunk geometry of the basis streaming for M kept points, N atoms and K k-vectors: the accumulate
+// paths allocate from it, and the serving loop's memory bound uses the same numbers.
+struct ChunkPlan {
+    int KP = 0;          // padded [cos | sin] width of the k-space operands
+    int NP = 0;          // padded atom width (N + V column + ones column)
+    int64_t rows = 0;    // kept points per chunk
+    int n_tiles = 0;     // Gram tiles touching the upper triangle
+    int splits = 1;      // row splits of the Gram per chunk
+};
+
+inline ChunkPlan plan_chunks(int64_t M, int N, int K, int precision) {
+    ChunkPlan c;
+    c.NP = (int)round_up(N + 2, 128);
+    if (precision == RP_PREC_FP64) {
+        c.KP = K > 0 ? (int)round_up(2 * K, 16) : 0;
+        const int64_t per_row = 8 * (int64_t)(c.KP + c.NP);
+        int64_t rows = std::min<int64_t>(65536, round_up(M, 128));
+        while (rows > 128 && rows * per_row > ((int64_t)1 << 31)) rows /= 2;
+        c.rows = std::max<int64_t>(128, round_up(rows, 128));
+        const int nt = c.NP / 128;
+        c.n_tiles = nt * (nt + 1) / 2;
+        while ((int64_t)c.n_tiles * c.splits < 512 && c.rows / (c.splits * 2) >= 1024 && (c.rows / (c.splits * 2)) % 64 == 0)
+            c.splits *= 2;
+    } else {
+        c.KP = K > 0 ? (int)round_up(2 * K, 64) : 0;
+        int64_t rows = std::min<int64_t>(65536, round_up(M, 512));
+        rows = std::max<int64_t>(512, rows);
+        // bound the chunk buffers (G slices, Phi^T, Phi slices) to about 2 GiB
+        while (rows > 512 && rows * (4LL * c.KP + 12LL * c.NP) > ((int64_t)1 << 31)) rows /= 2;
+        c.rows = std::max<int64_t>(512, rows - rows % 512);
+        c.n_tiles = 0;
+        for (int bi = 0; bi < c.NP / 64; ++bi)
+            for (int bj = 0; bj < c.NP / 128; ++bj)
+                if (bi * 64 <= bj * 128 + 127) ++c.n_tiles;
+        while ((int64_t)c.n_tiles * c.splits < 1024 && c.rows / (c.splits * 2) >= 1024 && (c.rows / (c.splits * 2)) % 64 == 0)
+            c.splits *= 2;
+    }
+    return c;
+}
+
 // ---------------------------------------------------------------- geometry / Ewald (host)
 
 struct Vec3 {
@@ -155,6 +198,8 @@ struct Setup {
 
 int build_setup(const rp_system *sys, const rp_options *opt, Setup &s, std::string &err);
 double sqrt_threshold(double r);  // largest double s with fl(sqrt(s)) <= r
+// Device bytes the vdW filter of this setup allocates at most (any kept count).
+uint64_t filter_bytes_bound(const Setup &s, int esp_dtype);
 std::vector<double> prune_shifts(const std::vector<double> &shifts, const double gmin[3], const double gmax[3],
                                  const std::vector<double> &atom_xyz, double r_cutoff);
 
@@ -190,7 +235,7 @@ struct rp_fit {
     rp::DBuf d_gram;      // NA x NA doubles, NA = N + 2 (augmented with V and 1)
     int32_t na = 0;       // augmented width N + 2
     // grow-only workspaces reused by every accumulate / solve / matvec of this fit
-    rp::DBuf ws[24];
+    rp::DBuf ws[32];
     rp::FitDevice *dev = nullptr;   // persistent device copy of the Ewald setup
     ~rp_fit();
 };
@@ -200,6 +245,7 @@ struct rp_ctx : rp::Ctx {};
 namespace rp {
 struct FitDevice;
 // Phases implemented in the .hip files.
+int load_system(rp_fit *fit, const rp_system *sys, const rp_options *opt);
 int upload_filter(rp_fit *fit, const rp_system *sys);
 int run_filter(rp_fit *fit);
 int run_accumulate(rp_fit *fit);

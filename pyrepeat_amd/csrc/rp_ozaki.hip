@@ -448,12 +448,12 @@ int accumulate_split(rp_fit *fit, FitDevice &dv) {
     Ctx *ctx = fit->ctx;
     hipStream_t st = ctx->stream;
     const int K = dv.K, NP = dv.NP, N = fit->n_atoms;
-    const int KP = K > 0 ? (int)round_up(2 * K, TK) : 0;
     const int64_t M = fit->n_keep;
-    int64_t rows = std::min<int64_t>(65536, round_up(M, 512));
-    rows = std::max<int64_t>(512, round_up(rows, 512));
-    // bound the chunk buffers (G slices, Phi^T, Phi slices) to about 2 GiB
-    while (rows > 512 && rows * ((int64_t)PMAX * KP + 12LL * NP) > ((int64_t)1 << 31)) rows /= 2;
+    const ChunkPlan cp = plan_chunks(M, N, K, RP_PREC_OZAKI);
+    const int KP = cp.KP;
+    const int64_t rows = cp.rows;
+    if (cp.NP != NP || KP % TK != 0 || PMAX != 4 || TM != 64 || TN != 128)
+        return ctx->fail(RP_EINVAL, "split path: chunk plan does not match the kernel tiling");
     // operands
     DBuf &d_G = fit->ws[0], &d_B = fit->ws[1], &d_phiT = fit->ws[2], &d_part = fit->ws[3], &d_slices = fit->ws[4],
          &d_colmax = fit->ws[5], &d_scale = fit->ws[6], &d_wsc = fit->ws[7], &d_ksl = fit->ws[8], &d_tiles = fit->ws[9],
@@ -486,8 +486,9 @@ int accumulate_split(rp_fit *fit, FitDevice &dv) {
         for (int bj = 0; bj < NP / TN; ++bj)
             if (bi * TM <= bj * TN + TN - 1) tiles.push_back(make_int2(bi, bj));
     const int n_tiles = (int)tiles.size();
-    int splits = 1;
-    while ((int64_t)n_tiles * splits < 1024 && rows / (splits * 2) >= 1024 && (rows / (splits * 2)) % TK == 0) splits *= 2;
+    const int splits = cp.splits;
+    if (n_tiles != cp.n_tiles || rows % (splits * TK) != 0)
+        return ctx->fail(RP_EINVAL, "split path: Gram tiling does not match the chunk plan");
     const int rps = (int)(rows / splits);
     RP_CHECK(d_tiles.grow(ctx, sizeof(int2) * n_tiles));
     RP_HIP(ctx, hipMemcpyAsync(d_tiles.p, tiles.data(), sizeof(int2) * n_tiles, hipMemcpyHostToDevice, st));

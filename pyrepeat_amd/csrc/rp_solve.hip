@@ -495,7 +495,7 @@ int run_solve(rp_fit *fit, const rp_constraints *con, double *q_out, double *sol
     const int ld = (int)round_up(ncols, 4);
     DBuf &d_K = fit->ws[12], &d_ptr = fit->ws[13], &d_src = fit->ws[14], &d_mult = fit->ws[15], &d_w = fit->ws[16],
          &d_diag = fit->ws[17], &d_rhs = fit->ws[18], &d_piv = fit->ws[19];
-    DBuf d_info, d_x, d_exp, d_q;
+    DBuf &d_info = fit->ws[24], &d_x = fit->ws[25], &d_exp = fit->ws[26], &d_q = fit->ws[27];
     RP_CHECK(d_K.grow(ctx, sizeof(double) * (size_t)n * ld));
     RP_CHECK(d_ptr.grow(ctx, sizeof(int) * (nr + 1)));
     RP_CHECK(d_src.grow(ctx, sizeof(int) * std::max(nsrc, 1)));
@@ -504,10 +504,10 @@ int run_solve(rp_fit *fit, const rp_constraints *con, double *q_out, double *sol
     RP_CHECK(d_diag.grow(ctx, sizeof(double) * nr));
     RP_CHECK(d_rhs.grow(ctx, sizeof(double) * nr));
     RP_CHECK(d_piv.grow(ctx, sizeof(int) * n));
-    RP_CHECK(d_info.alloc(ctx, sizeof(int)));
-    RP_CHECK(d_x.alloc(ctx, sizeof(double) * n));
-    RP_CHECK(d_exp.alloc(ctx, sizeof(int) * N));
-    RP_CHECK(d_q.alloc(ctx, sizeof(double) * N));
+    RP_CHECK(d_info.grow(ctx, sizeof(int)));
+    RP_CHECK(d_x.grow(ctx, sizeof(double) * n));
+    RP_CHECK(d_exp.grow(ctx, sizeof(int) * N));
+    RP_CHECK(d_q.grow(ctx, sizeof(double) * N));
     RP_HIP(ctx, hipMemcpyAsync(d_ptr.p, con->red_ptr, sizeof(int) * (nr + 1), hipMemcpyHostToDevice, st));
     RP_HIP(ctx, hipMemcpyAsync(d_src.p, con->red_src, sizeof(int) * nsrc, hipMemcpyHostToDevice, st));
     RP_HIP(ctx, hipMemcpyAsync(d_mult.p, con->red_mult, sizeof(double) * nsrc, hipMemcpyHostToDevice, st));
@@ -530,13 +530,13 @@ int run_solve(rp_fit *fit, const rp_constraints *con, double *q_out, double *sol
                                                                   d_K.as<double>(), ld);
     // ---- Cholesky of A (leading nr x nr block) + Schur step
     DBuf &d_K2 = fit->ws[20], &d_yw = fit->ws[21], &d_yb = fit->ws[22], &d_wc = fit->ws[23];
-    DBuf d_cinfo, d_bc;
+    DBuf &d_cinfo = fit->ws[28], &d_bc = fit->ws[29];
     RP_CHECK(d_K2.grow(ctx, sizeof(double) * (size_t)n * ld));
     RP_CHECK(d_yw.grow(ctx, sizeof(double) * nr));
     RP_CHECK(d_yb.grow(ctx, sizeof(double) * nr));
-    RP_CHECK(d_cinfo.alloc(ctx, sizeof(int)));
+    RP_CHECK(d_cinfo.grow(ctx, sizeof(int)));
     RP_CHECK(d_wc.grow(ctx, sizeof(double) * nr));
-    RP_CHECK(d_bc.alloc(ctx, sizeof(double) * nr));
+    RP_CHECK(d_bc.grow(ctx, sizeof(double) * nr));
     RP_HIP(ctx, hipMemsetAsync(d_cinfo.p, 0, sizeof(int), st));
     k_copy<<<ceil_div((int64_t)n * ld, 256), 256, 0, st>>>(d_K.as<double>(), d_K2.as<double>(), (int64_t)n * ld);
     // the w and b columns before factorisation

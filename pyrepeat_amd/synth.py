@@ -13,7 +13,10 @@ Workloads (random streams ``numpy.random.default_rng([cfg, req])``):
 - ``crystal200`` 200 atoms, triclinic 24/22/26 Bohr, 85/95/100 deg, 96x96x100, fp64,
                  20 symmetry groups of 2-4 same-element atoms
 - ``mof1000``    1000 atoms, cubic 40 Bohr, 160^3, fp32, RESP ``H 0.1 0.0``
-- ``stream``     request ``r`` of the Poisson serving stream (20-800 atoms, 0.3 Bohr spacing, fp32)
+- ``stream``     request ``r`` of the Poisson serving stream (20-800 atoms, 0.3 Bohr spacing, fp32);
+                 its ESP uses the splitting eta = 0.6 Bohr^-1, real cutoff 8 Bohr, |k| <= 5 Bohr^-1, which
+                 sums the same potential with the same truncation (erfc(eta Rc) and exp(-k^2/4 eta^2) at
+                 the cutoffs are as small or smaller) at 1/16 of the real-space work
 - ``large5000``  5000 atoms, cubic 70 Bohr, 232^3, fp32
 """
 from __future__ import annotations
@@ -220,8 +223,11 @@ def ewald_esp(n_grid, axis, atom_pos, charges, eta=ETA, rcut=RCUT, kcut=KCUT, nt
     return out
 
 
-def _finish(name, rng, n_grid, axis, Z, pos, q, dtype, **kw):
-    flat = ewald_esp(n_grid, axis, pos, q)
+STREAM_SPLIT = {"eta": 0.6, "rcut": 8.0, "kcut": 5.0}
+
+
+def _finish(name, rng, n_grid, axis, Z, pos, q, dtype, split=None, nthreads=None, **kw):
+    flat = ewald_esp(n_grid, axis, pos, q, nthreads=nthreads, **(split or {}))
     flat = flat + rng.normal(0.0, NOISE, size=flat.shape)
     nx, ny, nz = (int(v) for v in n_grid)
     cube = flat.reshape(nz, ny, nx).transpose(2, 1, 0).ravel().astype(dtype)
@@ -305,7 +311,7 @@ def stream_schedule(n_requests=512, rate=20.0):
     return arrivals, n_atoms
 
 
-def stream_request(r, n_atoms=None):
+def stream_request(r, n_atoms=None, nthreads=None):
     """Request ``r`` of the serving stream: cubic cell at 0.01 atoms/Bohr^3, 0.3 Bohr spacing, fp32."""
     if n_atoms is None:
         n_atoms = int(stream_schedule()[1][r])
@@ -320,7 +326,15 @@ def stream_request(r, n_atoms=None):
     q += (q_tot - q.sum()) / n_atoms
     groups = _symmetry_groups(rng, list(el), min(20, n_atoms // 10)) if rng.random() < 0.5 else []
     return _finish(f"stream{r}", rng, [n, n, n], box / n, [Z_OF[e] for e in el], pos, q, np.float32,
-                   symm_groups=groups, q_tot=q_tot)
+                   split=STREAM_SPLIT, nthreads=nthreads, symm_groups=groups, q_tot=q_tot)
+
+
+def stream_requests(count=512, workers=4):
+    """The first ``count`` requests of the serving stream, generated by ``workers`` threads."""
+    from concurrent.futures import ThreadPoolExecutor
+    per = max(1, len(os.sched_getaffinity(0)) // workers)
+    with ThreadPoolExecutor(workers) as ex:
+        return list(ex.map(lambda r: stream_request(r, nthreads=per), range(count)))
 
 
 def tiny(seed=0, n_atoms=3, n=10, L=9.0, triclinic=False):

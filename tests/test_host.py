@@ -15,7 +15,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 def test_library_exports_every_declared_symbol():
     lib = _abi.load()
     header = open(os.path.join(ROOT, "include", "repeat_hip.h")).read()
-    declared = set(re.findall(r"^\s*(?:int|const char \*)\s*\**(rp_\w+)\s*\(", header, re.M))
+    declared = set(re.findall(r"^\s*(?:int|double|const char \*)\s*\**(rp_\w+)\s*\(", header, re.M))
     assert declared, "no declarations parsed"
     assert declared == set(_abi.EXPORTED)
     for name in declared:
@@ -112,3 +112,33 @@ def test_cube_roundtrip_and_reference_layout(tmp_path):
     np.testing.assert_array_equal(v_pot_view(c), vp)
     # F-order ravel of V_pot is the flat order i + j*nx + k*nx*ny (repeat.py:2086)
     np.testing.assert_allclose(vp.ravel(order="F"), s.esp_flat(), rtol=1e-10)
+
+
+def test_request_bound_on_host_and_invalid_requests():
+    """rp_request_bytes is host-only: it grows with the grid and the atom count and rejects bad requests."""
+    from pyrepeat_amd import params, synth
+    small, large = synth.tiny(seed=1, n_atoms=4, n=12), synth.tiny(seed=1, n_atoms=4, n=24)
+    opts = _abi.make_options(1, _abi.RP_PREC_OZAKI)
+    con = params.constraint_arrays(4, small.atomic_numbers, 0.0, None, None, None)
+    args = lambda s: (s.n_grid, s.axis, s.atomic_numbers, s.atom_pos, s.esp_cube)
+    b_small = _abi.Server.request_bytes(args(small), con, opts)
+    b_large = _abi.Server.request_bytes(args(large), con, opts)
+    assert 0 < b_small < b_large
+    # the split chunk buffers dominate: 4 slices of 2K padded columns per row at least
+    assert b_small >= 4 * 512 * 1536
+    b64 = _abi.Server.request_bytes(args(small), con, _abi.make_options(1, _abi.RP_PREC_FP64))
+    assert b64 > 0
+    bad = dict(con, n_reduced=0)
+    with pytest.raises(_abi.RepeatHipError) as e:
+        _abi.Server.request_bytes(args(small), bad, opts)
+    assert e.value.code == _abi.RP_EINVAL
+
+
+def test_stream_splitting_sums_the_same_potential():
+    """The serving stream's Ewald splitting (eta 0.6, Rc 8, |k| <= 5) gives the potential of the
+    default splitting (eta 0.3, Rc 20, |k| <= 2.5): Ewald sums do not depend on eta once converged."""
+    from pyrepeat_amd import synth
+    s = synth.tiny(seed=4, n_atoms=6, n=24, L=12.0, triclinic=True)
+    a = synth.ewald_esp(s.n_grid, s.axis, s.atom_pos, s.true_charges)
+    b = synth.ewald_esp(s.n_grid, s.axis, s.atom_pos, s.true_charges, **synth.STREAM_SPLIT)
+    assert np.max(np.abs(a - b)) < 1e-7 * max(1.0, np.max(np.abs(a)))
