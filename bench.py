@@ -42,6 +42,36 @@ def _log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
+def calibrate_copy(mib):
+    """One device-to-device copy of `mib` MiB through the HIP runtime, a kernel of known traffic
+    (reads and writes mib MiB) whose FETCH_SIZE / WRITE_SIZE counters scale those of the fit kernels."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    n = int(mib) << 20
+    src, dst = ctypes.c_void_p(), ctypes.c_void_p()
+    for buf in (src, dst):
+        if hip.hipMalloc(ctypes.byref(buf), ctypes.c_size_t(n)) != 0:
+            raise RuntimeError("hipMalloc failed")
+    hip.hipMemset(src, 1, ctypes.c_size_t(n))
+    hip.hipDeviceSynchronize()
+    if hip.hipMemcpy(dst, src, ctypes.c_size_t(n), 3) != 0:      # hipMemcpyDeviceToDevice
+        raise RuntimeError("hipMemcpy failed")
+    hip.hipDeviceSynchronize()
+    hip.hipFree(src)
+    hip.hipFree(dst)
+
+
+def traffic_from_profile(kernel, workload):
+    """HBM bytes per launch of `kernel` from the committed counter summary, if there is one."""
+    path = os.path.join(ROOT, "profiles", f"{kernel}_{workload}_traffic.txt")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch:"):
+            return float(line.split(":", 1)[1])
+    return None
+
+
 def _workload(name):
     from pyrepeat_amd import synth
     if name.startswith("stream"):
@@ -84,7 +114,9 @@ def run_gpu(ctx, system, con, precision, steps, warmup):
     stats = ctx.kernel_stats()
     ctx.profile(False)
     info = fit.info
-    out = {"q": q, "lat": np.array(lat), "total": total, "kernels": stats, "M": int(info.n_keep),
+    M = int(info.n_keep)
+    rows = min(65536, -(-M // 512) * 512) if prec == _abi.RP_PREC_OZAKI else min(65536, -(-M // 128) * 128)
+    out = {"q": q, "lat": np.array(lat), "total": total, "kernels": stats, "M": M, "rows": rows,
            "N": int(fit.n_atoms), "K": int(info.n_k), "timings": fit.timings()}
     fit.close()
     return out
@@ -236,11 +268,15 @@ def main():
     ap.add_argument("--requests", type=int, default=512)
     ap.add_argument("--slots", type=int, default=4)
     ap.add_argument("--budget-gb", type=float, default=200.0)
+    ap.add_argument("--calibrate-copy-mib", type=int, default=0,
+                    help="first run one device copy of this many MiB (counter calibration under rocprofv3 --pmc)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this benchmark runs on one GPU: the fit path is measured on one MI355X; "
                                    "multi-GPU sharding is described, not run"}))
         return 2
+    if args.calibrate_copy_mib:
+        calibrate_copy(args.calibrate_copy_mib)
     if args.serve:
         return serve_main(args)
     import tempfile
@@ -275,21 +311,26 @@ def main():
     fits_s = args.steps / split["total"]
     fits_s64 = bsteps / f64["total"]
     M, N, K = split["M"], split["N"], split["K"]
-    # dominant kernel of the chosen path and its roofline
+    # dominant kernel of the chosen path (the int8 Gram) and its roofline
     kern = split["kernels"]
-    dom = max(("kspace", "gram"), key=lambda k: kern[k][0])
-    ms, launches = kern[dom]
-    per_fit_launches = launches / args.steps
-    if dom == "kspace":
-        alg = 4.0 * M * N * K                       # F_k = 4 M N K per fit
-    else:
-        alg = M * N * (N + 1) + 3.0 * M * N         # F_G per fit
-    achieved = alg / per_fit_launches / (ms / launches * 1e-3) / 1e12
-    roof = {"bound": "mfma", "kernel": "k_oz_kspace" if dom == "kspace" else "k_oz_gram",
-            "achieved": round(achieved, 3), "peak": PEAK_INT8_TOPS, "unit": "TFLOP/s",
-            "frac": round(achieved / PEAK_INT8_TOPS, 5), "traffic": None,
-            "note": "algorithmic fp flops per launch / mean launch time (HIP events); peak = dense int8 "
-                    "MFMA rate, the unit the kernel runs on (16 int8 slice products per fp product)"}
+    ms, launches = kern["gram"]
+    rows = split["rows"]
+    # F_G = M N (N + 1) + 3 M N per fit (Phi^T Phi, Phi^T V, sums); one launch covers `rows` points
+    alg_per_launch = (M * N * (N + 1) + 3.0 * M * N) / (launches / args.steps)
+    t_launch = ms / launches * 1e-3
+    achieved = alg_per_launch / t_launch / 1e12
+    np_ = ((N + 2 + 127) // 128) * 128
+    tiles = sum(1 for bi in range(np_ // 64) for bj in range(np_ // 128) if bi * 64 <= bj * 128 + 127)
+    executed = 2.0 * rows * tiles * 64 * 128 * 16 / t_launch / 1e12      # int8 ops issued, 16 slice products
+    roof = {"bound": "mfma", "kernel": "k_oz_gram", "achieved": round(achieved, 3), "peak": PEAK_INT8_TOPS,
+            "unit": "TFLOP/s", "frac": round(achieved / PEAK_INT8_TOPS, 5),
+            "traffic": traffic_from_profile("k_oz_gram", args.workload),
+            "executed_int8_tops": round(executed, 1), "executed_frac": round(executed / PEAK_INT8_TOPS, 4),
+            "launch_ms": round(ms / launches, 4), "rows_per_launch": rows,
+            "note": "achieved = algorithmic fp64-equivalent flops of the Gram per launch (F_G share of the "
+                    "launch's rows) / mean launch time (HIP events on the fit's stream); peak = dense int8 MFMA, "
+                    "the unit the kernel runs on; each fp product costs 16 int8 slice products, which "
+                    "executed_int8_tops counts"}
     cpu = None
     if not args.no_cpu:
         _log("cpu baseline (oracle sample) ...")

@@ -114,8 +114,10 @@ const char *rp_last_error(const rp_ctx *ctx);
 int rp_ctx_hbm_in_use(const rp_ctx *ctx, uint64_t *bytes);
 
 /* Kernel timing with HIP events on the context's stream (off by default; enabling resets the
- * counters).  kind: 0 k-space contraction, 1 Gram, 2 filter mask, 3 KKT solve, 4 grid phases,
- * 5 Phi slicing.  total_ms sums the launches of that kind since the last reset. */
+ * counters).  kind: 0 basis (k-space + real space), 1 Gram, 2 filter mask, 3 KKT solve,
+ * 4 per-fit tables (grid phases, factorized reciprocal tables, line table), 5 Phi slicing,
+ * 6 reduction of the Gram's row-split partials.  total_ms sums the launches of that kind
+ * since the last reset. */
 int rp_ctx_profile(rp_ctx *ctx, int enable);
 int rp_ctx_kernel_stats(const rp_ctx *ctx, int kind, double *total_ms, int64_t *launches);
 

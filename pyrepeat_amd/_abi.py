@@ -36,7 +36,7 @@ EXPORTED = (
     "rp_fit_reset", "rp_server_create", "rp_server_destroy", "rp_server_error", "rp_server_now",
     "rp_request_bytes", "rp_submit", "rp_poll", "rp_server_stats_get",
 )
-KERNEL_KINDS = {"kspace": 0, "gram": 1, "filter": 2, "solve": 3, "phase": 4, "slice": 5}
+KERNEL_KINDS = {"kspace": 0, "gram": 1, "filter": 2, "solve": 3, "phase": 4, "slice": 5, "reduce": 6}
 
 
 class RepeatHipError(RuntimeError):

@@ -313,7 +313,7 @@ int accumulate_f64(rp_fit *fit, FitDevice &dv) {
                                        n_splits > 1 ? d_part.as<double>() : fit->d_gram.as<double>(), n_splits > 1 ? 0 : 1);
     }
     if (n_splits > 1) {
-        KTimer kt(ctx, KT_GRAM);
+        KTimer kt(ctx, KT_REDUCE);
         k_reduce_splits<<<n_tiles, 256, 0, st>>>(d_part.as<double>(), n_splits, n_tiles, dv.d_tiles.as<int2>(),
                                                  fit->d_gram.as<double>(), NP);
     }

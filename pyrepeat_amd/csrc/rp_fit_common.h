@@ -99,6 +99,42 @@ struct EpiArgs {
     }
 };
 
+// fp32 real-space sum with compensated (Kahan) accumulation; displacements and cutoff test in fp64.
+__device__ __forceinline__ double real_space_f32(const EpiArgs &e, double gx, double gy, double gz, int a, int f) {
+    const double ax = e.atoms[3 * a], ay = e.atoms[3 * a + 1], az = e.atoms[3 * a + 2];
+    if (e.fit_type == 0) {
+        const double dx = gx - ax, dy = gy - ay, dz = gz - az;
+        const double d = sqrt((dx * dx + dy * dy) + dz * dz);
+        return d > 1e-10 ? 1.0 / d : 0.0;
+    }
+    const float sa = (float)e.sqrt_alpha;
+    double sx0, sy0, sz0;
+    if (e.nearest_image(f, a, sx0, sy0, sz0)) {
+        const double dx = gx - (ax + sx0), dy = gy - (ay + sy0), dz = gz - (az + sz0);
+        const double r2 = (dx * dx + dy * dy) + dz * dz;
+        if (!(r2 <= e.R2)) return 0.0;
+        if (sx0 == 0.0 && sy0 == 0.0 && sz0 == 0.0 && r2 <= 1e-20) return 0.0;
+        const float r = sqrtf((float)r2);
+        return (double)(erfcf(sa * r) / r);
+    }
+    float sum = 0.f, comp = 0.f;
+    for (int s = 0; s < e.S; ++s) {
+        const double sx = e.shifts[3 * s], sy = e.shifts[3 * s + 1], sz = e.shifts[3 * s + 2];
+        const double dx = gx - (ax + sx), dy = gy - (ay + sy), dz = gz - (az + sz);
+        const double r2 = (dx * dx + dy * dy) + dz * dz;
+        if (r2 <= e.R2) {
+            if (sx == 0.0 && sy == 0.0 && sz == 0.0 && r2 <= 1e-20) continue;
+            const float r = sqrtf((float)r2);
+            const float term = erfcf(sa * r) / r;
+            const float y = term - comp;
+            const float tt = sum + y;
+            comp = (tt - sum) - y;
+            sum = tt;
+        }
+    }
+    return (double)sum - (double)comp;
+}
+
 // Device copies of the Ewald setup and the chunking plan of one fit.
 struct FitDevice {
     int K = 0, KP = 0, NP = 0, S = 0;
@@ -111,6 +147,9 @@ struct FitDevice {
     EpiArgs epi{};
     DBuf khkl, kcoef, kvecs, ex, ey, ez, shifts, d_tiles;
     std::vector<int2> tiles;
+    // split form: factorized reciprocal sum (k-index box, per-plane partial sums) and the
+    // table of grid lines holding kept points
+    DBuf sep_kbox, sep_t1, line_row, line_key, line_cnt, line_off, chunk_lines;
 };
 
 int prepare_device(rp_fit *fit, FitDevice &dv, int precision);
@@ -118,6 +157,20 @@ int prepare_device(rp_fit *fit, FitDevice &dv, int precision);
 int device_setup(rp_fit *fit, int precision, FitDevice *&dv);
 int accumulate_f64(rp_fit *fit, FitDevice &dv);
 int accumulate_split(rp_fit *fit, FitDevice &dv);
+
+// Factorized reciprocal sum of the split form (rp_kspace.hip).
+struct SepPlan {
+    std::vector<int> first_line, end_line;  // per chunk: lines [first, end) holding its rows
+};
+// Per accumulate: the k-index box, T1 for every plane, and the line table of the kept points.
+int sep_prepare(rp_fit *fit, FitDevice &dv, int64_t rows, SepPlan &plan);
+// Phi rows [r0, r0 + valid) of one chunk (k-space + real space + augmented columns), row-major
+// [rows][NP] fp64, with the column maxima of |Phi| (bit patterns) into colmax.
+void sep_phi_chunk(rp_fit *fit, FitDevice &dv, const SepPlan &plan, int chunk, int64_t r0, int64_t valid,
+                   double *phi, unsigned long long *colmax, hipStream_t st);
+// Row-major Phi chunk -> int8 slices [4][NP][rows] with per-column power-of-two scales.
+void sep_slice_chunk(const double *phi, int NP, int64_t rows, int64_t valid, const unsigned long long *colmax,
+                     int8_t *slices, double *scale_out, hipStream_t st);
 __global__ void k_symmetrize(double *gram, int NP, int tile);
 
 }  // namespace rp

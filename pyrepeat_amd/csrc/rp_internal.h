@@ -20,7 +20,7 @@ void set_global_error(const std::string &msg);
 struct Ctx;
 
 // Kernel classes timed with HIP events when profiling is on (rp_ctx_profile).
-enum { KT_KSPACE = 0, KT_GRAM = 1, KT_FILTER = 2, KT_SOLVE = 3, KT_PHASE = 4, KT_SLICE = 5, KT_N = 6 };
+enum { KT_KSPACE = 0, KT_GRAM = 1, KT_FILTER = 2, KT_SOLVE = 3, KT_PHASE = 4, KT_SLICE = 5, KT_REDUCE = 6, KT_N = 7 };
 
 // Device allocation charged against the owning context's HBM budget.
 struct DBuf {
@@ -152,11 +152,11 @@ inline ChunkPlan plan_chunks(int64_t M, int N, int K, int precision) {
         while ((int64_t)c.n_tiles * c.splits < 512 && c.rows / (c.splits * 2) >= 1024 && (c.rows / (c.splits * 2)) % 64 == 0)
             c.splits *= 2;
     } else {
-        c.KP = K > 0 ? (int)round_up(2 * K, 64) : 0;
+        c.KP = 0;  // the factorized reciprocal sum needs no k-space operand
         int64_t rows = std::min<int64_t>(65536, round_up(M, 512));
         rows = std::max<int64_t>(512, rows);
-        // bound the chunk buffers (G slices, Phi^T, Phi slices) to about 2 GiB
-        while (rows > 512 && rows * (4LL * c.KP + 12LL * c.NP) > ((int64_t)1 << 31)) rows /= 2;
+        // bound the chunk buffers (Phi fp64, Phi slices) to about 2 GiB
+        while (rows > 512 && rows * (12LL * c.NP) > ((int64_t)1 << 31)) rows /= 2;
         c.rows = std::max<int64_t>(512, rows - rows % 512);
         c.n_tiles = 0;
         for (int bi = 0; bi < c.NP / 64; ++bi)
@@ -223,7 +223,7 @@ struct rp_fit {
     std::vector<double> shifts_pruned;
     bool accumulated = false;
     bool lu_fallback = false;     // last solve used LU (A not positive definite)
-    int64_t kspace_products = 0;  // split form: int8 MFMA k-steps x slice products per row/col tile
+    int64_t kspace_products = 0;  // int8 MFMA products of a k-space GEMM (0: the split form factorizes it)
     double t_filter_ms = 0, t_accum_ms = 0, t_solve_ms = 0;
 
     // device state

@@ -1,0 +1,455 @@
+// Reciprocal-space part of the basis, factorized over the lattice axes (split form).
+//
+// The reference sums phi_rec(g, a) = sum_k w_k cos(k . (g - R_a)) over all K k-vectors for every
+// (point, atom) pair (repeat.py:831-846, 887-904; k-set and weights repeat.py:1077-1143).  A grid
+// point is g = (i a0 + j a1) + m a2 and a k-vector has integer indices (h, k, l) over the
+// reciprocal cell, so k . g = 2 pi (h i / nx + k j / ny + l m / nz) and
+//
+//     phi_rec(g, a) = Re sum_{h,k,l} c_a(h,k,l) X(h,i) Y(k,j) Z(l,m),
+//     c_a(h,k,l) = w_hkl e^{-i k . R_a},  X(h,i) = e^{2 pi i h i / nx}  (Y, Z likewise).
+//
+// The triple sum is done as three short sums:
+//     T1[m][a][h][k] = sum_l c_a(h,k,l) Z(l,m)        once per plane m and atom      (13 terms)
+//     T2[a][h]       = sum_k T1[m][a][h][k] Y(k,j)    once per grid line (j, m)       (13 terms)
+//     phi_rec        = Re sum_h T2[a][h] X(h,i)       once per kept point on the line (7 terms)
+// all in fp64, which replaces the K = 754-term sum per (point, atom) by about 20 multiply-adds
+// per (point, atom) plus 52 per (line, atom).  The phase factors come from the integer-reduced
+// tables of the fp64 path, so the result equals the direct sum to fp64 rounding.
+//
+// Phi rows are produced per chunk of kept points (rows in ascending flat index, so a chunk is a
+// run of whole or partial grid lines): one work-group takes up to SEP_LINES consecutive lines and
+// 32 atoms; each wave forms T2 of those lines for 8 atoms (one lane per (atom, h)), then evaluates
+// the points (one lane per (atom, point slot)), adds the real-space term (fp64 displacements and
+// cutoff test, fp32 erfc) and writes Phi row-major.
+#include "rp_fit_common.h"
+
+namespace rp {
+
+namespace {
+
+constexpr int SH = 8;           // h = 0..7 (the k-set uses 0..6)
+constexpr int SK = 13;          // k, l = -6..6
+constexpr int SOFF = 6;
+constexpr int SBOX = SH * SK;   // T1 entries per (plane, atom)
+constexpr int SEP_WAVES = 4;
+constexpr int SEP_ATOMS = 8 * SEP_WAVES;     // atoms per work-group, 8 per wave
+constexpr int SEP_THREADS = 64 * SEP_WAVES;
+constexpr int SEP_LINES = 8;
+constexpr int T1_PLANES = 16;   // planes per T1 work-group
+
+__device__ __forceinline__ double2 cmac(double2 acc, double2 a, double2 b) {
+    acc.x = fma(a.x, b.x, fma(-a.y, b.y, acc.x));
+    acc.y = fma(a.x, b.y, fma(a.y, b.x, acc.y));
+    return acc;
+}
+
+// T1[m][a][h][k] for planes [blockIdx.y * T1_PLANES, +T1_PLANES); block = atom, lane = (h, k).
+__global__ __launch_bounds__(128) void k_sep_t1(int N, int nz, const int *__restrict__ kbox,
+                                                const double *__restrict__ kvecs, const double *__restrict__ kcoef,
+                                                const double *__restrict__ atoms, const double2 *__restrict__ ez,
+                                                double2 *__restrict__ T1) {
+    const int a = blockIdx.x, t = threadIdx.x;
+    if (t >= SBOX) return;
+    const double ax = atoms[3 * a], ay = atoms[3 * a + 1], az = atoms[3 * a + 2];
+    double2 c[SK];
+#pragma unroll
+    for (int ll = 0; ll < SK; ++ll) {
+        const int p = kbox[t * SK + ll];
+        c[ll] = make_double2(0.0, 0.0);
+        if (p >= 0) {
+            // the angle with the rounding of the atom tables: (kx x + ky y) + kz z
+            const double th = (kvecs[3 * p] * ax + kvecs[3 * p + 1] * ay) + kvecs[3 * p + 2] * az;
+            double s, co;
+            sincos(th, &s, &co);
+            const double w = kcoef[p];
+            c[ll] = make_double2(w * co, -(w * s));
+        }
+    }
+    const int m0 = blockIdx.y * T1_PLANES, m1 = min(m0 + T1_PLANES, nz);
+    for (int m = m0; m < m1; ++m) {
+        double2 acc = make_double2(0.0, 0.0);
+#pragma unroll
+        for (int ll = 0; ll < SK; ++ll) acc = cmac(acc, c[ll], ez[(ll - SOFF + 7) * nz + m]);
+        T1[((int64_t)m * N + a) * SBOX + t] = acc;
+    }
+}
+
+// Line starts among the kept points: row r starts a line when its line key f / nx differs from row r-1's.
+__device__ __forceinline__ bool line_start(const int *__restrict__ idx, int64_t r, int64_t M, int nx) {
+    return r < M && (r == 0 || idx[r] / nx != idx[r - 1] / nx);
+}
+
+__global__ __launch_bounds__(256) void k_line_count(const int *__restrict__ idx, int64_t M, int nx, int *__restrict__ cnt) {
+    const int64_t r = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const unsigned long long b = __ballot(line_start(idx, r, M, nx));
+    __shared__ int wsum[4];
+    if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = __popcll(b);
+    __syncthreads();
+    if (threadIdx.x == 0) cnt[blockIdx.x] = wsum[0] + wsum[1] + wsum[2] + wsum[3];
+}
+
+// Exclusive offsets of the per-block counts, total in out[n].
+__global__ __launch_bounds__(1024) void k_line_scan(const int *__restrict__ cnt, int64_t n, int64_t *__restrict__ out) {
+    __shared__ int64_t part[1024];
+    const int64_t per = (n + 1023) / 1024;
+    const int64_t b0 = (int64_t)threadIdx.x * per;
+    const int64_t b1 = min(b0 + per, n);
+    int64_t s = 0;
+    for (int64_t b = b0; b < b1; ++b) s += cnt[b];
+    part[threadIdx.x] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int64_t run = 0;
+        for (int t = 0; t < 1024; ++t) {
+            const int64_t v = part[t];
+            part[t] = run;
+            run += v;
+        }
+        out[n] = run;
+    }
+    __syncthreads();
+    int64_t run = part[threadIdx.x];
+    for (int64_t b = b0; b < b1; ++b) {
+        out[b] = run;
+        run += cnt[b];
+    }
+}
+
+// line_row[L] = first row of line L (line_row[n_lines] = M), line_key[L] = j + m ny.
+__global__ __launch_bounds__(256) void k_line_emit(const int *__restrict__ idx, int64_t M, int nx,
+                                                   const int64_t *__restrict__ off, int64_t nblocks,
+                                                   int *__restrict__ line_row, int *__restrict__ line_key) {
+    const int64_t r = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const bool st = line_start(idx, r, M, nx);
+    const unsigned long long b = __ballot(st);
+    __shared__ int wbase[4];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0) wbase[wave] = __popcll(b);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int run = 0;
+        for (int w = 0; w < 4; ++w) {
+            const int v = wbase[w];
+            wbase[w] = run;
+            run += v;
+        }
+    }
+    __syncthreads();
+    if (st) {
+        const int64_t pos = off[blockIdx.x] + wbase[wave] + __popcll(b & ((1ull << lane) - 1ull));
+        line_row[pos] = (int)r;
+        line_key[pos] = idx[r] / nx;
+    }
+    if (r == M - 1) line_row[off[nblocks]] = (int)M;
+}
+
+// Per chunk c: the lines [first, end) that hold rows [c rows, min(M, (c + 1) rows)).
+__global__ __launch_bounds__(256) void k_chunk_lines(const int *__restrict__ line_row, const int64_t *__restrict__ n_lines_p,
+                                                     int64_t M, int64_t rows, int n_chunks, int *__restrict__ out) {
+    const int c = blockIdx.x * 256 + threadIdx.x;
+    if (c >= n_chunks) return;
+    const int64_t L = *n_lines_p;
+    const int64_t r0 = (int64_t)c * rows, r1 = min(M, r0 + rows) - 1;
+    // last line whose start is <= r
+    auto line_of = [&](int64_t r) {
+        int64_t lo = 0, hi = L - 1;
+        while (lo < hi) {
+            const int64_t mid = (lo + hi + 1) >> 1;
+            if (line_row[mid] <= r) lo = mid; else hi = mid - 1;
+        }
+        return lo;
+    };
+    out[2 * c] = (int)line_of(r0);
+    out[2 * c + 1] = (int)line_of(r1) + 1;
+}
+
+struct SepArgs {
+    const int *line_row, *line_key;
+    int l_first, l_end;
+    int64_t r0, valid;
+    const double2 *T1;          // null when there is no reciprocal part (fit-type 0)
+    int N, NP;
+    const double2 *ex, *ey;
+    EpiArgs e;
+    double *phi;                // [rows][NP], chunk-local rows
+    unsigned long long *colmax;
+    int items, items_per_xcd;   // line groups of the chunk; line groups per XCD
+};
+
+// One wave takes 8 atoms of the work-group's lines; lane = (atom a_l = lane / 8, s = lane % 8).
+// First the wave forms T2 of every line of the work-group (s = h), into a wave-private LDS block;
+// then it streams the work-group's rows, which are contiguous across its lines (s = point slot),
+// each lane following the line its rows fall in.  Waves never wait on each other after the line
+// table is staged.
+__global__ __launch_bounds__(SEP_THREADS) void k_sep_phi(SepArgs p) {
+    __shared__ double2 t2w[SEP_WAVES][SEP_LINES][8][SH];
+    __shared__ int lrow[SEP_LINES + 1], lkey[SEP_LINES];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int a_l = lane >> 3, s = lane & 7;
+    // Work-groups go round-robin to the 8 XCDs (linear id L -> XCD L % 8): XCD x takes line groups
+    // [x ipx, (x + 1) ipx) and walks the atom blocks in order, so its L2 keeps the T1 rows of the
+    // few planes those lines lie in.
+    const int L = blockIdx.x, xcd = L & 7, jj = L >> 3;
+    const int item = xcd * p.items_per_xcd + jj % p.items_per_xcd;
+    const int ablk = jj / p.items_per_xcd;
+    if (item >= p.items) return;
+    const int a = ablk * SEP_ATOMS + wave * 8 + a_l;
+    const EpiArgs &e = p.e;
+    const int nx = e.geom.nx, ny = e.geom.ny;
+    const bool atom = a < p.N;
+    const bool recip = atom && p.T1 != nullptr;
+    const int l_begin = p.l_first + item * SEP_LINES;
+    const int n_l = min(SEP_LINES, p.l_end - l_begin);
+    if ((int)threadIdx.x <= n_l) lrow[threadIdx.x] = p.line_row[l_begin + threadIdx.x];
+    if ((int)threadIdx.x < n_l) lkey[threadIdx.x] = p.line_key[l_begin + threadIdx.x];
+    __syncthreads();
+    // ---- T2 of every line (lane s = h)
+    if (recip) {
+        double2 t1[SK];
+        int cur_m = -1;
+        for (int li = 0; li < n_l; ++li) {
+            const int key = lkey[li];
+            const int m = key / ny, j = key - m * ny;
+            if (m != cur_m) {
+                cur_m = m;
+                const double2 *src = p.T1 + ((int64_t)m * p.N + a) * SBOX + s * SK;
+#pragma unroll
+                for (int kk = 0; kk < SK; ++kk) t1[kk] = src[kk];
+            }
+            double2 t2 = make_double2(0.0, 0.0);
+#pragma unroll
+            for (int kk = 0; kk < SK; ++kk) t2 = cmac(t2, t1[kk], p.ey[(kk - SOFF + 7) * ny + j]);
+            t2w[wave][li][a_l][s] = t2;
+        }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    // ---- points (lane s = point slot)
+    double Rx = 0.0, Ry = 0.0, Rz = 0.0, Fx = 0.0, Fy = 0.0, Fz = 0.0;
+    if (atom) {
+        Rx = e.atoms[3 * a];
+        Ry = e.atoms[3 * a + 1];
+        Rz = e.atoms[3 * a + 2];
+        Fx = e.afrac[3 * a];
+        Fy = e.afrac[3 * a + 1];
+        Fz = e.afrac[3 * a + 2];
+    }
+    const float sa = (float)e.sqrt_alpha;
+    const int64_t rs = max<int64_t>(lrow[0], p.r0);
+    const int64_t rend = min<int64_t>(lrow[n_l], p.r0 + p.valid);
+    double cmax = 0.0;
+    int li = 0, key = 0, j = 0, m = 0;
+    for (int64_t r = rs + s; r < rend; r += SH) {
+        if (r >= lrow[li + 1] || r == rs + s) {
+            while (r >= lrow[li + 1]) ++li;
+            key = lkey[li];
+            m = key / ny;
+            j = key - m * ny;
+        }
+        const int f = e.idx[r];
+        const int i = f - key * nx;
+        double val = 0.0;
+        if (atom) {
+            double acc = 0.0;
+            if (recip) {
+#pragma unroll
+                for (int q = 0; q < SH; ++q) {
+                    const double2 x = p.ex[q * nx + i];
+                    const double2 t = t2w[wave][li][a_l][q];
+                    acc = fma(t.x, x.x, fma(-t.y, x.y, acc));
+                }
+            }
+            const double id = (double)i, jd = (double)j, md = (double)m;
+            double real = 0.0;
+            if (e.fit_type == 0) {
+                const double gx = (id * e.geom.ax[0] + jd * e.geom.ax[3]) + md * e.geom.ax[6];
+                const double gy = (id * e.geom.ax[1] + jd * e.geom.ax[4]) + md * e.geom.ax[7];
+                const double gz = (id * e.geom.ax[2] + jd * e.geom.ax[5]) + md * e.geom.ax[8];
+                const double dx = gx - Rx, dy = gy - Ry, dz = gz - Rz;
+                const double d = sqrt((dx * dx + dy * dy) + dz * dz);
+                real = d > 1e-10 ? 1.0 / d : 0.0;
+            } else {
+                // nearest image n = rint(g_f - a_f); only it can be within Rc when the distance bound
+                // min_i (1 - |w_i|) h_i to every other image exceeds Rc (w = g_f - a_f - n)
+                const double d0 = id * e.inv_n[0] - Fx, d1 = jd * e.inv_n[1] - Fy, d2 = md * e.inv_n[2] - Fz;
+                const double n0 = rint(d0), n1 = rint(d1), n2 = rint(d2);
+                const double w0 = d0 - n0, w1 = d1 - n1, w2 = d2 - n2;
+                const double bnd = fmin(fmin((1.0 - fabs(w0)) * e.h[0], (1.0 - fabs(w1)) * e.h[1]),
+                                        (1.0 - fabs(w2)) * e.h[2]);
+                if (bnd > e.r_cut) {
+                    // displacement to that image, w . cell
+                    const double dx = fma(w0, e.box[0], fma(w1, e.box[3], w2 * e.box[6]));
+                    const double dy = fma(w0, e.box[1], fma(w1, e.box[4], w2 * e.box[7]));
+                    const double dz = fma(w0, e.box[2], fma(w1, e.box[5], w2 * e.box[8]));
+                    const double r2 = fma(dx, dx, fma(dy, dy, dz * dz));
+                    if (r2 <= e.R2 && !(n0 == 0.0 && n1 == 0.0 && n2 == 0.0 && r2 <= 1e-20)) {
+                        const float rr = sqrtf((float)r2);
+                        real = (double)(erfcf(sa * rr) / rr);
+                    }
+                } else {
+                    // several images within Rc: the shift-list sum with the reference roundings
+                    const double gx = (id * e.geom.ax[0] + jd * e.geom.ax[3]) + md * e.geom.ax[6];
+                    const double gy = (id * e.geom.ax[1] + jd * e.geom.ax[4]) + md * e.geom.ax[7];
+                    const double gz = (id * e.geom.ax[2] + jd * e.geom.ax[5]) + md * e.geom.ax[8];
+                    float sum = 0.f, comp = 0.f;
+                    for (int t = 0; t < e.S; ++t) {
+                        const double sx = e.shifts[3 * t], sy = e.shifts[3 * t + 1], sz = e.shifts[3 * t + 2];
+                        const double dx = gx - (Rx + sx), dy = gy - (Ry + sy), dz = gz - (Rz + sz);
+                        const double r2 = (dx * dx + dy * dy) + dz * dz;
+                        if (r2 <= e.R2 && !(sx == 0.0 && sy == 0.0 && sz == 0.0 && r2 <= 1e-20)) {
+                            const float rr = sqrtf((float)r2);
+                            const float term = erfcf(sa * rr) / rr;
+                            const float y = term - comp;
+                            const float tsum = sum + y;
+                            comp = (tsum - sum) - y;
+                            sum = tsum;
+                        }
+                    }
+                    real = (double)sum - (double)comp;
+                }
+            }
+            val = acc + real;
+        } else if (a == p.N) {
+            val = e.v[r];
+        } else if (a == p.N + 1) {
+            val = 1.0;
+        }
+        p.phi[(r - p.r0) * p.NP + a] = val;
+        cmax = fmax(cmax, fabs(val));
+    }
+    // column maximum over the 8 point slots of the atom
+    cmax = fmax(cmax, __shfl_xor(cmax, 1, 64));
+    cmax = fmax(cmax, __shfl_xor(cmax, 2, 64));
+    cmax = fmax(cmax, __shfl_xor(cmax, 4, 64));
+    if (s == 0 && cmax > 0.0 && a < p.NP) atomicMax(p.colmax + a, (unsigned long long)__double_as_longlong(cmax));
+}
+
+// Row-major Phi tile (64 rows x 64 columns) -> slices [4][NP][rows] of x / S_col, S_col the power
+// of two with max |Phi_col| / S_col <= 127/128; rows at or past `valid` are zero.
+__global__ __launch_bounds__(256) void k_slice_rows(const double *__restrict__ phi, int NP, int64_t rows, int64_t valid,
+                                                    const unsigned long long *__restrict__ colmax,
+                                                    int8_t *__restrict__ out, double *__restrict__ scale_out) {
+    __shared__ double tile[64][65];
+    const int t = threadIdx.x;
+    const int64_t rt = (int64_t)blockIdx.x * 64;
+    const int c0 = blockIdx.y * 64;
+#pragma unroll
+    for (int e = 0; e < 16; ++e) {
+        const int q = e * 256 + t, row = q >> 6, col = q & 63;
+        tile[row][col] = (rt + row < valid) ? phi[(rt + row) * NP + c0 + col] : 0.0;
+    }
+    __syncthreads();
+    const int col = t >> 2, part = t & 3;
+    const int a = c0 + col;
+    const double mx = __longlong_as_double((long long)colmax[a]);
+    double S = 1.0;
+    if (mx > 0.0) S = ldexp(1.0, (int)ceil(log2(mx * (128.0 / 127.0))));
+    while (mx / S > 127.0 / 128.0) S *= 2.0;
+    if (blockIdx.x == 0 && part == 0) scale_out[a] = S;
+    const double inv = 1.0 / S;
+    uint32_t w[4][4];
+#pragma unroll
+    for (int b = 0; b < 4; ++b)
+#pragma unroll
+        for (int s = 0; s < 4; ++s) w[s][b] = 0u;
+#pragma unroll
+    for (int rr = 0; rr < 16; ++rr) {
+        double r = tile[part * 16 + rr][col] * inv;
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+            const double tv = r * 128.0;
+            const double c = rint(tv);
+            r = tv - c;
+            w[s][rr >> 2] |= (uint32_t)(uint8_t)(int8_t)(int)c << (8 * (rr & 3));
+        }
+    }
+    const int64_t plane = (int64_t)NP * rows;
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+        uint4 v = make_uint4(w[s][0], w[s][1], w[s][2], w[s][3]);
+        *reinterpret_cast<uint4 *>(out + s * plane + (int64_t)a * rows + rt + part * 16) = v;
+    }
+}
+
+}  // namespace
+
+int sep_prepare(rp_fit *fit, FitDevice &dv, int64_t rows, SepPlan &plan) {
+    Ctx *ctx = fit->ctx;
+    hipStream_t st = ctx->stream;
+    const Setup &s = fit->setup;
+    const int N = fit->n_atoms, K = dv.K;
+    const int64_t M = fit->n_keep;
+    if (K > 0) {
+        std::vector<int> kbox(SH * SK * SK, -1);
+        for (int p = 0; p < K; ++p) {
+            const int h = s.khkl[3 * p], k = s.khkl[3 * p + 1] + SOFF, l = s.khkl[3 * p + 2] + SOFF;
+            if (h < 0 || h >= SH || k < 0 || k >= SK || l < 0 || l >= SK)
+                return ctx->fail(RP_EINVAL, "k-vector index outside the factorized box");
+            kbox[(h * SK + k) * SK + l] = p;
+        }
+        RP_CHECK(dv.sep_kbox.grow(ctx, sizeof(int) * kbox.size()));
+        RP_HIP(ctx, hipMemcpyAsync(dv.sep_kbox.p, kbox.data(), sizeof(int) * kbox.size(), hipMemcpyHostToDevice, st));
+        RP_CHECK(dv.sep_t1.grow(ctx, sizeof(double2) * (size_t)s.nz * N * SBOX));
+        dim3 g(N, (unsigned)ceil_div(s.nz, T1_PLANES));
+        k_sep_t1<<<g, 128, 0, st>>>(N, s.nz, dv.sep_kbox.as<int>(), dv.kvecs.as<double>(), dv.kcoef.as<double>(),
+                                    fit->d_atoms.as<double>(), dv.ez.as<double2>(), dv.sep_t1.as<double2>());
+    }
+    // line table of the kept points
+    const int64_t nb = ceil_div(M, 256);
+    RP_CHECK(dv.line_cnt.grow(ctx, sizeof(int) * nb));
+    RP_CHECK(dv.line_off.grow(ctx, sizeof(int64_t) * (nb + 1)));
+    RP_CHECK(dv.line_row.grow(ctx, sizeof(int) * (M + 1)));
+    RP_CHECK(dv.line_key.grow(ctx, sizeof(int) * M));
+    k_line_count<<<nb, 256, 0, st>>>(fit->d_idx.as<int>(), M, s.nx, dv.line_cnt.as<int>());
+    k_line_scan<<<1, 1024, 0, st>>>(dv.line_cnt.as<int>(), nb, dv.line_off.as<int64_t>());
+    k_line_emit<<<nb, 256, 0, st>>>(fit->d_idx.as<int>(), M, s.nx, dv.line_off.as<int64_t>(), nb, dv.line_row.as<int>(),
+                                    dv.line_key.as<int>());
+    const int n_chunks = (int)ceil_div(M, rows);
+    RP_CHECK(dv.chunk_lines.grow(ctx, sizeof(int) * 2 * n_chunks));
+    k_chunk_lines<<<ceil_div(n_chunks, 256), 256, 0, st>>>(dv.line_row.as<int>(), dv.line_off.as<int64_t>() + nb, M, rows,
+                                                           n_chunks, dv.chunk_lines.as<int>());
+    RP_HIP(ctx, hipGetLastError());
+    std::vector<int> cl(2 * n_chunks);
+    RP_HIP(ctx, hipMemcpyAsync(cl.data(), dv.chunk_lines.p, sizeof(int) * cl.size(), hipMemcpyDeviceToHost, st));
+    RP_HIP(ctx, hipStreamSynchronize(st));
+    plan.first_line.resize(n_chunks);
+    plan.end_line.resize(n_chunks);
+    for (int c = 0; c < n_chunks; ++c) {
+        plan.first_line[c] = cl[2 * c];
+        plan.end_line[c] = cl[2 * c + 1];
+        if (plan.end_line[c] <= plan.first_line[c]) return ctx->fail(RP_EINVAL, "empty chunk in the line table");
+    }
+    return RP_OK;
+}
+
+void sep_phi_chunk(rp_fit *fit, FitDevice &dv, const SepPlan &plan, int chunk, int64_t r0, int64_t valid, double *phi,
+                   unsigned long long *colmax, hipStream_t st) {
+    SepArgs a{};
+    a.line_row = dv.line_row.as<int>();
+    a.line_key = dv.line_key.as<int>();
+    a.l_first = plan.first_line[chunk];
+    a.l_end = plan.end_line[chunk];
+    a.r0 = r0;
+    a.valid = valid;
+    a.T1 = dv.K > 0 ? dv.sep_t1.as<double2>() : nullptr;
+    a.N = fit->n_atoms;
+    a.NP = dv.NP;
+    a.ex = dv.ex.as<double2>();
+    a.ey = dv.ey.as<double2>();
+    a.e = dv.epi;
+    a.phi = phi;
+    a.colmax = colmax;
+    a.items = (int)ceil_div(a.l_end - a.l_first, SEP_LINES);
+    a.items_per_xcd = (int)ceil_div(a.items, 8);
+    const int blocks = 8 * a.items_per_xcd * (dv.NP / SEP_ATOMS);
+    k_sep_phi<<<blocks, SEP_THREADS, 0, st>>>(a);
+}
+
+void sep_slice_chunk(const double *phi, int NP, int64_t rows, int64_t valid, const unsigned long long *colmax,
+                     int8_t *slices, double *scale_out, hipStream_t st) {
+    dim3 g((unsigned)(rows / 64), NP / 64);
+    k_slice_rows<<<g, 256, 0, st>>>(phi, NP, rows, valid, colmax, slices, scale_out);
+}
+
+}  // namespace rp

@@ -93,7 +93,7 @@ int validate_request(const rp_request *r, std::string &err) {
 }  // namespace
 
 // Upper bound of the device bytes one fit of this request allocates: vdW filter, Ewald tables,
-// the chunk buffers of its precision, the fp64 matvec of --stats, Gram and solve, with the kept
+// the chunk buffers of its precision (and the factorized reciprocal tables of the split form), the fp64 matvec of --stats, Gram and solve, with the kept
 // count bounded by the grid size.  The fit's grow-only workspaces are shared between phases,
 // so each is bounded by the largest size any phase asks of it.
 uint64_t fit_bytes_bound(const Setup &s, int n_atoms, int esp_dtype, int precision, int n_reduced, int n_src) {
@@ -118,20 +118,20 @@ uint64_t fit_bytes_bound(const Setup &s, int n_atoms, int esp_dtype, int precisi
     if (precision == RP_PREC_FP64) at(3, 8 * split_cap(c64.rows, c64.n_tiles) * c64.n_tiles * 128 * 128);
     at(22, 8 * N);
     at(23, 8 * M);
+    uint64_t sep = 0;
     if (precision != RP_PREC_FP64) {
         const ChunkPlan c = plan_chunks(M, n_atoms, (int)K, RP_PREC_OZAKI);
-        at(0, 4 * (uint64_t)c.rows * c.KP);
-        at(1, 4 * (uint64_t)c.NP * c.KP);
         at(2, 8 * (uint64_t)c.NP * c.rows);
         at(3, 8 * split_cap(c.rows, c.n_tiles) * c.n_tiles * 64 * 128);
         at(4, 4 * (uint64_t)c.NP * c.rows);
         at(5, 8 * (uint64_t)c.NP);
         at(6, 8 * (uint64_t)c.NP);
-        at(7, 8 * K);
-        at(8, 4 * K);
         at(9, 8 * (uint64_t)c.n_tiles);
-        at(10, 12 * K);
-        at(11, 24 * K);
+        // factorized reciprocal sum: k box, T1 per plane (8 x 13 complex per plane and atom),
+        // line table of the kept points, per-chunk line ranges
+        const uint64_t nb = (uint64_t)ceil_div(M, 256), n_chunks = (uint64_t)ceil_div(M, c.rows);
+        sep += 4 * 8 * 13 * 13 + (K > 0 ? 16 * 8 * 13 * (uint64_t)s.nz * N : 0);
+        sep += 4 * nb + 8 * (nb + 1) + 4 * (M + 1) + 4 * M + 8 * n_chunks + 16 * 7;
     }
     // bordered solve
     const uint64_t nr = n_reduced, n = nr + 1, ld = (uint64_t)round_up((int64_t)n + 1, 4);
@@ -150,7 +150,8 @@ uint64_t fit_bytes_bound(const Setup &s, int n_atoms, int esp_dtype, int precisi
     at(27, 8 * N);
     at(28, 4);
     at(29, 8 * nr);
-    uint64_t b = filter_bytes_bound(s, esp_dtype);
+    at(30, 8 * 64 * 64 * (uint64_t)ceil_div((int64_t)nr, 64));
+    uint64_t b = filter_bytes_bound(s, esp_dtype) + sep;
     for (uint64_t w : ws) b += w;
     // device copy of the Ewald setup, atoms, augmented Gram
     b += 16 * (8 + 15 + 15) + 12 * K + 8 * K + 24 * K + 24 * N + 24 * S + 8 * (uint64_t)c64.n_tiles + 16 * 8;

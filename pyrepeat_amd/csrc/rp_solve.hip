@@ -14,9 +14,10 @@
 //   pivoting instead; an exactly zero LU pivot is reported as RP_ESINGULAR, where the
 //   reference falls back to lstsq.
 //
-// Cholesky: right-looking, 64-column blocks: diagonal block factored in LDS by one
-// workgroup, the panel below by a triangular solve, the trailing lower triangle by a
-// rank-64 update.  The two right-hand sides are solved by one workgroup, blocked.
+// Cholesky: right-looking, 64-column blocks: the diagonal block is factored in LDS (8 x 8
+// sub-blocks) together with its inverse; the panel below is the product with that inverse, the
+// trailing lower triangle takes a rank-64 update.  The forward substitution of w and b rides
+// along with the factorisation; the multiplier and one backward substitution finish the solve.
 //
 // LU fallback: the augmented matrix [K | rhs] (n x (n+1), n = N'+1) is factored in place,
 // right-looking and blocked by NB columns: one-workgroup panel factorisation,
@@ -219,77 +220,222 @@ __global__ __launch_bounds__(256) void k_update(double *__restrict__ A, int ld, 
 
 constexpr int CB = 64;
 
-// Uniform broadcast of lane l's double (l a compile-time constant after unrolling: v_readlane).
-__device__ __forceinline__ double readlane_d(double x, int l) {
-    const long long v = __double_as_longlong(x);
-    const int lo = __builtin_amdgcn_readlane((int)(v & 0xffffffffll), l);
-    const int hi = __builtin_amdgcn_readlane((int)(v >> 32), l);
-    return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+__device__ __forceinline__ void wave_sync_lds() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// Cholesky of one 64x64 diagonal block by a single wave, lane t holding row t in registers.
-// A pivot at or below 1e-13 of the original diagonal entry counts as a failure: the system is then
-// left to the pivoted LU, which reports exact singularity as the reference's LAPACK call does.
-__global__ __launch_bounds__(64) void k_potrf_diag(double *__restrict__ A, int ld, int n, int k0,
-                                                   const double *__restrict__ orig, int *__restrict__ info) {
+// Diagonal block of the blocked Cholesky (64 x 64, 256 threads, in LDS): factor L, its inverse
+// X = L^-1 (written to Dinv, row-major, identity past the block) and the forward-substitution step
+// of the two right-hand sides, z_k = X r_k (rows k0.. of rw, rb, in place).  Inside the block the
+// factorisation runs over 8 x 8 sub-blocks: wave 0 factors and inverts each diagonal sub-block,
+// then all threads form the sub-panel below it (times the sub-block inverse), update the rest of
+// the block, and advance the right-looking inverse: X starts as the identity (B), block row K
+// becomes X_KK B_K once final, and the rows below take B_I -= L_IK X_K.
+// A pivot at or below 1e-13 of the original diagonal entry counts as a failure: the system is
+// then left to the pivoted LU, which reports exact singularity as the reference's LAPACK call does.
+constexpr int SB = 8;                        // sub-block
+constexpr size_t POTRF_LDS = (2 * CB * (CB + 1) + 2 * CB) * sizeof(double);
+
+__global__ __launch_bounds__(256) void k_potrf_inv(double *__restrict__ A, int ld, int n, int k0,
+                                                   const double *__restrict__ orig, int *__restrict__ info,
+                                                   double *__restrict__ Dinv, double *__restrict__ rw,
+                                                   double *__restrict__ rb) {
+    extern __shared__ double sm[];
+    double(*L)[CB + 1] = reinterpret_cast<double(*)[CB + 1]>(sm);
+    double(*X)[CB + 1] = reinterpret_cast<double(*)[CB + 1]>(sm + CB * (CB + 1));
+    double *zw = sm + 2 * CB * (CB + 1);
+    double *zb = zw + CB;
     const int nb = min(CB, n - k0);
-    const int t = threadIdx.x;
-    double r[CB];
-#pragma unroll
-    for (int c = 0; c < CB; ++c) r[c] = (t < nb && c <= t) ? A[(int64_t)(k0 + t) * ld + k0 + c] : 0.0;
-    int bad = 0;
-#pragma unroll
-    for (int j = 0; j < CB; ++j) {
-        if (j < nb) {
-            const double d = readlane_d(r[j], j);
-            const double od = orig[(int64_t)(k0 + j) * ld + k0 + j];
-            double piv;
-            if (!(d > 1e-13 * fabs(od))) {
-                if (!bad) bad = k0 + j + 1;
-                piv = 1.0;
-            } else {
-                piv = sqrt(d);
-            }
-            if (t == j)
-                r[j] = piv;
-            else if (t > j)
-                r[j] = r[j] / piv;
-            const double ltj = r[j];
-#pragma unroll
-            for (int c = j + 1; c < CB; ++c) {
-                if (c < nb) {
-                    const double lcj = readlane_d(r[j], c);
-                    if (t >= c) r[c] -= ltj * lcj;
-                }
-            }
-        }
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    #pragma unroll
+    for (int e = t; e < CB * CB; e += 256) {
+        const int i = e / CB, c = e % CB;
+        L[i][c] = (i < nb && c <= i) ? A[(int64_t)(k0 + i) * ld + k0 + c] : (i == c ? 1.0 : 0.0);
+        X[i][c] = i == c ? 1.0 : 0.0;
     }
-#pragma unroll
-    for (int c = 0; c < CB; ++c)
-        if (t < nb && c <= t) A[(int64_t)(k0 + t) * ld + k0 + c] = r[c];
-    if (t == 0 && bad && info[0] == 0) info[0] = bad;
-}
-
-// L21 = A21 L11^{-T}: one thread per row below the block.
-__global__ __launch_bounds__(256) void k_chol_trsm(double *__restrict__ A, int ld, int n, int k0) {
-    __shared__ double L[CB][CB + 1];
-    const int nb = min(CB, n - k0);
-    for (int e = threadIdx.x; e < nb * nb; e += 256) {
-        const int i = e / nb, j = e % nb;
-        L[i][j] = j <= i ? A[(int64_t)(k0 + i) * ld + k0 + j] : 0.0;
+    __shared__ double od[CB];   // original diagonal entries for the pivot test, fetched up front
+    if (t < CB) {
+        zw[t] = t < nb ? rw[k0 + t] : 0.0;
+        zb[t] = t < nb ? rb[k0 + t] : 0.0;
+        od[t] = t < nb ? orig[(int64_t)(k0 + t) * ld + k0 + t] : 1.0;
     }
     __syncthreads();
-    const int r = k0 + nb + blockIdx.x * 256 + threadIdx.x;
-    if (r >= n) return;
-    double x[CB];
-    double *row = A + (int64_t)r * ld + k0;
+    int bad = 0;
+    for (int kb = 0; kb < CB; kb += SB) {
+        // ---- diagonal sub-block: factor and invert (wave 0, lane = (r, c) of the 8 x 8)
+        if (wave == 0) {
+            const int r = lane >> 3, c = lane & 7;
+            for (int j = 0; j < SB; ++j) {
+                const double d = L[kb + j][kb + j];
+                double piv = 1.0;
+                if (kb + j < nb) {
+                    if (!(d > 1e-13 * fabs(od[kb + j]))) {
+                        if (!bad) bad = k0 + kb + j + 1;
+                    } else {
+                        piv = sqrt(d);
+                    }
+                }
+                wave_sync_lds();
+                if (r == j && c == j) L[kb + j][kb + j] = piv;
+                if (r > j && c == j) L[kb + r][kb + j] = L[kb + r][kb + j] / piv;
+                wave_sync_lds();
+                if (r > j && c > j && c <= r) L[kb + r][kb + c] -= L[kb + r][kb + j] * L[kb + c][kb + j];
+                wave_sync_lds();
+            }
+            for (int i = 0; i < SB; ++i) {
+                if (r == i && c <= i) {
+                    double x;
+                    if (c == i) {
+                        x = 1.0 / L[kb + i][kb + i];
+                    } else {
+                        double acc = 0.0;
+                        for (int m = c; m < i; ++m) acc += L[kb + i][kb + m] * X[kb + m][kb + c];
+                        x = -acc / L[kb + i][kb + i];
+                    }
+                    X[kb + i][kb + c] = x;
+                }
+                wave_sync_lds();
+            }
+        }
+        __syncthreads();
+        // ---- sub-panel below: L[i][kb..kb+8) <- A[i][kb..kb+8) X_kk^T
+        const int below = CB - (kb + SB);
+        double pv[2];
+        int pi[2], pc[2];
 #pragma unroll
-    for (int j = 0; j < CB; ++j) {
-        if (j < nb) {
-            double v = row[j];
-            for (int q = 0; q < j; ++q) v -= x[q] * L[j][q];
-            x[j] = v / L[j][j];
-            row[j] = x[j];
+        for (int u = 0; u < 2; ++u) {
+            const int e = t + 256 * u;
+            pi[u] = -1;
+            if (e < below * SB) {
+                pi[u] = kb + SB + e / SB;
+                pc[u] = e % SB;
+                double acc = 0.0;
+                for (int m = 0; m <= pc[u]; ++m) acc += L[pi[u]][kb + m] * X[kb + pc[u]][kb + m];
+                pv[u] = acc;
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int u = 0; u < 2; ++u)
+            if (pi[u] >= 0) L[pi[u]][kb + pc[u]] = pv[u];
+        __syncthreads();
+        // ---- rest of the block: L[i][c] -= sum_m L[i][kb+m] L[c][kb+m], kb+8 <= c <= i; and the
+        // inverse's block row kb: X[kb+r][c] = sum_m X_kk[r][m] B[kb+m][c] for c < kb (B in X)
+        for (int e = t; e < below * below; e += 256) {
+            const int i = kb + SB + e / below, c = kb + SB + e % below;
+            if (c <= i) {
+                double acc = 0.0;
+#pragma unroll
+                for (int m = 0; m < SB; ++m) acc += L[i][kb + m] * L[c][kb + m];
+                L[i][c] -= acc;
+            }
+        }
+        double xv[2];
+        int xr[2], xc[2];
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+            const int e = t + 256 * u;
+            xr[u] = -1;
+            if (e < SB * kb) {
+                xr[u] = e / kb;
+                xc[u] = e % kb;
+                double acc = 0.0;
+                for (int m = 0; m <= xr[u]; ++m) acc += X[kb + xr[u]][kb + m] * X[kb + m][xc[u]];
+                xv[u] = acc;
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int u = 0; u < 2; ++u)
+            if (xr[u] >= 0) X[kb + xr[u]][xc[u]] = xv[u];
+        __syncthreads();
+        // B[i][c] -= sum_m L[i][kb+m] X[kb+m][c] for rows i below the sub-block, columns c < kb + 8
+        for (int e = t; e < below * (kb + SB); e += 256) {
+            const int i = kb + SB + e / (kb + SB), c = e % (kb + SB);
+            double acc = 0.0;
+#pragma unroll
+            for (int m = 0; m < SB; ++m) acc += L[i][kb + m] * X[kb + m][c];
+            X[i][c] -= acc;
+        }
+        __syncthreads();
+    }
+    // ---- outputs: factor, inverse, z_k = X r_k
+    #pragma unroll
+    for (int e = t; e < CB * CB; e += 256) {
+        const int i = e / CB, c = e % CB;
+        Dinv[e] = c <= i ? X[i][c] : 0.0;
+        if (i < nb && c <= i) A[(int64_t)(k0 + i) * ld + k0 + c] = L[i][c];
+    }
+    if (t < nb) {
+        double a = 0.0, b = 0.0;
+        for (int c = 0; c <= t; ++c) {
+            a += X[t][c] * zw[c];
+            b += X[t][c] * zb[c];
+        }
+        rw[k0 + t] = a;
+        rb[k0 + t] = b;
+    }
+    if (wave == 0 && lane == 0 && bad && info[0] == 0) info[0] = bad;
+}
+
+// Panel below the diagonal block: L21 = A21 L11^{-T} = A21 Dinv^T, 32 rows per workgroup, and the
+// forward-substitution update of those rows of the right-hand sides, r_i -= L21[i] . z_k.
+__global__ __launch_bounds__(256) void k_chol_trsm_inv(double *__restrict__ A, int ld, int n, int k0,
+                                                       const double *__restrict__ Dinv, double *__restrict__ rw,
+                                                       double *__restrict__ rb) {
+    __shared__ double Xs[CB][CB + 1];
+    __shared__ double As[32][CB + 1];
+    __shared__ double z[2][CB];
+    const int nb = min(CB, n - k0);
+    const int r0 = k0 + nb + blockIdx.x * 32;
+    const int t = threadIdx.x;
+    #pragma unroll
+    for (int e = t; e < CB * CB; e += 256) Xs[e / CB][e % CB] = Dinv[e];
+    #pragma unroll
+    for (int e = t; e < 32 * CB; e += 256) {
+        const int i = e / CB, c = e % CB;
+        As[i][c] = (r0 + i < n && c < nb) ? A[(int64_t)(r0 + i) * ld + k0 + c] : 0.0;
+    }
+    if (t < CB) {
+        z[0][t] = t < nb ? rw[k0 + t] : 0.0;
+        z[1][t] = t < nb ? rb[k0 + t] : 0.0;
+    }
+    __syncthreads();
+    const int ty = t / 16, tx = t % 16;
+    double acc[2][4] = {};
+    for (int q = 0; q < CB; ++q) {
+        double l[2], u[4];
+#pragma unroll
+        for (int a = 0; a < 2; ++a) l[a] = As[ty * 2 + a][q];
+#pragma unroll
+        for (int b = 0; b < 4; ++b) u[b] = Xs[tx + 16 * b][q];
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+            for (int b = 0; b < 4; ++b) acc[a][b] += l[a] * u[b];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int a = 0; a < 2; ++a) {
+        const int r = r0 + ty * 2 + a;
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            const int c = tx + 16 * b;
+            As[ty * 2 + a][c] = c < nb ? acc[a][b] : 0.0;
+            if (r < n && c < nb) A[(int64_t)r * ld + k0 + c] = acc[a][b];
+        }
+    }
+    __syncthreads();
+    if (t < 64) {
+        const int i = t & 31, which = t >> 5;
+        const int r = r0 + i;
+        if (r < n) {
+            double s = 0.0;
+            for (int c = 0; c < nb; ++c) s += As[i][c] * z[which][c];
+            double *v = which ? rb : rw;
+            v[r] -= s;
         }
     }
 }
@@ -304,6 +450,7 @@ __global__ __launch_bounds__(256) void k_chol_update(double *__restrict__ A, int
     __shared__ double Us[CB][64];
     const int tr = r0 + ti * 64, tc = r0 + tj * 64;
     const int t = threadIdx.x;
+    #pragma unroll
     for (int e = t; e < 64 * CB; e += 256) {
         const int r = e / CB, q = e % CB;
         Ls[r][q] = (tr + r < n && q < nb) ? A[(int64_t)(tr + r) * ld + k0 + q] : 0.0;
@@ -335,111 +482,29 @@ __global__ __launch_bounds__(256) void k_chol_update(double *__restrict__ A, int
     }
 }
 
-// Solves L L^T Y = [w b] (columns cw, cb of A), then the Schur step for the Lagrange row.
-// One workgroup; the right-hand sides live in global memory (L1/L2 coherent within the block).
-__global__ __launch_bounds__(256) void k_chol_solve(const double *__restrict__ A, int ld, int n, const double *__restrict__ w,
-                                                      const double *__restrict__ b, double q_tot, double *__restrict__ yw,
-                                                      double *__restrict__ yb, double *__restrict__ x) {
-    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
-    for (int r = t; r < n; r += 256) {
-        yw[r] = w[r];
-        yb[r] = b[r];
-    }
-    __syncthreads();
-    // forward: L z = rhs, 64-row blocks
-    for (int j0 = 0; j0 < n; j0 += CB) {
-        const int nb = min(CB, n - j0);
-        if (wave == 0) {
-            double zw = lane < nb ? yw[j0 + lane] : 0.0, zb = lane < nb ? yb[j0 + lane] : 0.0;
-            // lane's row of the diagonal block in registers (fully unrolled loop keeps it there)
-            double lr[CB];
-#pragma unroll
-            for (int j = 0; j < CB; ++j) lr[j] = (lane < nb && j < nb) ? A[(int64_t)(j0 + lane) * ld + j0 + j] : 0.0;
-#pragma unroll
-            for (int j = 0; j < CB; ++j) {
-                if (j < nb) {
-                    const double d = readlane_d(lr[j], j);
-                    const double xw = readlane_d(zw, j) / d, xb = readlane_d(zb, j) / d;
-                    if (lane == j) {
-                        zw = xw;
-                        zb = xb;
-                    } else if (lane > j) {
-                        zw -= lr[j] * xw;
-                        zb -= lr[j] * xb;
-                    }
-                }
-            }
-            if (lane < nb) {
-                yw[j0 + lane] = zw;
-                yb[j0 + lane] = zb;
-            }
-        }
-        __syncthreads();
-        for (int r = j0 + nb + t; r < n; r += 256) {
-            double sw = 0.0, sb = 0.0;
-            const double *row = A + (int64_t)r * ld + j0;
-            for (int q = 0; q < nb; ++q) {
-                sw += row[q] * yw[j0 + q];
-                sb += row[q] * yb[j0 + q];
-            }
-            yw[r] -= sw;
-            yb[r] -= sb;
-        }
-        __syncthreads();
-    }
-    // backward: L^T y = z
-    for (int j1 = n; j1 > 0; j1 -= CB) {
-        const int j0 = max(0, j1 - CB), nb = j1 - j0;
-        if (wave == 0) {
-            double zw = lane < nb ? yw[j0 + lane] : 0.0, zb = lane < nb ? yb[j0 + lane] : 0.0;
-            // lc[j] = L[j0+j][j0+lane]: column 'lane' of the diagonal block
-            double lc[CB];
-#pragma unroll
-            for (int j = 0; j < CB; ++j) lc[j] = (lane < nb && j < nb) ? A[(int64_t)(j0 + j) * ld + j0 + lane] : 0.0;
-#pragma unroll
-            for (int jj = 0; jj < CB; ++jj) {
-                const int j = CB - 1 - jj;
-                if (j < nb) {
-                    const double d = readlane_d(lc[j], j);
-                    const double xw = readlane_d(zw, j) / d, xb = readlane_d(zb, j) / d;
-                    if (lane == j) {
-                        zw = xw;
-                        zb = xb;
-                    } else if (lane < j) {
-                        zw -= lc[j] * xw;
-                        zb -= lc[j] * xb;
-                    }
-                }
-            }
-            if (lane < nb) {
-                yw[j0 + lane] = zw;
-                yb[j0 + lane] = zb;
-            }
-        }
-        __syncthreads();
-        for (int r = t; r < j0; r += 256) {
-            double sw = 0.0, sb = 0.0;
-            for (int q = 0; q < nb; ++q) {
-                const double l = A[(int64_t)(j0 + q) * ld + r];
-                sw += l * yw[j0 + q];
-                sb += l * yb[j0 + q];
-            }
-            yw[r] -= sw;
-            yb[r] -= sb;
-        }
-        __syncthreads();
-    }
-    // Schur complement of the Lagrange row
-    __shared__ double red[2][256];
+// After the factorisation (which leaves z_w = L^-1 w, z_b = L^-1 b in zw, zb): the Lagrange
+// multiplier lambda = (w.A^-1 b - q_tot) / (w.A^-1 w) = (z_w.z_b - q_tot) / (z_w.z_w), then
+// q = A^-1 (b - lambda w) = L^-T (z_b - lambda z_w), blocked with the diagonal-block inverses:
+//     y_k = D_k^-T (u_k - L_{>k,k}^T y_{>k}).
+// One workgroup of 1024 threads; y is kept in x.
+__global__ __launch_bounds__(1024) void k_chol_finish(const double *__restrict__ A, int ld, int n,
+                                                      const double *__restrict__ Dinv, const double *__restrict__ zw,
+                                                      const double *__restrict__ zb, double q_tot,
+                                                      double *__restrict__ x) {
+    __shared__ double red[2][1024];
+    __shared__ double part[16][CB];
+    __shared__ double Dk[CB][CB + 1];
+    __shared__ double u[CB];
+    const int t = threadIdx.x, lane = t & 63, grp = t >> 6;
     double pw = 0.0, pb = 0.0;
-    for (int r = t; r < n; r += 256) {
-        pw += w[r] * yw[r];
-        pb += w[r] * yb[r];
+    for (int r = t; r < n; r += 1024) {
+        pw += zw[r] * zw[r];
+        pb += zw[r] * zb[r];
     }
     red[0][t] = pw;
     red[1][t] = pb;
     __syncthreads();
-    for (int s = 128; s > 0; s >>= 1) {
+    for (int s = 512; s > 0; s >>= 1) {
         if (t < s) {
             red[0][t] += red[0][t + s];
             red[1][t] += red[1][t + s];
@@ -447,7 +512,41 @@ __global__ __launch_bounds__(256) void k_chol_solve(const double *__restrict__ A
         __syncthreads();
     }
     const double lambda = (red[1][0] - q_tot) / red[0][0];
-    for (int r = t; r < n; r += 256) x[r] = yb[r] - lambda * yw[r];
+    for (int r = t; r < n; r += 1024) x[r] = zb[r] - lambda * zw[r];
+    __syncthreads();
+    const int nblk = (n + CB - 1) / CB;
+    for (int kb = nblk - 1; kb >= 0; --kb) {
+        const int j0 = kb * CB, j1 = min(n, j0 + CB), nb = j1 - j0;
+        #pragma unroll
+        for (int e = t; e < CB * CB; e += 1024) Dk[e / CB][e % CB] = Dinv[(int64_t)kb * CB * CB + e];
+        // (L_{>k,k}^T y_{>k})_c: lane = column c, 16 groups of rows
+        double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+        if (lane < nb) {
+            int r = j1 + grp;
+            for (; r + 48 < n; r += 64) {
+                s0 += A[(int64_t)r * ld + j0 + lane] * x[r];
+                s1 += A[(int64_t)(r + 16) * ld + j0 + lane] * x[r + 16];
+                s2 += A[(int64_t)(r + 32) * ld + j0 + lane] * x[r + 32];
+                s3 += A[(int64_t)(r + 48) * ld + j0 + lane] * x[r + 48];
+            }
+            for (; r < n; r += 16) s0 += A[(int64_t)r * ld + j0 + lane] * x[r];
+        }
+        part[grp][lane] = (s0 + s1) + (s2 + s3);
+        __syncthreads();
+        if (t < nb) {
+            double acc = 0.0;
+#pragma unroll
+            for (int g = 0; g < 16; ++g) acc += part[g][t];
+            u[t] = x[j0 + t] - acc;
+        }
+        __syncthreads();
+        if (t < nb) {
+            double y = 0.0;
+            for (int i = t; i < nb; ++i) y += Dk[i][t] * u[i];
+            x[j0 + t] = y;
+        }
+        __syncthreads();
+    }
     if (t == 0) x[n] = lambda;
 }
 
@@ -529,14 +628,13 @@ int run_solve(rp_fit *fit, const rp_constraints *con, double *q_out, double *sol
                                                                   con->rhs_add ? d_rhs.as<double>() : nullptr, con->q_tot, nr,
                                                                   d_K.as<double>(), ld);
     // ---- Cholesky of A (leading nr x nr block) + Schur step
-    DBuf &d_K2 = fit->ws[20], &d_yw = fit->ws[21], &d_yb = fit->ws[22], &d_wc = fit->ws[23];
-    DBuf &d_cinfo = fit->ws[28], &d_bc = fit->ws[29];
+    DBuf &d_K2 = fit->ws[20], &d_wc = fit->ws[23];
+    DBuf &d_cinfo = fit->ws[28], &d_bc = fit->ws[29], &d_dinv = fit->ws[30];
     RP_CHECK(d_K2.grow(ctx, sizeof(double) * (size_t)n * ld));
-    RP_CHECK(d_yw.grow(ctx, sizeof(double) * nr));
-    RP_CHECK(d_yb.grow(ctx, sizeof(double) * nr));
     RP_CHECK(d_cinfo.grow(ctx, sizeof(int)));
     RP_CHECK(d_wc.grow(ctx, sizeof(double) * nr));
     RP_CHECK(d_bc.grow(ctx, sizeof(double) * nr));
+    RP_CHECK(d_dinv.grow(ctx, sizeof(double) * CB * CB * (size_t)ceil_div(nr, CB)));
     RP_HIP(ctx, hipMemsetAsync(d_cinfo.p, 0, sizeof(int), st));
     k_copy<<<ceil_div((int64_t)n * ld, 256), 256, 0, st>>>(d_K.as<double>(), d_K2.as<double>(), (int64_t)n * ld);
     // the w and b columns before factorisation
@@ -544,12 +642,17 @@ int run_solve(rp_fit *fit, const rp_constraints *con, double *q_out, double *sol
                                  hipMemcpyDeviceToDevice, st));
     RP_HIP(ctx, hipMemcpy2DAsync(d_bc.p, sizeof(double), d_K.as<double>() + n, sizeof(double) * ld, sizeof(double), nr,
                                  hipMemcpyDeviceToDevice, st));
+    RP_HIP(ctx, hipFuncSetAttribute(reinterpret_cast<const void *>(&k_potrf_inv),
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)POTRF_LDS));
     for (int k0 = 0; k0 < nr; k0 += CB) {
         const int nb = std::min(CB, nr - k0);
-        k_potrf_diag<<<1, 64, 0, st>>>(d_K.as<double>(), ld, nr, k0, d_K2.as<double>(), d_cinfo.as<int>());
+        double *dinv = d_dinv.as<double>() + (int64_t)(k0 / CB) * CB * CB;
+        k_potrf_inv<<<1, 256, POTRF_LDS, st>>>(d_K.as<double>(), ld, nr, k0, d_K2.as<double>(), d_cinfo.as<int>(), dinv,
+                                               d_wc.as<double>(), d_bc.as<double>());
         const int below = nr - (k0 + nb);
         if (below > 0) {
-            k_chol_trsm<<<ceil_div(below, 256), 256, 0, st>>>(d_K.as<double>(), ld, nr, k0);
+            k_chol_trsm_inv<<<ceil_div(below, 32), 256, 0, st>>>(d_K.as<double>(), ld, nr, k0, dinv, d_wc.as<double>(),
+                                                                 d_bc.as<double>());
             const int nt = (int)ceil_div(below, 64);
             dim3 g(nt, nt);
             k_chol_update<<<g, 256, 0, st>>>(d_K.as<double>(), ld, nr, k0);
@@ -560,8 +663,8 @@ int run_solve(rp_fit *fit, const rp_constraints *con, double *q_out, double *sol
     RP_HIP(ctx, hipStreamSynchronize(st));
     fit->lu_fallback = cinfo != 0;
     if (cinfo == 0) {
-        k_chol_solve<<<1, 256, 0, st>>>(d_K.as<double>(), ld, nr, d_wc.as<double>(), d_bc.as<double>(), con->q_tot,
-                                         d_yw.as<double>(), d_yb.as<double>(), d_x.as<double>());
+        k_chol_finish<<<1, 1024, 0, st>>>(d_K.as<double>(), ld, nr, d_dinv.as<double>(), d_wc.as<double>(), d_bc.as<double>(),
+                                          con->q_tot, d_x.as<double>());
     } else {
         // indefinite or singular A: LU with partial pivoting on the bordered system
         double *K2 = d_K2.as<double>();

@@ -211,7 +211,17 @@ def test_deterministic_bitwise(gpu_ctx):
     np.testing.assert_array_equal(qs[0], qs[1])
 
 
-# ---------------------------------------------------------------- split precision (int8 Ozaki) path
+# ---------------------------------------------------------------- split precision path
+# (factorized fp64 reciprocal sum, fp32 real space, int8-sliced Gram with exact int32 groups)
+
+def test_fit_type0_split_matches_oracle(oracle, gpu_ctx):
+    """Bare 1/r basis (no reciprocal part) through the split path."""
+    s = synth.tiny(seed=13, n_atoms=4, n=14, L=10.0)
+    r = oracle.fit(s.n_grid, s.axis, s.atomic_numbers, s.atom_pos, s.esp_flat(), fit_flag=0)
+    calc = ChargeCalculator(cube=cube_from_system(s), fit_flag=0, precision="split", context=gpu_ctx)
+    assert np.max(np.abs(calc.fit_charges() - r.charges)) <= 1e-5
+    calc.close()
+
 
 @pytest.mark.parametrize("name", ["water", "crystal200"])
 def test_fit_split_matches_golden(gpu_ctx, tmp_path, name):

@@ -124,8 +124,8 @@ def test_request_bound_on_host_and_invalid_requests():
     b_small = _abi.Server.request_bytes(args(small), con, opts)
     b_large = _abi.Server.request_bytes(args(large), con, opts)
     assert 0 < b_small < b_large
-    # the split chunk buffers dominate: 4 slices of 2K padded columns per row at least
-    assert b_small >= 4 * 512 * 1536
+    # at least one 512-row chunk of Phi (fp64) and its 4 int8 slices over 128 padded columns
+    assert b_small >= 12 * 128 * 512
     b64 = _abi.Server.request_bytes(args(small), con, _abi.make_options(1, _abi.RP_PREC_FP64))
     assert b64 > 0
     bad = dict(con, n_reduced=0)
