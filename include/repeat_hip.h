@@ -44,9 +44,10 @@ extern "C" {
 #define RP_ORDER_FLAT 1  /* flat order f = i + j*nx + k*nx*ny (repeat.py:2086) */
 
 #define RP_PREC_FP64 0   /* plain fp64: fp64 MFMA k-space GEMM, fp64 real space, fp64 Gram */
-#define RP_PREC_OZAKI 1  /* split form: int8 fixed-point slices on MFMA with exact int32
-                            accumulation for the k-space contraction and the Gram, fp32
-                            real space, fp64 across tiles and in the KKT solve */
+#define RP_PREC_OZAKI 1  /* split form: reciprocal sum factorized over the lattice axes (fp64),
+                            real space with fp64 displacements and fp32 erfc, Gram from
+                            int8 fixed-point slices on MFMA with exact int32 accumulation,
+                            fp64 across chunks and in the KKT solve */
 
 typedef struct rp_ctx rp_ctx;
 typedef struct rp_fit rp_fit;

@@ -11,8 +11,10 @@ Same constructor arguments, method names and results as ``repeat.py``'s
   ``avrg_coul`` and optionally ``V_coul`` (``repeat.py:2326-2441``).
 
 ``precision="fp64"`` is the plain fp64 path (the reference's ``--fp64``);
-``precision="split"`` is the split form (int8 fixed-point slices on MFMA with
-exact accumulation, fp32 real space, fp64 across tiles and in the solve).
+``precision="split"`` is the split form: the reciprocal sum factorized over the
+lattice axes in fp64, the real-space sum with fp64 displacements and fp32 erfc,
+the Gram of the basis from int8 fixed-point slices on MFMA with exact int32
+accumulation, fp64 across chunks and in the solve.
 Every computation runs in ``librepeat_hip.so``; nothing falls back to the CPU.
 """
 from __future__ import annotations
