@@ -115,8 +115,17 @@ def run_gpu(ctx, system, con, precision, steps, warmup):
     ctx.profile(False)
     info = fit.info
     M = int(info.n_keep)
-    rows = min(65536, -(-M // 512) * 512) if prec == _abi.RP_PREC_OZAKI else min(65536, -(-M // 128) * 128)
-    out = {"q": q, "lat": np.array(lat), "total": total, "kernels": stats, "M": M, "rows": rows,
+    N = int(fit.n_atoms)
+    np_ = -(-(N + 2) // 128) * 128
+    slices = 4
+    if prec == _abi.RP_PREC_OZAKI:     # chunk plan of the split path: <= 65536 rows, ~2 GiB of chunk buffers
+        rows = max(512, min(65536, -(-M // 512) * 512))
+        while rows > 512 and rows * (8 + slices) * np_ > 1 << 31:
+            rows //= 2
+        rows -= rows % 512
+    else:
+        rows = min(65536, -(-M // 128) * 128)
+    out = {"q": q, "lat": np.array(lat), "total": total, "kernels": stats, "M": M, "rows": rows, "slices": slices,
            "N": int(fit.n_atoms), "K": int(info.n_k), "timings": fit.timings()}
     fit.close()
     return out
@@ -321,7 +330,8 @@ def main():
     achieved = alg_per_launch / t_launch / 1e12
     np_ = ((N + 2 + 127) // 128) * 128
     tiles = sum(1 for bi in range(np_ // 64) for bj in range(np_ // 128) if bi * 64 <= bj * 128 + 127)
-    executed = 2.0 * rows * tiles * 64 * 128 * 16 / t_launch / 1e12      # int8 ops issued, 16 slice products
+    P = split["slices"]
+    executed = 2.0 * rows * tiles * 64 * 128 * P * P / t_launch / 1e12    # int8 ops issued, P^2 slice products
     roof = {"bound": "mfma", "kernel": "k_oz_gram", "achieved": round(achieved, 3), "peak": PEAK_INT8_TOPS,
             "unit": "TFLOP/s", "frac": round(achieved / PEAK_INT8_TOPS, 5),
             "traffic": traffic_from_profile("k_oz_gram", args.workload),
@@ -329,8 +339,8 @@ def main():
             "launch_ms": round(ms / launches, 4), "rows_per_launch": rows,
             "note": "achieved = algorithmic fp64-equivalent flops of the Gram per launch (F_G share of the "
                     "launch's rows) / mean launch time (HIP events on the fit's stream); peak = dense int8 MFMA, "
-                    "the unit the kernel runs on; each fp product costs 16 int8 slice products, which "
-                    "executed_int8_tops counts"}
+                    "the unit the kernel runs on; each fp product costs P^2 = 16 int8 slice products, which "
+                    "executed_int8_tops counts", "slices": P}
     cpu = None
     if not args.no_cpu:
         _log("cpu baseline (oracle sample) ...")

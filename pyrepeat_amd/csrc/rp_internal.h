@@ -136,6 +136,7 @@ struct ChunkPlan {
     int64_t rows = 0;    // kept points per chunk
     int n_tiles = 0;     // Gram tiles touching the upper triangle
     int splits = 1;      // row splits of the Gram per chunk
+    int slices = 4;      // split form: int8 slices per Phi element
 };
 
 inline ChunkPlan plan_chunks(int64_t M, int N, int K, int precision) {
@@ -153,10 +154,11 @@ inline ChunkPlan plan_chunks(int64_t M, int N, int K, int precision) {
             c.splits *= 2;
     } else {
         c.KP = 0;  // the factorized reciprocal sum needs no k-space operand
+        c.slices = 4;
         int64_t rows = std::min<int64_t>(65536, round_up(M, 512));
         rows = std::max<int64_t>(512, rows);
         // bound the chunk buffers (Phi fp64, Phi slices) to about 2 GiB
-        while (rows > 512 && rows * (12LL * c.NP) > ((int64_t)1 << 31)) rows /= 2;
+        while (rows > 512 && rows * ((8LL + c.slices) * c.NP) > ((int64_t)1 << 31)) rows /= 2;
         c.rows = std::max<int64_t>(512, rows - rows % 512);
         c.n_tiles = 0;
         for (int bi = 0; bi < c.NP / 64; ++bi)

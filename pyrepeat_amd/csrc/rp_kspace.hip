@@ -325,8 +325,9 @@ __global__ __launch_bounds__(SEP_THREADS) void k_sep_phi(SepArgs p) {
     if (s == 0 && cmax > 0.0 && a < p.NP) atomicMax(p.colmax + a, (unsigned long long)__double_as_longlong(cmax));
 }
 
-// Row-major Phi tile (64 rows x 64 columns) -> slices [4][NP][rows] of x / S_col, S_col the power
+// Row-major Phi tile (64 rows x 64 columns) -> slices [P][NP][rows] of x / S_col, S_col the power
 // of two with max |Phi_col| / S_col <= 127/128; rows at or past `valid` are zero.
+template <int P>
 __global__ __launch_bounds__(256) void k_slice_rows(const double *__restrict__ phi, int NP, int64_t rows, int64_t valid,
                                                     const unsigned long long *__restrict__ colmax,
                                                     int8_t *__restrict__ out, double *__restrict__ scale_out) {
@@ -348,16 +349,16 @@ __global__ __launch_bounds__(256) void k_slice_rows(const double *__restrict__ p
     while (mx / S > 127.0 / 128.0) S *= 2.0;
     if (blockIdx.x == 0 && part == 0) scale_out[a] = S;
     const double inv = 1.0 / S;
-    uint32_t w[4][4];
+    uint32_t w[P][4];
 #pragma unroll
     for (int b = 0; b < 4; ++b)
 #pragma unroll
-        for (int s = 0; s < 4; ++s) w[s][b] = 0u;
+        for (int s = 0; s < P; ++s) w[s][b] = 0u;
 #pragma unroll
     for (int rr = 0; rr < 16; ++rr) {
         double r = tile[part * 16 + rr][col] * inv;
 #pragma unroll
-        for (int s = 0; s < 4; ++s) {
+        for (int s = 0; s < P; ++s) {
             const double tv = r * 128.0;
             const double c = rint(tv);
             r = tv - c;
@@ -366,7 +367,7 @@ __global__ __launch_bounds__(256) void k_slice_rows(const double *__restrict__ p
     }
     const int64_t plane = (int64_t)NP * rows;
 #pragma unroll
-    for (int s = 0; s < 4; ++s) {
+    for (int s = 0; s < P; ++s) {
         uint4 v = make_uint4(w[s][0], w[s][1], w[s][2], w[s][3]);
         *reinterpret_cast<uint4 *>(out + s * plane + (int64_t)a * rows + rt + part * 16) = v;
     }
@@ -449,7 +450,7 @@ void sep_phi_chunk(rp_fit *fit, FitDevice &dv, const SepPlan &plan, int chunk, i
 void sep_slice_chunk(const double *phi, int NP, int64_t rows, int64_t valid, const unsigned long long *colmax,
                      int8_t *slices, double *scale_out, hipStream_t st) {
     dim3 g((unsigned)(rows / 64), NP / 64);
-    k_slice_rows<<<g, 256, 0, st>>>(phi, NP, rows, valid, colmax, slices, scale_out);
+    k_slice_rows<4><<<g, 256, 0, st>>>(phi, NP, rows, valid, colmax, slices, scale_out);
 }
 
 }  // namespace rp

@@ -27,64 +27,75 @@ namespace rp {
 
 namespace {
 
-constexpr int PMAX = 4;
 constexpr int TM = 64, TN = 128, TK = 64;
 constexpr int TKP = TK + 16;   // padded LDS row (bytes)
 constexpr int NTHR = 512;
 
-// Slices [PMAX][NP][rows] of the chunk's Phi (column-major per slice: rows contiguous).
+// Slices [P][NP][rows] of the chunk's Phi (column-major per slice: rows contiguous).
 struct GramOperands {
     const int8_t *S;
     int64_t plane;   // NP * rows
     int64_t ld;      // rows
 };
 
-// Main loop: the exact int32 group sums I_s = sum_{p+q=s} A_p^T B_q (s <= SMAX) of the block tile
-// (columns m0.. of A, n0.. of B) over k (= grid rows) [k0, k1), a multiple of TK.  Eight waves as
-// 2 x 4, one 32 x 32 tile each.  LDS rows are padded to TKP bytes so the 16-byte fragment reads of
-// 16 consecutive lanes fall on distinct banks.  Each stage holds TK = 64 k (two MFMA k-steps); the
-// next stage is loaded into registers while the current one is consumed.
-template <int SMAX>
+template <int P>
+constexpr size_t gram_lds_bytes() { return 2 * P * (TM + TN) * TKP; }   // 120 KiB (P = 4), 150 KiB (P = 5)
+
+// Main loop: the exact int32 group sums I_s = sum_{p+q=s} A_p^T B_q (s = 0 .. 2P-2) of the block
+// tile (columns m0.. of A, n0.. of B) over k (= grid rows) [k0, k1), a multiple of TK.  Eight waves
+// as 2 x 4, one 32 x 32 tile each.  LDS rows are padded to TKP bytes so the 16-byte fragment reads
+// of 16 consecutive lanes fall on distinct banks.  Each stage holds TK = 64 k (two MFMA k-steps);
+// the next stage is loaded into registers while the current one is consumed.
+template <int P>
 __device__ __forceinline__ void gram_mainloop(const GramOperands &op, int64_t m0, int64_t n0, int k0, int k1,
-                                              ivec16 (&acc)[SMAX + 1], int8_t *lds) {
-    int8_t(*As)[PMAX][TM][TKP] = reinterpret_cast<int8_t(*)[PMAX][TM][TKP]>(lds);
-    int8_t(*Bs)[PMAX][TN][TKP] = reinterpret_cast<int8_t(*)[PMAX][TN][TKP]>(lds + 2 * PMAX * TM * TKP);
+                                              ivec16 (&acc)[2 * P - 1], int8_t *lds) {
+    constexpr int NA = (256 * P + NTHR - 1) / NTHR, NB = (512 * P + NTHR - 1) / NTHR;
+    int8_t(*As)[P][TM][TKP] = reinterpret_cast<int8_t(*)[P][TM][TKP]>(lds);
+    int8_t(*Bs)[P][TN][TKP] = reinterpret_cast<int8_t(*)[P][TN][TKP]>(lds + 2 * P * TM * TKP);
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
     const int wm = wave >> 2, wn = wave & 3;
 #pragma unroll
-    for (int g = 0; g <= SMAX; ++g)
+    for (int g = 0; g < 2 * P - 1; ++g)
 #pragma unroll
         for (int r = 0; r < 16; ++r) acc[g][r] = 0;
     const int nk = (k1 - k0) / TK;
-    // staging, 16-byte chunks: A c in [0, 1024): slice c/256, row (c%256)/4, quarter c%4;
-    //                          B c in [0, 2048): slice c/512, row (c%512)/4, quarter c%4
-    ivec4 ra[2], rb[4];
+    // staging, 16-byte chunks: A c in [0, 256 P): slice c/256, row (c%256)/4, quarter c%4;
+    //                          B c in [0, 512 P): slice c/512, row (c%512)/4, quarter c%4
+    ivec4 ra[NA], rb[NB];
     auto gload = [&](int kbase) {
 #pragma unroll
-        for (int u = 0; u < 2; ++u) {
+        for (int u = 0; u < NA; ++u) {
             const int c = t + NTHR * u;
-            const int p = c >> 8, row = (c & 255) >> 2, qq = c & 3;
-            ra[u] = *reinterpret_cast<const ivec4 *>(op.S + p * op.plane + (m0 + row) * op.ld + kbase + 16 * qq);
+            if (c < 256 * P) {
+                const int p = c >> 8, row = (c & 255) >> 2, qq = c & 3;
+                ra[u] = *reinterpret_cast<const ivec4 *>(op.S + p * op.plane + (m0 + row) * op.ld + kbase + 16 * qq);
+            }
         }
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
+        for (int u = 0; u < NB; ++u) {
             const int c = t + NTHR * u;
-            const int p = c >> 9, row = (c & 511) >> 2, qq = c & 3;
-            rb[u] = *reinterpret_cast<const ivec4 *>(op.S + p * op.plane + (n0 + row) * op.ld + kbase + 16 * qq);
+            if (c < 512 * P) {
+                const int p = c >> 9, row = (c & 511) >> 2, qq = c & 3;
+                rb[u] = *reinterpret_cast<const ivec4 *>(op.S + p * op.plane + (n0 + row) * op.ld + kbase + 16 * qq);
+            }
         }
     };
     auto lstore = [&](int buf) {
 #pragma unroll
-        for (int u = 0; u < 2; ++u) {
+        for (int u = 0; u < NA; ++u) {
             const int c = t + NTHR * u;
-            const int p = c >> 8, row = (c & 255) >> 2, qq = c & 3;
-            *reinterpret_cast<ivec4 *>(&As[buf][p][row][16 * qq]) = ra[u];
+            if (c < 256 * P) {
+                const int p = c >> 8, row = (c & 255) >> 2, qq = c & 3;
+                *reinterpret_cast<ivec4 *>(&As[buf][p][row][16 * qq]) = ra[u];
+            }
         }
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
+        for (int u = 0; u < NB; ++u) {
             const int c = t + NTHR * u;
-            const int p = c >> 9, row = (c & 511) >> 2, qq = c & 3;
-            *reinterpret_cast<ivec4 *>(&Bs[buf][p][row][16 * qq]) = rb[u];
+            if (c < 512 * P) {
+                const int p = c >> 9, row = (c & 511) >> 2, qq = c & 3;
+                *reinterpret_cast<ivec4 *>(&Bs[buf][p][row][16 * qq]) = rb[u];
+            }
         }
     };
     if (nk <= 0) return;
@@ -97,17 +108,17 @@ __device__ __forceinline__ void gram_mainloop(const GramOperands &op, int64_t m0
         if (more) gload(k0 + (kt + 1) * TK);
 #pragma unroll
         for (int h = 0; h < TK / 32; ++h) {
-            ivec4 af[PMAX], bf[PMAX];
+            ivec4 af[P], bf[P];
 #pragma unroll
-            for (int p = 0; p < PMAX; ++p) {
+            for (int p = 0; p < P; ++p) {
                 af[p] = *reinterpret_cast<const ivec4 *>(&As[cur][p][wm * 32 + (lane & 31)][32 * h + 16 * (lane >> 5)]);
                 bf[p] = *reinterpret_cast<const ivec4 *>(&Bs[cur][p][wn * 32 + (lane & 31)][32 * h + 16 * (lane >> 5)]);
             }
 #pragma unroll
-            for (int p = 0; p < PMAX; ++p)
+            for (int p = 0; p < P; ++p)
 #pragma unroll
-                for (int q = 0; q < PMAX; ++q)
-                    if (p + q <= SMAX) acc[p + q] = __builtin_amdgcn_mfma_i32_32x32x32_i8(af[p], bf[q], acc[p + q], 0, 0, 0);
+                for (int q = 0; q < P; ++q)
+                    acc[p + q] = __builtin_amdgcn_mfma_i32_32x32x32_i8(af[p], bf[q], acc[p + q], 0, 0, 0);
         }
         if (more) lstore(cur ^ 1);
         __syncthreads();
@@ -123,9 +134,8 @@ struct GramEpi {
     double *part;          // [split][tile][TM*TN]
 };
 
-// Highest slice group kept: all 16 products of the 4-slice operands (groups s = p + q <= 6).
-constexpr int GRAM_SMAX = 2 * PMAX - 2;
-
+// All P^2 products of the P-slice operands, 2P - 1 exact groups s = p + q.
+template <int P>
 __global__ __launch_bounds__(NTHR) void k_oz_gram(GramOperands op, GramEpi ge, int rows_per_split, int n_splits) {
     extern __shared__ int8_t lds[];
     // Work-groups are dispatched round-robin over the 8 XCDs (linear id L -> XCD L % 8).  When the
@@ -144,8 +154,8 @@ __global__ __launch_bounds__(NTHR) void k_oz_gram(GramOperands op, GramEpi ge, i
     const int2 tile = ge.tiles[tix];
     const int64_t m0 = (int64_t)tile.x * TM, n0 = (int64_t)tile.y * TN;
     // this split covers k (= grid rows) [split * rps, (split+1) * rps)
-    ivec16 acc[GRAM_SMAX + 1];
-    gram_mainloop<GRAM_SMAX>(op, m0, n0, split * rows_per_split, (split + 1) * rows_per_split, acc, lds);
+    ivec16 acc[2 * P - 1];
+    gram_mainloop<P>(op, m0, n0, split * rows_per_split, (split + 1) * rows_per_split, acc, lds);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int wm = wave >> 2, wn = wave & 3;
     double *dst = ge.part + ((int64_t)split * ge.n_tiles + tix) * (TM * TN);
@@ -156,7 +166,7 @@ __global__ __launch_bounds__(NTHR) void k_oz_gram(GramOperands op, GramEpi ge, i
         // exact integer groups -> fp64, weights 2^{-7(s+2)}
         double v = 0.0;
 #pragma unroll
-        for (int g = 0; g <= GRAM_SMAX; ++g) v += (double)acc[g][r] * ldexp(1.0, -7 * (g + 2));
+        for (int g = 0; g < 2 * P - 1; ++g) v += (double)acc[g][r] * ldexp(1.0, -7 * (g + 2));
         const int rr = wm * 32 + M32::d_row(lane, r);
         dst[rr * TN + c] += v * (ge.scale[m0 + rr] * sc);   // slot owned by this block across chunks
     }
@@ -186,7 +196,8 @@ int accumulate_split(rp_fit *fit, FitDevice &dv) {
     const int64_t M = fit->n_keep;
     const ChunkPlan cp = plan_chunks(M, N, K, RP_PREC_OZAKI);
     const int64_t rows = cp.rows;
-    if (cp.NP != NP || PMAX != 4 || TM != 64 || TN != 128 || rows % 512 != 0)
+    constexpr int P = 4;
+    if (cp.NP != NP || cp.slices != P || TM != 64 || TN != 128 || rows % 512 != 0)
         return ctx->fail(RP_EINVAL, "split path: chunk plan does not match the kernel tiling");
     DBuf &d_phi = fit->ws[2], &d_part = fit->ws[3], &d_slices = fit->ws[4], &d_colmax = fit->ws[5],
          &d_scale = fit->ws[6], &d_tiles = fit->ws[9];
@@ -196,7 +207,7 @@ int accumulate_split(rp_fit *fit, FitDevice &dv) {
         RP_CHECK(sep_prepare(fit, dv, rows, plan));
     }
     RP_CHECK(d_phi.grow(ctx, sizeof(double) * NP * rows));
-    RP_CHECK(d_slices.grow(ctx, (size_t)PMAX * NP * rows));
+    RP_CHECK(d_slices.grow(ctx, (size_t)P * NP * rows));
     RP_CHECK(d_colmax.grow(ctx, sizeof(unsigned long long) * NP));
     RP_CHECK(d_scale.grow(ctx, sizeof(double) * NP));
     // Gram tiles (TM x TN) touching the upper triangle, and row splits
@@ -209,13 +220,18 @@ int accumulate_split(rp_fit *fit, FitDevice &dv) {
     if (n_tiles != cp.n_tiles || rows % (splits * TK) != 0)
         return ctx->fail(RP_EINVAL, "split path: Gram tiling does not match the chunk plan");
     const int rps = (int)(rows / splits);
+    // exactness of the int32 groups: a slice digit is at most 127 (first slice) or 64 (the rest), so
+    // a group's per-row sum is at most 2 * 127 * 64 + (P - 2) * 64 * 64; rps rows of it must fit
+    const int64_t group_row_max = 2LL * 127 * 64 + (int64_t)(P - 2) * 64 * 64;
+    if (group_row_max * rps > 2147483647LL)
+        return ctx->fail(RP_EINVAL, "split path: row split too long for exact int32 slice groups");
     RP_CHECK(d_tiles.grow(ctx, sizeof(int2) * n_tiles));
     RP_HIP(ctx, hipMemcpyAsync(d_tiles.p, tiles.data(), sizeof(int2) * n_tiles, hipMemcpyHostToDevice, st));
     RP_CHECK(d_part.grow(ctx, sizeof(double) * (size_t)splits * n_tiles * TM * TN));
     RP_HIP(ctx, hipMemsetAsync(d_part.p, 0, sizeof(double) * (size_t)splits * n_tiles * TM * TN, st));
 
-    const size_t lds_bytes = 2 * PMAX * (TM + TN) * TKP;  // 120 KiB
-    RP_HIP(ctx, hipFuncSetAttribute(reinterpret_cast<const void *>(&k_oz_gram),
+    const size_t lds_bytes = gram_lds_bytes<P>();
+    RP_HIP(ctx, hipFuncSetAttribute(reinterpret_cast<const void *>(&k_oz_gram<P>),
                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes));
     GramOperands go{d_slices.as<int8_t>(), (int64_t)NP * rows, rows};
     GramEpi ge{d_scale.as<double>(), d_tiles.as<int2>(), n_tiles, d_part.as<double>()};
@@ -235,7 +251,7 @@ int accumulate_split(rp_fit *fit, FitDevice &dv) {
                             d_slices.as<int8_t>(), d_scale.as<double>(), st);
         }
         KTimer ktg(ctx, KT_GRAM);
-        k_oz_gram<<<n_tiles * splits, NTHR, lds_bytes, st>>>(go, ge, rps, splits);
+        k_oz_gram<P><<<n_tiles * splits, NTHR, lds_bytes, st>>>(go, ge, rps, splits);
     }
     {
         KTimer ktr(ctx, KT_REDUCE);

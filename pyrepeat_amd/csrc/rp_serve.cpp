@@ -123,7 +123,7 @@ uint64_t fit_bytes_bound(const Setup &s, int n_atoms, int esp_dtype, int precisi
         const ChunkPlan c = plan_chunks(M, n_atoms, (int)K, RP_PREC_OZAKI);
         at(2, 8 * (uint64_t)c.NP * c.rows);
         at(3, 8 * split_cap(c.rows, c.n_tiles) * c.n_tiles * 64 * 128);
-        at(4, 4 * (uint64_t)c.NP * c.rows);
+        at(4, (uint64_t)c.slices * c.NP * c.rows);
         at(5, 8 * (uint64_t)c.NP);
         at(6, 8 * (uint64_t)c.NP);
         at(9, 8 * (uint64_t)c.n_tiles);
