@@ -99,42 +99,6 @@ struct EpiArgs {
     }
 };
 
-// fp32 real-space sum with compensated (Kahan) accumulation; displacements and cutoff test in fp64.
-__device__ __forceinline__ double real_space_f32(const EpiArgs &e, double gx, double gy, double gz, int a, int f) {
-    const double ax = e.atoms[3 * a], ay = e.atoms[3 * a + 1], az = e.atoms[3 * a + 2];
-    if (e.fit_type == 0) {
-        const double dx = gx - ax, dy = gy - ay, dz = gz - az;
-        const double d = sqrt((dx * dx + dy * dy) + dz * dz);
-        return d > 1e-10 ? 1.0 / d : 0.0;
-    }
-    const float sa = (float)e.sqrt_alpha;
-    double sx0, sy0, sz0;
-    if (e.nearest_image(f, a, sx0, sy0, sz0)) {
-        const double dx = gx - (ax + sx0), dy = gy - (ay + sy0), dz = gz - (az + sz0);
-        const double r2 = (dx * dx + dy * dy) + dz * dz;
-        if (!(r2 <= e.R2)) return 0.0;
-        if (sx0 == 0.0 && sy0 == 0.0 && sz0 == 0.0 && r2 <= 1e-20) return 0.0;
-        const float r = sqrtf((float)r2);
-        return (double)(erfcf(sa * r) / r);
-    }
-    float sum = 0.f, comp = 0.f;
-    for (int s = 0; s < e.S; ++s) {
-        const double sx = e.shifts[3 * s], sy = e.shifts[3 * s + 1], sz = e.shifts[3 * s + 2];
-        const double dx = gx - (ax + sx), dy = gy - (ay + sy), dz = gz - (az + sz);
-        const double r2 = (dx * dx + dy * dy) + dz * dz;
-        if (r2 <= e.R2) {
-            if (sx == 0.0 && sy == 0.0 && sz == 0.0 && r2 <= 1e-20) continue;
-            const float r = sqrtf((float)r2);
-            const float term = erfcf(sa * r) / r;
-            const float y = term - comp;
-            const float tt = sum + y;
-            comp = (tt - sum) - y;
-            sum = tt;
-        }
-    }
-    return (double)sum - (double)comp;
-}
-
 // Device copies of the Ewald setup and the chunking plan of one fit.
 struct FitDevice {
     int K = 0, KP = 0, NP = 0, S = 0;

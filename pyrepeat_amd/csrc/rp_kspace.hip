@@ -19,13 +19,32 @@
 // Phi rows are produced per chunk of kept points (rows in ascending flat index, so a chunk is a
 // run of whole or partial grid lines): one work-group takes up to SEP_LINES consecutive lines and
 // 32 atoms; each wave forms T2 of those lines for 8 atoms (one lane per (atom, h)), then evaluates
-// the points (one lane per (atom, point slot)), adds the real-space term (fp64 displacements and
-// cutoff test, fp32 erfc) and writes Phi row-major.
+// the points (one lane per (atom, point slot)), adds the real-space term (fp64, erfc from a
+// piecewise-polynomial table on the argument range [0, pi]) and writes Phi row-major.
 #include "rp_fit_common.h"
+#include "rp_erfc_table.h"
 
 namespace rp {
 
 namespace {
+
+// erfc(x) for 0 <= x <= pi from the segment table (max abs error 7e-16): the real-space argument
+// sqrt(alpha) r never exceeds sqrt(alpha) Rc = pi because alpha = (pi / Rc)^2 (repeat.py:1087).
+__device__ __forceinline__ double erfc_pi(double x) {
+    const int k = min((int)(x * (ERFC_NSEG / 3.141592653589793)), ERFC_NSEG - 1);
+    const double *c = kErfcTable[k];
+    const double t = x - c[0];
+    double y = c[ERFC_DEG + 1];
+#pragma unroll
+    for (int d = ERFC_DEG; d >= 1; --d) y = fma(y, t, c[d]);
+    return y;
+}
+
+// erfc(sqrt_alpha r) / r from r^2, in fp64.
+__device__ __forceinline__ double erfc_over_r(double r2, double sqrt_alpha) {
+    const double rinv = rsqrt(r2);
+    return erfc_pi(sqrt_alpha * (r2 * rinv)) * rinv;
+}
 
 constexpr int SH = 8;           // h = 0..7 (the k-set uses 0..6)
 constexpr int SK = 13;          // k, l = -6..6
@@ -182,7 +201,9 @@ struct SepArgs {
 // each lane following the line its rows fall in.  Waves never wait on each other after the line
 // table is staged.
 __global__ __launch_bounds__(SEP_THREADS) void k_sep_phi(SepArgs p) {
-    __shared__ double2 t2w[SEP_WAVES][SEP_LINES][8][SH];
+    // T2 rows of the wave's 8 atoms, padded to SH + 1 entries so the 8 atoms' reads of one h fall on
+    // disjoint LDS banks
+    __shared__ double2 t2w[SEP_WAVES][SEP_LINES][8][SH + 1];
     __shared__ int lrow[SEP_LINES + 1], lkey[SEP_LINES];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int a_l = lane >> 3, s = lane & 7;
@@ -235,7 +256,6 @@ __global__ __launch_bounds__(SEP_THREADS) void k_sep_phi(SepArgs p) {
         Fy = e.afrac[3 * a + 1];
         Fz = e.afrac[3 * a + 2];
     }
-    const float sa = (float)e.sqrt_alpha;
     const int64_t rs = max<int64_t>(lrow[0], p.r0);
     const int64_t rend = min<int64_t>(lrow[n_l], p.r0 + p.valid);
     double cmax = 0.0;
@@ -283,30 +303,22 @@ __global__ __launch_bounds__(SEP_THREADS) void k_sep_phi(SepArgs p) {
                     const double dy = fma(w0, e.box[1], fma(w1, e.box[4], w2 * e.box[7]));
                     const double dz = fma(w0, e.box[2], fma(w1, e.box[5], w2 * e.box[8]));
                     const double r2 = fma(dx, dx, fma(dy, dy, dz * dz));
-                    if (r2 <= e.R2 && !(n0 == 0.0 && n1 == 0.0 && n2 == 0.0 && r2 <= 1e-20)) {
-                        const float rr = sqrtf((float)r2);
-                        real = (double)(erfcf(sa * rr) / rr);
-                    }
+                    if (r2 <= e.R2 && !(n0 == 0.0 && n1 == 0.0 && n2 == 0.0 && r2 <= 1e-20))
+                        real = erfc_over_r(r2, e.sqrt_alpha);
                 } else {
                     // several images within Rc: the shift-list sum with the reference roundings
                     const double gx = (id * e.geom.ax[0] + jd * e.geom.ax[3]) + md * e.geom.ax[6];
                     const double gy = (id * e.geom.ax[1] + jd * e.geom.ax[4]) + md * e.geom.ax[7];
                     const double gz = (id * e.geom.ax[2] + jd * e.geom.ax[5]) + md * e.geom.ax[8];
-                    float sum = 0.f, comp = 0.f;
+                    double sum = 0.0;
                     for (int t = 0; t < e.S; ++t) {
                         const double sx = e.shifts[3 * t], sy = e.shifts[3 * t + 1], sz = e.shifts[3 * t + 2];
                         const double dx = gx - (Rx + sx), dy = gy - (Ry + sy), dz = gz - (Rz + sz);
                         const double r2 = (dx * dx + dy * dy) + dz * dz;
-                        if (r2 <= e.R2 && !(sx == 0.0 && sy == 0.0 && sz == 0.0 && r2 <= 1e-20)) {
-                            const float rr = sqrtf((float)r2);
-                            const float term = erfcf(sa * rr) / rr;
-                            const float y = term - comp;
-                            const float tsum = sum + y;
-                            comp = (tsum - sum) - y;
-                            sum = tsum;
-                        }
+                        if (r2 <= e.R2 && !(sx == 0.0 && sy == 0.0 && sz == 0.0 && r2 <= 1e-20))
+                            sum += erfc_over_r(r2, e.sqrt_alpha);
                     }
-                    real = (double)sum - (double)comp;
+                    real = sum;
                 }
             }
             val = acc + real;
@@ -325,23 +337,24 @@ __global__ __launch_bounds__(SEP_THREADS) void k_sep_phi(SepArgs p) {
     if (s == 0 && cmax > 0.0 && a < p.NP) atomicMax(p.colmax + a, (unsigned long long)__double_as_longlong(cmax));
 }
 
-// Row-major Phi tile (64 rows x 64 columns) -> slices [P][NP][rows] of x / S_col, S_col the power
-// of two with max |Phi_col| / S_col <= 127/128; rows at or past `valid` are zero.
+// Row-major Phi tile (128 rows x 32 columns) -> slices [P][NP][rows] of x / S_col, S_col the power
+// of two with max |Phi_col| / S_col <= 127/128; rows at or past `valid` are zero.  Reads are 256-byte
+// row segments, writes 128-byte runs of one column's slice.
 template <int P>
 __global__ __launch_bounds__(256) void k_slice_rows(const double *__restrict__ phi, int NP, int64_t rows, int64_t valid,
                                                     const unsigned long long *__restrict__ colmax,
                                                     int8_t *__restrict__ out, double *__restrict__ scale_out) {
-    __shared__ double tile[64][65];
+    __shared__ double tile[128][33];
     const int t = threadIdx.x;
-    const int64_t rt = (int64_t)blockIdx.x * 64;
-    const int c0 = blockIdx.y * 64;
+    const int64_t rt = (int64_t)blockIdx.x * 128;
+    const int c0 = blockIdx.y * 32;
 #pragma unroll
     for (int e = 0; e < 16; ++e) {
-        const int q = e * 256 + t, row = q >> 6, col = q & 63;
+        const int q = e * 256 + t, row = q >> 5, col = q & 31;
         tile[row][col] = (rt + row < valid) ? phi[(rt + row) * NP + c0 + col] : 0.0;
     }
     __syncthreads();
-    const int col = t >> 2, part = t & 3;
+    const int col = t >> 3, part = t & 7;    // 8 threads per column, 16 rows each
     const int a = c0 + col;
     const double mx = __longlong_as_double((long long)colmax[a]);
     double S = 1.0;
@@ -449,7 +462,7 @@ void sep_phi_chunk(rp_fit *fit, FitDevice &dv, const SepPlan &plan, int chunk, i
 
 void sep_slice_chunk(const double *phi, int NP, int64_t rows, int64_t valid, const unsigned long long *colmax,
                      int8_t *slices, double *scale_out, hipStream_t st) {
-    dim3 g((unsigned)(rows / 64), NP / 64);
+    dim3 g((unsigned)(rows / 128), NP / 32);
     k_slice_rows<4><<<g, 256, 0, st>>>(phi, NP, rows, valid, colmax, slices, scale_out);
 }
 

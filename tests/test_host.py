@@ -142,3 +142,28 @@ def test_stream_splitting_sums_the_same_potential():
     a = synth.ewald_esp(s.n_grid, s.axis, s.atom_pos, s.true_charges)
     b = synth.ewald_esp(s.n_grid, s.axis, s.atom_pos, s.true_charges, **synth.STREAM_SPLIT)
     assert np.max(np.abs(a - b)) < 1e-7 * max(1.0, np.max(np.abs(a)))
+
+
+def test_erfc_table_matches_library_erfc():
+    """The split path's real-space erfc (piecewise polynomials on [0, pi], csrc/rp_erfc_table.h)
+    against scipy.special.erfc; the argument sqrt(alpha) r never exceeds pi since alpha = (pi/Rc)^2."""
+    from scipy.special import erfc
+    text = open(os.path.join(ROOT, "pyrepeat_amd", "csrc", "rp_erfc_table.h")).read()
+    nseg = int(re.search(r"ERFC_NSEG = (\d+)", text).group(1))
+    deg = int(re.search(r"ERFC_DEG = (\d+)", text).group(1))
+    body = text[text.index("= {") + 3:text.rindex("};")]
+    rows = [[float(v) for v in r.split(",") if v.strip()] for r in re.findall(r"\{([^{}]*)\}", body)]
+    assert len(rows) == nseg and all(len(r) == deg + 2 for r in rows)
+    x = np.linspace(0.0, np.pi, 20001)
+    k = np.minimum((x * (nseg / np.pi)).astype(int), nseg - 1)
+    y = np.zeros_like(x)
+    for i in range(len(x)):
+        c = rows[k[i]]
+        t = x[i] - c[0]
+        acc = c[deg + 1]
+        for d in range(deg, 0, -1):
+            acc = acc * t + c[d]
+        y[i] = acc
+    assert np.max(np.abs(y - erfc(x))) < 2e-15
+    # and relative accuracy where erfc is smallest (x = pi: 8.5e-6)
+    assert np.max(np.abs(y - erfc(x)) / erfc(x)) < 1e-12
