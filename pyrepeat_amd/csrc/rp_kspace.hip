@@ -19,34 +19,50 @@
 // Phi rows are produced per chunk of kept points (rows in ascending flat index, so a chunk is a
 // run of whole or partial grid lines): one work-group takes up to SEP_LINES consecutive lines and
 // 32 atoms; each wave forms T2 of those lines for 8 atoms (one lane per (atom, h)), then evaluates
-// the points (one lane per (atom, point slot)), adds the real-space term (fp64, erfc from a
-// piecewise-polynomial table on the argument range [0, pi]) and writes Phi row-major.
+// the points (one lane per (atom, point slot)) as a Horner sum in X(1, i), adds the real-space term
+// (fp64, erfc as one polynomial on the argument range [0, pi]) and writes Phi row-major.
 #include "rp_fit_common.h"
-#include "rp_erfc_table.h"
+#include "rp_erf_poly.h"
 
 namespace rp {
 
 namespace {
 
-// erfc(x) for 0 <= x <= pi from the segment table (max abs error 7e-16): the real-space argument
-// sqrt(alpha) r never exceeds sqrt(alpha) Rc = pi because alpha = (pi / Rc)^2 (repeat.py:1087).
-__device__ __forceinline__ double erfc_pi(double x) {
-    const int k = min((int)(x * (ERFC_NSEG / 3.141592653589793)), ERFC_NSEG - 1);
-    const double *c = kErfcTable[k];
-    const double t = x - c[0];
-    double y = c[ERFC_DEG + 1];
-#pragma unroll
-    for (int d = ERFC_DEG; d >= 1; --d) y = fma(y, t, c[d]);
-    return y;
+// A coefficient as an SGPR pair set up where it is used.  Left to itself the compiler keeps all the
+// polynomial coefficients in VGPRs for the life of the kernel (two per coefficient, a third of the
+// register file here), which halves the waves a SIMD can hold; two scalar moves per use cost no
+// VALU issue.
+template <uint64_t Bits>
+__device__ __forceinline__ double sgpr_const() {
+    uint32_t lo, hi;
+    asm volatile("s_mov_b32 %0, %2\n\ts_mov_b32 %1, %3" : "=s"(lo), "=s"(hi)
+                 : "i"((int32_t)(uint32_t)(Bits & 0xffffffffu)), "i"((int32_t)(uint32_t)(Bits >> 32)));
+    return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
 }
 
-// erfc(sqrt_alpha r) / r from r^2, in fp64.
-__device__ __forceinline__ double erfc_over_r(double r2, double sqrt_alpha) {
+template <int D>
+__device__ __forceinline__ double erf_poly_horner(double u, double p) {
+    if constexpr (D < 0) {
+        return p;
+    } else {
+        constexpr uint64_t bits = __builtin_bit_cast(uint64_t, kErfPoly[D]);
+        return erf_poly_horner<D - 1>(u, fma(p, u, sgpr_const<bits>()));
+    }
+}
+
+// erfc(sqrt(alpha) r) / r for 0 < r <= Rc.  alpha = (pi / Rc)^2 (repeat.py:1087), so u = alpha r^2 lies
+// in [0, pi^2] and erfc(sqrt(alpha) r) / r = 1/r - sqrt(alpha) P(u) with erf(x) = x P(x^2); P is one
+// polynomial (max abs error of the erfc 7.5e-16), so the term needs one rsqrt and no table.
+__device__ __forceinline__ double erfc_over_r(double r2, double alpha, double sqrt_alpha) {
     const double rinv = rsqrt(r2);
-    return erfc_pi(sqrt_alpha * (r2 * rinv)) * rinv;
+    const double u = fma(alpha, r2, -ERF_U0);
+    constexpr uint64_t top = __builtin_bit_cast(uint64_t, kErfPoly[ERF_DEG]);
+    const double p = erf_poly_horner<ERF_DEG - 1>(u, sgpr_const<top>());
+    return fma(-sqrt_alpha, p, rinv);
 }
 
-constexpr int SH = 8;           // h = 0..7 (the k-set uses 0..6)
+constexpr int SH = 8;           // h slots per atom: lanes of a wave are (8 atoms) x (8 h or 8 point slots)
+constexpr int HMAX = 6;         // the k-set has h^2 + k^2 + l^2 < 49, so 0 <= h <= 6
 constexpr int SK = 13;          // k, l = -6..6
 constexpr int SOFF = 6;
 constexpr int SBOX = SH * SK;   // T1 entries per (plane, atom)
@@ -224,12 +240,13 @@ __global__ __launch_bounds__(SEP_THREADS) void k_sep_phi(SepArgs p) {
     if ((int)threadIdx.x <= n_l) lrow[threadIdx.x] = p.line_row[l_begin + threadIdx.x];
     if ((int)threadIdx.x < n_l) lkey[threadIdx.x] = p.line_key[l_begin + threadIdx.x];
     __syncthreads();
-    // ---- T2 of every line (lane s = h)
+    // ---- T2 of every line (lane s = h); the line key is uniform, so the Y row comes through the
+    // scalar cache
     if (recip) {
         double2 t1[SK];
         int cur_m = -1;
         for (int li = 0; li < n_l; ++li) {
-            const int key = lkey[li];
+            const int key = __builtin_amdgcn_readfirstlane(lkey[li]);
             const int m = key / ny, j = key - m * ny;
             if (m != cur_m) {
                 cur_m = m;
@@ -256,29 +273,38 @@ __global__ __launch_bounds__(SEP_THREADS) void k_sep_phi(SepArgs p) {
         Fy = e.afrac[3 * a + 1];
         Fz = e.afrac[3 * a + 2];
     }
+    const double alpha = e.sqrt_alpha * e.sqrt_alpha;
+    const double2 *x1 = p.ex + nx;   // X(1, i)
     const int64_t rs = max<int64_t>(lrow[0], p.r0);
     const int64_t rend = min<int64_t>(lrow[n_l], p.r0 + p.valid);
     double cmax = 0.0;
     int li = 0, key = 0, j = 0, m = 0;
-    for (int64_t r = rs + s; r < rend; r += SH) {
+    int64_t r = rs + s;
+    int f_next = r < rend ? e.idx[r] : 0;
+    for (; r < rend; r += SH) {
         if (r >= lrow[li + 1] || r == rs + s) {
             while (r >= lrow[li + 1]) ++li;
             key = lkey[li];
             m = key / ny;
             j = key - m * ny;
         }
-        const int f = e.idx[r];
-        const int i = f - key * nx;
+        const int i = f_next - key * nx;
+        if (r + SH < rend) f_next = e.idx[r + SH];
         double val = 0.0;
         if (atom) {
             double acc = 0.0;
             if (recip) {
+                // Re sum_h T2[h] X(1, i)^h by Horner from h = HMAX
+                const double2 x = x1[i];
+                double2 t = t2w[wave][li][a_l][HMAX];
 #pragma unroll
-                for (int q = 0; q < SH; ++q) {
-                    const double2 x = p.ex[q * nx + i];
-                    const double2 t = t2w[wave][li][a_l][q];
-                    acc = fma(t.x, x.x, fma(-t.y, x.y, acc));
+                for (int q = HMAX - 1; q >= 1; --q) {
+                    const double2 c = t2w[wave][li][a_l][q];
+                    const double tx = fma(t.x, x.x, fma(-t.y, x.y, c.x));
+                    const double ty = fma(t.x, x.y, fma(t.y, x.x, c.y));
+                    t = make_double2(tx, ty);
                 }
+                acc = fma(t.x, x.x, fma(-t.y, x.y, t2w[wave][li][a_l][0].x));
             }
             const double id = (double)i, jd = (double)j, md = (double)m;
             double real = 0.0;
@@ -304,7 +330,7 @@ __global__ __launch_bounds__(SEP_THREADS) void k_sep_phi(SepArgs p) {
                     const double dz = fma(w0, e.box[2], fma(w1, e.box[5], w2 * e.box[8]));
                     const double r2 = fma(dx, dx, fma(dy, dy, dz * dz));
                     if (r2 <= e.R2 && !(n0 == 0.0 && n1 == 0.0 && n2 == 0.0 && r2 <= 1e-20))
-                        real = erfc_over_r(r2, e.sqrt_alpha);
+                        real = erfc_over_r(r2, alpha, e.sqrt_alpha);
                 } else {
                     // several images within Rc: the shift-list sum with the reference roundings
                     const double gx = (id * e.geom.ax[0] + jd * e.geom.ax[3]) + md * e.geom.ax[6];
@@ -316,7 +342,7 @@ __global__ __launch_bounds__(SEP_THREADS) void k_sep_phi(SepArgs p) {
                         const double dx = gx - (Rx + sx), dy = gy - (Ry + sy), dz = gz - (Rz + sz);
                         const double r2 = (dx * dx + dy * dy) + dz * dz;
                         if (r2 <= e.R2 && !(sx == 0.0 && sy == 0.0 && sz == 0.0 && r2 <= 1e-20))
-                            sum += erfc_over_r(r2, e.sqrt_alpha);
+                            sum += erfc_over_r(r2, alpha, e.sqrt_alpha);
                     }
                     real = sum;
                 }
@@ -398,7 +424,7 @@ int sep_prepare(rp_fit *fit, FitDevice &dv, int64_t rows, SepPlan &plan) {
         std::vector<int> kbox(SH * SK * SK, -1);
         for (int p = 0; p < K; ++p) {
             const int h = s.khkl[3 * p], k = s.khkl[3 * p + 1] + SOFF, l = s.khkl[3 * p + 2] + SOFF;
-            if (h < 0 || h >= SH || k < 0 || k >= SK || l < 0 || l >= SK)
+            if (h < 0 || h > HMAX || k < 0 || k >= SK || l < 0 || l >= SK)
                 return ctx->fail(RP_EINVAL, "k-vector index outside the factorized box");
             kbox[(h * SK + k) * SK + l] = p;
         }

@@ -144,26 +144,27 @@ def test_stream_splitting_sums_the_same_potential():
     assert np.max(np.abs(a - b)) < 1e-7 * max(1.0, np.max(np.abs(a)))
 
 
-def test_erfc_table_matches_library_erfc():
-    """The split path's real-space erfc (piecewise polynomials on [0, pi], csrc/rp_erfc_table.h)
-    against scipy.special.erfc; the argument sqrt(alpha) r never exceeds pi since alpha = (pi/Rc)^2."""
+def test_erf_polynomial_matches_library_erfc():
+    """The split path's real-space term erfc(sqrt(a) r)/r = 1/r - sqrt(a) P(a r^2), erf(x) = x P(x^2)
+    (csrc/rp_erf_poly.h), against scipy.special.erfc: a = (pi/Rc)^2 keeps a r^2 within [0, pi^2]."""
     from scipy.special import erfc
-    text = open(os.path.join(ROOT, "pyrepeat_amd", "csrc", "rp_erfc_table.h")).read()
-    nseg = int(re.search(r"ERFC_NSEG = (\d+)", text).group(1))
-    deg = int(re.search(r"ERFC_DEG = (\d+)", text).group(1))
-    body = text[text.index("= {") + 3:text.rindex("};")]
-    rows = [[float(v) for v in r.split(",") if v.strip()] for r in re.findall(r"\{([^{}]*)\}", body)]
-    assert len(rows) == nseg and all(len(r) == deg + 2 for r in rows)
+    text = open(os.path.join(ROOT, "pyrepeat_amd", "csrc", "rp_erf_poly.h")).read()
+    deg = int(re.search(r"ERF_DEG = (\d+)", text).group(1))
+    u0 = float(re.search(r"ERF_U0 = ([-0-9.e+]+);", text).group(1))
+    body = text[text.index("kErfPoly[ERF_DEG + 1] = {") + 25:text.rindex("};")]
+    c = [float(v) for v in body.replace("\n", " ").split(",") if v.strip()]
+    assert len(c) == deg + 1 and abs(u0 - np.pi ** 2 / 2) < 1e-15
+
+    def P(u):
+        s = u - u0
+        acc = np.full_like(u, c[deg])
+        for d in range(deg - 1, -1, -1):
+            acc = acc * s + c[d]
+        return acc
     x = np.linspace(0.0, np.pi, 20001)
-    k = np.minimum((x * (nseg / np.pi)).astype(int), nseg - 1)
-    y = np.zeros_like(x)
-    for i in range(len(x)):
-        c = rows[k[i]]
-        t = x[i] - c[0]
-        acc = c[deg + 1]
-        for d in range(deg, 0, -1):
-            acc = acc * t + c[d]
-        y[i] = acc
-    assert np.max(np.abs(y - erfc(x))) < 2e-15
-    # and relative accuracy where erfc is smallest (x = pi: 8.5e-6)
-    assert np.max(np.abs(y - erfc(x)) / erfc(x)) < 1e-12
+    assert np.max(np.abs((1.0 - x * P(x * x)) - erfc(x))) < 2e-15
+    for rc in (9.0, 12.0, 20.0):
+        sa = np.pi / rc
+        r = np.linspace(0.5, rc, 5001)
+        got = 1.0 / r - sa * P(sa * sa * r * r)
+        assert np.max(np.abs(got - erfc(sa * r) / r)) < 4e-15
