@@ -28,10 +28,10 @@ namespace rp {
 
 namespace {
 
-// A coefficient as an SGPR pair set up where it is used.  Left to itself the compiler keeps all the
-// polynomial coefficients in VGPRs for the life of the kernel (two per coefficient, a third of the
-// register file here), which halves the waves a SIMD can hold; two scalar moves per use cost no
-// VALU issue.
+// A coefficient as an SGPR pair set up where it is used, and the FMA that takes it as its addend in
+// the three-operand form.  Left to itself the compiler either keeps every coefficient in VGPRs for the
+// life of the kernel (two per coefficient, a third of the register file here, which cuts the waves a
+// SIMD holds) or copies each one into VGPRs for the two-operand FMA (three VALU issues per term).
 template <uint64_t Bits>
 __device__ __forceinline__ double sgpr_const() {
     uint32_t lo, hi;
@@ -40,13 +40,20 @@ __device__ __forceinline__ double sgpr_const() {
     return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
 }
 
+// a * b + c with c in SGPRs
+__device__ __forceinline__ double fma_sgpr_addend(double a, double b, double c) {
+    double d;
+    asm("v_fma_f64 %0, %1, %2, %3" : "=v"(d) : "v"(a), "v"(b), "s"(c));
+    return d;
+}
+
 template <int D>
 __device__ __forceinline__ double erf_poly_horner(double u, double p) {
     if constexpr (D < 0) {
         return p;
     } else {
         constexpr uint64_t bits = __builtin_bit_cast(uint64_t, kErfPoly[D]);
-        return erf_poly_horner<D - 1>(u, fma(p, u, sgpr_const<bits>()));
+        return erf_poly_horner<D - 1>(u, fma_sgpr_addend(p, u, sgpr_const<bits>()));
     }
 }
 
@@ -55,7 +62,8 @@ __device__ __forceinline__ double erf_poly_horner(double u, double p) {
 // polynomial (max abs error of the erfc 7.5e-16), so the term needs one rsqrt and no table.
 __device__ __forceinline__ double erfc_over_r(double r2, double alpha, double sqrt_alpha) {
     const double rinv = rsqrt(r2);
-    const double u = fma(alpha, r2, -ERF_U0);
+    constexpr uint64_t mu0 = __builtin_bit_cast(uint64_t, -ERF_U0);
+    const double u = fma_sgpr_addend(alpha, r2, sgpr_const<mu0>());
     constexpr uint64_t top = __builtin_bit_cast(uint64_t, kErfPoly[ERF_DEG]);
     const double p = erf_poly_horner<ERF_DEG - 1>(u, sgpr_const<top>());
     return fma(-sqrt_alpha, p, rinv);
@@ -220,7 +228,7 @@ __global__ __launch_bounds__(SEP_THREADS) void k_sep_phi(SepArgs p) {
     // T2 rows of the wave's 8 atoms, padded to SH + 1 entries so the 8 atoms' reads of one h fall on
     // disjoint LDS banks
     __shared__ double2 t2w[SEP_WAVES][SEP_LINES][8][SH + 1];
-    __shared__ int lrow[SEP_LINES + 1], lkey[SEP_LINES];
+    __shared__ int lrow[SEP_LINES + 1], lkey[SEP_LINES], lj[SEP_LINES], lm[SEP_LINES];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int a_l = lane >> 3, s = lane & 7;
     // Work-groups go round-robin to the 8 XCDs (linear id L -> XCD L % 8): XCD x takes line groups
@@ -238,7 +246,12 @@ __global__ __launch_bounds__(SEP_THREADS) void k_sep_phi(SepArgs p) {
     const int l_begin = p.l_first + item * SEP_LINES;
     const int n_l = min(SEP_LINES, p.l_end - l_begin);
     if ((int)threadIdx.x <= n_l) lrow[threadIdx.x] = p.line_row[l_begin + threadIdx.x];
-    if ((int)threadIdx.x < n_l) lkey[threadIdx.x] = p.line_key[l_begin + threadIdx.x];
+    if ((int)threadIdx.x < n_l) {
+        const int key = p.line_key[l_begin + threadIdx.x];
+        lkey[threadIdx.x] = key;
+        lm[threadIdx.x] = key / ny;
+        lj[threadIdx.x] = key - (key / ny) * ny;
+    }
     __syncthreads();
     // ---- T2 of every line (lane s = h); the line key is uniform, so the Y row comes through the
     // scalar cache
@@ -246,8 +259,8 @@ __global__ __launch_bounds__(SEP_THREADS) void k_sep_phi(SepArgs p) {
         double2 t1[SK];
         int cur_m = -1;
         for (int li = 0; li < n_l; ++li) {
-            const int key = __builtin_amdgcn_readfirstlane(lkey[li]);
-            const int m = key / ny, j = key - m * ny;
+            const int m = __builtin_amdgcn_readfirstlane(lm[li]);
+            const int j = __builtin_amdgcn_readfirstlane(lj[li]);
             if (m != cur_m) {
                 cur_m = m;
                 const double2 *src = p.T1 + ((int64_t)m * p.N + a) * SBOX + s * SK;
@@ -278,18 +291,21 @@ __global__ __launch_bounds__(SEP_THREADS) void k_sep_phi(SepArgs p) {
     const int64_t rs = max<int64_t>(lrow[0], p.r0);
     const int64_t rend = min<int64_t>(lrow[n_l], p.r0 + p.valid);
     double cmax = 0.0;
-    int li = 0, key = 0, j = 0, m = 0;
+    int li = 0, base = lkey[0] * nx, j = lj[0], m = lm[0];
     int64_t r = rs + s;
-    int f_next = r < rend ? e.idx[r] : 0;
-    for (; r < rend; r += SH) {
-        if (r >= lrow[li + 1] || r == rs + s) {
+    const int *idx = e.idx + r;
+    double *out = p.phi + (r - p.r0) * p.NP + a;
+    const int64_t out_step = (int64_t)SH * p.NP;
+    int f_next = r < rend ? *idx : 0;
+    for (; r < rend; r += SH, idx += SH, out += out_step) {
+        if (r >= lrow[li + 1]) {
             while (r >= lrow[li + 1]) ++li;
-            key = lkey[li];
-            m = key / ny;
-            j = key - m * ny;
+            base = lkey[li] * nx;
+            m = lm[li];
+            j = lj[li];
         }
-        const int i = f_next - key * nx;
-        if (r + SH < rend) f_next = e.idx[r + SH];
+        const int i = f_next - base;
+        if (r + SH < rend) f_next = idx[SH];
         double val = 0.0;
         if (atom) {
             double acc = 0.0;
@@ -353,7 +369,7 @@ __global__ __launch_bounds__(SEP_THREADS) void k_sep_phi(SepArgs p) {
         } else if (a == p.N + 1) {
             val = 1.0;
         }
-        p.phi[(r - p.r0) * p.NP + a] = val;
+        *out = val;
         cmax = fmax(cmax, fabs(val));
     }
     // column maximum over the 8 point slots of the atom
