@@ -220,164 +220,138 @@ __global__ __launch_bounds__(256) void k_update(double *__restrict__ A, int ld, 
 
 constexpr int CB = 64;
 
-__device__ __forceinline__ void wave_sync_lds() {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
-// Diagonal block of the blocked Cholesky (64 x 64, 256 threads, in LDS): factor L, its inverse
-// X = L^-1 (written to Dinv, row-major, identity past the block) and the forward-substitution step
-// of the two right-hand sides, z_k = X r_k (rows k0.. of rw, rb, in place).  Inside the block the
-// factorisation runs over 8 x 8 sub-blocks: wave 0 factors and inverts each diagonal sub-block,
-// then all threads form the sub-panel below it (times the sub-block inverse), update the rest of
-// the block, and advance the right-looking inverse: X starts as the identity (B), block row K
-// becomes X_KK B_K once final, and the rows below take B_I -= L_IK X_K.
-// A pivot at or below 1e-13 of the original diagonal entry counts as a failure: the system is
-// then left to the pivoted LU, which reports exact singularity as the reference's LAPACK call does.
-constexpr int SB = 8;                        // sub-block
-constexpr size_t POTRF_LDS = (2 * CB * (CB + 1) + 2 * CB) * sizeof(double);
-
+// Diagonal block of the blocked Cholesky (64 x 64, 256 threads): factor L, its inverse X = L^-1
+// (written to Dinv, row-major, identity past the block) and the forward-substitution step of the two
+// right-hand sides, z_k = X r_k (rows k0.. of rw, rb, in place).
+// The block lives in registers: thread (tr, tc) holds rows 4 tr.. and columns 4 tc.. of the Schur
+// complement S and of W, the right-looking inverse (W = I at the start).  Pivot step j needs only
+// column j of S and row j of W, which their owners publish to LDS at the end of step j - 1; after one
+// barrier every thread forms p_j = sqrt(S_jj), L_rj = S_rj / p_j, X_j = W_j / p_j and updates its
+// blocks, S_rc -= L_rj L_cj (r, c > j) and W_r -= L_rj X_j (r > j).  Column j of S and row j of W are
+// final after step j and stay unscaled in the registers; the pivots are applied once at the end
+// (L_rc = S_rc / p_c, L_cc = p_c, X_r = W_r / p_r).  64 steps, one barrier each.
+// A pivot at or below 1e-13 of the original diagonal entry counts as a failure: the system is then
+// left to the pivoted LU, which reports exact singularity as the reference's LAPACK call does.
 __global__ __launch_bounds__(256) void k_potrf_inv(double *__restrict__ A, int ld, int n, int k0,
                                                    const double *__restrict__ orig, int *__restrict__ info,
                                                    double *__restrict__ Dinv, double *__restrict__ rw,
                                                    double *__restrict__ rb) {
-    extern __shared__ double sm[];
-    double(*L)[CB + 1] = reinterpret_cast<double(*)[CB + 1]>(sm);
-    double(*X)[CB + 1] = reinterpret_cast<double(*)[CB + 1]>(sm + CB * (CB + 1));
-    double *zw = sm + 2 * CB * (CB + 1);
-    double *zb = zw + CB;
+    __shared__ double col[2][CB], xrow[2][CB];
+    __shared__ double od[CB];        // original diagonal entries for the pivot test
+    __shared__ double pv[CB], pinv[CB];
+    __shared__ double Xs[CB][CB + 1];
+    __shared__ double zw[CB], zb[CB];
     const int nb = min(CB, n - k0);
-    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
-    #pragma unroll
-    for (int e = t; e < CB * CB; e += 256) {
-        const int i = e / CB, c = e % CB;
-        L[i][c] = (i < nb && c <= i) ? A[(int64_t)(k0 + i) * ld + k0 + c] : (i == c ? 1.0 : 0.0);
-        X[i][c] = i == c ? 1.0 : 0.0;
-    }
-    __shared__ double od[CB];   // original diagonal entries for the pivot test, fetched up front
+    const int t = threadIdx.x, tr = t >> 4, tc = t & 15;
+    const bool lower = tc <= tr;
+    double a[4][4], x[4][4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int i = 4 * tr + u, c = 4 * tc + v;
+            a[u][v] = (i < nb && c <= i) ? A[(int64_t)(k0 + i) * ld + k0 + c] : (i == c ? 1.0 : 0.0);
+            x[u][v] = i == c ? 1.0 : 0.0;
+        }
     if (t < CB) {
         zw[t] = t < nb ? rw[k0 + t] : 0.0;
         zb[t] = t < nb ? rb[k0 + t] : 0.0;
         od[t] = t < nb ? orig[(int64_t)(k0 + t) * ld + k0 + t] : 1.0;
+        // W is lower triangular: the entries of a published row past the owners' blocks stay zero
+        xrow[1][t] = 0.0;
+    }
+    __syncthreads();
+    // column 0 of S and row 0 of W
+    if (tc == 0) {
+#pragma unroll
+        for (int u = 0; u < 4; ++u) col[0][4 * tr + u] = a[u][0];
+    }
+    if (tr == 0) {
+#pragma unroll
+        for (int v = 0; v < 4; ++v) xrow[0][4 * tc + v] = x[0][v];
     }
     __syncthreads();
     int bad = 0;
-    for (int kb = 0; kb < CB; kb += SB) {
-        // ---- diagonal sub-block: factor and invert (wave 0, lane = (r, c) of the 8 x 8)
-        if (wave == 0) {
-            const int r = lane >> 3, c = lane & 7;
-            for (int j = 0; j < SB; ++j) {
-                const double d = L[kb + j][kb + j];
-                double piv = 1.0;
-                if (kb + j < nb) {
-                    if (!(d > 1e-13 * fabs(od[kb + j]))) {
-                        if (!bad) bad = k0 + kb + j + 1;
-                    } else {
-                        piv = sqrt(d);
-                    }
+    for (int j = 0; j < CB; ++j) {
+        const int buf = j & 1;
+        const double d = col[buf][j];
+        double inv = 1.0, piv = 1.0;
+        if (j < nb) {
+            if (!(d > 1e-13 * fabs(od[j]))) {
+                if (!bad) bad = k0 + j + 1;
+            } else {
+                inv = rsqrt(d);
+                piv = d * inv;
+            }
+        }
+        if (t == 0) {
+            pv[j] = piv;
+            pinv[j] = inv;
+        }
+        if (lower) {
+            // masked multipliers: L_rj is zero for rows r <= j and L_cj for columns c <= j, and W's
+            // row j is zero past column j, so the update is branch-free
+            double lr[4], lc[4], xj[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const double l = col[buf][4 * tr + u] * inv;
+                lr[u] = 4 * tr + u > j ? l : 0.0;
+            }
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const double l = col[buf][4 * tc + v] * inv;
+                lc[v] = 4 * tc + v > j ? l : 0.0;
+                xj[v] = xrow[buf][4 * tc + v] * inv;
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    a[u][v] = fma(-lr[u], lc[v], a[u][v]);
+                    x[u][v] = fma(-lr[u], xj[v], x[u][v]);
                 }
-                wave_sync_lds();
-                if (r == j && c == j) L[kb + j][kb + j] = piv;
-                if (r > j && c == j) L[kb + r][kb + j] = L[kb + r][kb + j] / piv;
-                wave_sync_lds();
-                if (r > j && c > j && c <= r) L[kb + r][kb + c] -= L[kb + r][kb + j] * L[kb + c][kb + j];
-                wave_sync_lds();
-            }
-            for (int i = 0; i < SB; ++i) {
-                if (r == i && c <= i) {
-                    double x;
-                    if (c == i) {
-                        x = 1.0 / L[kb + i][kb + i];
-                    } else {
-                        double acc = 0.0;
-                        for (int m = c; m < i; ++m) acc += L[kb + i][kb + m] * X[kb + m][kb + c];
-                        x = -acc / L[kb + i][kb + i];
-                    }
-                    X[kb + i][kb + c] = x;
+            // publish column j + 1 of S and row j + 1 of W for the next step; rows above j + 1 of
+            // the column are never read unmasked
+            const int nj = j + 1, njb = nj >> 2, nq = nj & 3;
+            if (tc == njb) {
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const double v = nq == 0 ? a[u][0] : nq == 1 ? a[u][1] : nq == 2 ? a[u][2] : a[u][3];
+                    col[buf ^ 1][4 * tr + u] = v;
                 }
-                wave_sync_lds();
             }
-        }
-        __syncthreads();
-        // ---- sub-panel below: L[i][kb..kb+8) <- A[i][kb..kb+8) X_kk^T
-        const int below = CB - (kb + SB);
-        double pv[2];
-        int pi[2], pc[2];
+            if (tr == njb) {
 #pragma unroll
-        for (int u = 0; u < 2; ++u) {
-            const int e = t + 256 * u;
-            pi[u] = -1;
-            if (e < below * SB) {
-                pi[u] = kb + SB + e / SB;
-                pc[u] = e % SB;
-                double acc = 0.0;
-                for (int m = 0; m <= pc[u]; ++m) acc += L[pi[u]][kb + m] * X[kb + pc[u]][kb + m];
-                pv[u] = acc;
+                for (int v = 0; v < 4; ++v) {
+                    const double w = nq == 0 ? x[0][v] : nq == 1 ? x[1][v] : nq == 2 ? x[2][v] : x[3][v];
+                    xrow[buf ^ 1][4 * tc + v] = w;
+                }
             }
-        }
-        __syncthreads();
-#pragma unroll
-        for (int u = 0; u < 2; ++u)
-            if (pi[u] >= 0) L[pi[u]][kb + pc[u]] = pv[u];
-        __syncthreads();
-        // ---- rest of the block: L[i][c] -= sum_m L[i][kb+m] L[c][kb+m], kb+8 <= c <= i; and the
-        // inverse's block row kb: X[kb+r][c] = sum_m X_kk[r][m] B[kb+m][c] for c < kb (B in X)
-        for (int e = t; e < below * below; e += 256) {
-            const int i = kb + SB + e / below, c = kb + SB + e % below;
-            if (c <= i) {
-                double acc = 0.0;
-#pragma unroll
-                for (int m = 0; m < SB; ++m) acc += L[i][kb + m] * L[c][kb + m];
-                L[i][c] -= acc;
-            }
-        }
-        double xv[2];
-        int xr[2], xc[2];
-#pragma unroll
-        for (int u = 0; u < 2; ++u) {
-            const int e = t + 256 * u;
-            xr[u] = -1;
-            if (e < SB * kb) {
-                xr[u] = e / kb;
-                xc[u] = e % kb;
-                double acc = 0.0;
-                for (int m = 0; m <= xr[u]; ++m) acc += X[kb + xr[u]][kb + m] * X[kb + m][xc[u]];
-                xv[u] = acc;
-            }
-        }
-        __syncthreads();
-#pragma unroll
-        for (int u = 0; u < 2; ++u)
-            if (xr[u] >= 0) X[kb + xr[u]][xc[u]] = xv[u];
-        __syncthreads();
-        // B[i][c] -= sum_m L[i][kb+m] X[kb+m][c] for rows i below the sub-block, columns c < kb + 8
-        for (int e = t; e < below * (kb + SB); e += 256) {
-            const int i = kb + SB + e / (kb + SB), c = e % (kb + SB);
-            double acc = 0.0;
-#pragma unroll
-            for (int m = 0; m < SB; ++m) acc += L[i][kb + m] * X[kb + m][c];
-            X[i][c] -= acc;
         }
         __syncthreads();
     }
-    // ---- outputs: factor, inverse, z_k = X r_k
-    #pragma unroll
-    for (int e = t; e < CB * CB; e += 256) {
-        const int i = e / CB, c = e % CB;
-        Dinv[e] = c <= i ? X[i][c] : 0.0;
-        if (i < nb && c <= i) A[(int64_t)(k0 + i) * ld + k0 + c] = L[i][c];
-    }
+    // ---- apply the pivots; outputs: factor, inverse, z_k = X r_k
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int i = 4 * tr + u, c = 4 * tc + v;
+            const double xv = lower && c <= i ? x[u][v] * pinv[i] : 0.0;
+            Xs[i][c] = xv;
+            Dinv[i * CB + c] = xv;
+            if (lower && i < nb && c <= i) A[(int64_t)(k0 + i) * ld + k0 + c] = c == i ? pv[c] : a[u][v] * pinv[c];
+        }
+    __syncthreads();
     if (t < nb) {
-        double a = 0.0, b = 0.0;
+        double sa = 0.0, sb = 0.0;
         for (int c = 0; c <= t; ++c) {
-            a += X[t][c] * zw[c];
-            b += X[t][c] * zb[c];
+            sa += Xs[t][c] * zw[c];
+            sb += Xs[t][c] * zb[c];
         }
-        rw[k0 + t] = a;
-        rb[k0 + t] = b;
+        rw[k0 + t] = sa;
+        rb[k0 + t] = sb;
     }
-    if (wave == 0 && lane == 0 && bad && info[0] == 0) info[0] = bad;
+    if (t == 0 && bad && info[0] == 0) info[0] = bad;
 }
 
 // Panel below the diagonal block: L21 = A21 L11^{-T} = A21 Dinv^T, 32 rows per workgroup, and the
@@ -642,12 +616,10 @@ int run_solve(rp_fit *fit, const rp_constraints *con, double *q_out, double *sol
                                  hipMemcpyDeviceToDevice, st));
     RP_HIP(ctx, hipMemcpy2DAsync(d_bc.p, sizeof(double), d_K.as<double>() + n, sizeof(double) * ld, sizeof(double), nr,
                                  hipMemcpyDeviceToDevice, st));
-    RP_HIP(ctx, hipFuncSetAttribute(reinterpret_cast<const void *>(&k_potrf_inv),
-                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)POTRF_LDS));
     for (int k0 = 0; k0 < nr; k0 += CB) {
         const int nb = std::min(CB, nr - k0);
         double *dinv = d_dinv.as<double>() + (int64_t)(k0 / CB) * CB * CB;
-        k_potrf_inv<<<1, 256, POTRF_LDS, st>>>(d_K.as<double>(), ld, nr, k0, d_K2.as<double>(), d_cinfo.as<int>(), dinv,
+        k_potrf_inv<<<1, 256, 0, st>>>(d_K.as<double>(), ld, nr, k0, d_K2.as<double>(), d_cinfo.as<int>(), dinv,
                                                d_wc.as<double>(), d_bc.as<double>());
         const int below = nr - (k0 + nb);
         if (below > 0) {
