@@ -247,6 +247,33 @@ int rp_submit(rp_server *srv, const rp_request *req, uint64_t *ticket);
 int rp_poll(rp_server *srv, uint64_t ticket, int32_t wait_ms, rp_result *res, double *q_out);
 int rp_server_stats_get(const rp_server *srv, rp_server_stats *out);
 
+/* ---------------------------------------------------------------- input files
+ * Host-side readers (no device work; errors via rp_last_error(NULL)).  Values are parsed
+ * multithreaded and bitwise equal to Python's float() on each token.
+ */
+typedef struct {
+    int32_t n_atoms;
+    int32_t n_grid[3];
+    double origin[3];            /* read and reported; the fit ignores it, as the reference does */
+    double axis[9];              /* grid step vectors (rows), Bohr */
+    int64_t values_offset;       /* byte offset of the first ESP value */
+} rp_cube_header;
+
+/* read_gauss_cube (repeat.py:770-804), header part: atom count, origin, grid and step vectors.
+ * A negative atom count is rejected (the reference's reader fails on it too). */
+int rp_cube_read_header(const char *path, rp_cube_header *h);
+
+/* read_gauss_cube (repeat.py:770-804), data part: atoms (Z, x y z per line; NULL skips) and the first
+ * n_total ESP values in file order (RP_ORDER_CUBE).  Fails on fewer values than the grid needs or on
+ * a token float() rejects; values past n_total are checked and ignored, as the reference ignores
+ * them.  n_threads <= 0: up to 16 host threads. */
+int rp_cube_read(const char *path, const rp_cube_header *h, int32_t *atomic_number, double *atom_xyz,
+                 double *esp, int n_threads);
+
+/* The value loop of read_gauss_cube / read_vasp_potential (repeat.py:797, 731): parse every
+ * whitespace-separated token of text[0, len), store the first cap into out, total count in *n_found. */
+int rp_parse_values(const char *text, int64_t len, double *out, int64_t cap, int n_threads, int64_t *n_found);
+
 #ifdef __cplusplus
 }
 #endif

@@ -35,6 +35,7 @@ EXPORTED = (
     "rp_fit_phi", "rp_fit_timings", "rp_ctx_profile", "rp_ctx_kernel_stats", "rp_fit_refit",
     "rp_fit_reset", "rp_server_create", "rp_server_destroy", "rp_server_error", "rp_server_now",
     "rp_request_bytes", "rp_submit", "rp_poll", "rp_server_stats_get",
+    "rp_cube_read_header", "rp_cube_read", "rp_parse_values",
 )
 KERNEL_KINDS = {"kspace": 0, "gram": 1, "filter": 2, "solve": 3, "phase": 4, "slice": 5, "reduce": 6}
 
@@ -93,6 +94,11 @@ class RpServerStats(ctypes.Structure):
                 ("peak_reserved", ctypes.c_uint64)]
 
 
+class RpCubeHeader(ctypes.Structure):
+    _fields_ = [("n_atoms", ctypes.c_int32), ("n_grid", ctypes.c_int32 * 3), ("origin", ctypes.c_double * 3),
+                ("axis", ctypes.c_double * 9), ("values_offset", ctypes.c_int64)]
+
+
 _lib = None
 
 
@@ -138,6 +144,10 @@ def load(path: str | None = None):
     L.rp_submit.argtypes = [vp, ctypes.POINTER(RpRequest), ctypes.POINTER(ctypes.c_uint64)]
     L.rp_poll.argtypes = [vp, ctypes.c_uint64, ctypes.c_int32, ctypes.POINTER(RpResult), dp]
     L.rp_server_stats_get.argtypes = [vp, ctypes.POINTER(RpServerStats)]
+    L.rp_cube_read_header.argtypes = [ctypes.c_char_p, ctypes.POINTER(RpCubeHeader)]
+    L.rp_cube_read.argtypes = [ctypes.c_char_p, ctypes.POINTER(RpCubeHeader), ctypes.POINTER(ctypes.c_int32), dp, dp,
+                               ctypes.c_int]
+    L.rp_parse_values.argtypes = [ctypes.c_char_p, ctypes.c_int64, dp, ctypes.c_int64, ctypes.c_int, i64p]
     for name in EXPORTED:
         if getattr(L, name).restype is None and name not in ("rp_last_error", "rp_server_error", "rp_server_now"):
             getattr(L, name).restype = ctypes.c_int
@@ -208,6 +218,33 @@ def default_context():
     if _default_ctx is None:
         _default_ctx = Context(0, 0)
     return _default_ctx
+
+
+def read_cube(path, n_threads=0):
+    """``read_gauss_cube`` through the library's parser: header, atoms and the ESP in file order."""
+    L = load()
+    bpath = os.fsencode(path)
+    h = RpCubeHeader()
+    check(L.rp_cube_read_header(bpath, ctypes.byref(h)))
+    n_grid = np.array(list(h.n_grid), dtype=np.int64)
+    Z = np.zeros(h.n_atoms, dtype=np.int32)
+    xyz = np.zeros((h.n_atoms, 3))
+    esp = np.empty(int(np.prod(n_grid)))
+    check(L.rp_cube_read(bpath, ctypes.byref(h), ptr(Z, ctypes.c_int32), ptr(xyz), ptr(esp), int(n_threads)))
+    return {"n_atoms": int(h.n_atoms), "n_grid": n_grid, "atom_index": Z.astype(np.int64), "atom_pos": xyz,
+            "axis_vector": np.array(list(h.axis)).reshape(3, 3), "origin": np.array(list(h.origin)), "esp_cube": esp}
+
+
+def parse_values(text: bytes, cap=None, n_threads=0):
+    """Every whitespace-separated value of ``text`` as float64 (bitwise ``float()``); the first ``cap``."""
+    L = load()
+    n = ctypes.c_int64()
+    if cap is None:
+        check(L.rp_parse_values(text, len(text), None, 0, int(n_threads), ctypes.byref(n)))
+        cap = n.value
+    out = np.empty(int(cap))
+    check(L.rp_parse_values(text, len(text), ptr(out), int(cap), int(n_threads), ctypes.byref(n)))
+    return out, int(n.value)
 
 
 def make_system(n_grid, axis, atomic_numbers, atom_xyz, esp, esp_order=RP_ORDER_CUBE):

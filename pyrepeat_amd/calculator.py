@@ -22,7 +22,7 @@ from __future__ import annotations
 import numpy as np
 
 from . import _abi
-from .cube import read_gauss_cube, v_pot_view
+from .cube import read_data, v_pot_view
 from .params import constraint_arrays, read_qeq_params, read_resp_params, read_symmetry
 
 PRECISIONS = {"fp64": _abi.RP_PREC_FP64, "split": _abi.RP_PREC_OZAKI}
@@ -40,11 +40,12 @@ class ChargeCalculator:
         self.R_cutoff = float(R_cutoff)
         self.q_tot = float(q_tot)
         self.precision = precision
-        self.cube = cube if cube is not None else read_gauss_cube(cube_file)
+        self.cube = cube if cube is not None else read_data(cube_file)
         self.n_atoms = int(self.cube["n_atoms"])
         self.ctx = context or _abi.default_context()
         self._fit = _abi.Fit(self.ctx, self.cube["n_grid"], self.cube["axis_vector"], self.cube["atom_index"],
-                             self.cube["atom_pos"], self.cube["esp_cube"], _abi.RP_ORDER_CUBE, self.fit_flag,
+                             self.cube["atom_pos"], self.cube["esp_cube"],
+                             self.cube.get("esp_order", _abi.RP_ORDER_CUBE), self.fit_flag,
                              PRECISIONS[precision], self.vdw_factor, self.vdw_max, self.R_cutoff)
         self.symm_data = read_symmetry(symm_file) if symm_file else None
         self.resp_data = read_resp_params(resp_file) if resp_file else None

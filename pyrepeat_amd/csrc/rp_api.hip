@@ -217,8 +217,8 @@ int run_accumulate(rp_fit *fit) {
     float ms = 0.f;
     RP_HIP(ctx, hipEventElapsedTime(&ms, e0, e1));
     fit->t_accum_ms = ms;
-    hipEventDestroy(e0);
-    hipEventDestroy(e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
     fit->na = fit->n_atoms + 2;
     fit->accumulated = true;
     return RP_OK;

@@ -405,8 +405,8 @@ int run_filter(rp_fit *fit) {
     RP_HIP(ctx, hipStreamSynchronize(st));
     if (n_keep == 0) {
         fit->n_keep = 0;
-        hipEventDestroy(e0);
-        hipEventDestroy(e1);
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
         return ctx->fail(RP_EEMPTY, "the vdW filter kept no grid point");
     }
     fit->n_keep = n_keep;
@@ -435,8 +435,8 @@ int run_filter(rp_fit *fit) {
     float ms = 0.f;
     RP_HIP(ctx, hipEventElapsedTime(&ms, e0, e1));
     fit->t_filter_ms = ms;
-    hipEventDestroy(e0);
-    hipEventDestroy(e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
     double gmin[3], gmax[3];
     for (int c = 0; c < 3; ++c) {
         gmin[c] = bounds[c];

@@ -98,19 +98,9 @@ class System:
 
     def write_cube(self, path):
         """Gaussian cube text: origin 0, atom lines ``Z 0.0 x y z``, values %.10E, 6 per line."""
-        nx, ny, nz = (int(v) for v in self.n_grid)
-        with open(path, "w") as f:
-            f.write(f"{self.name} synthetic ESP\nexact Ewald potential plus noise\n")
-            f.write(f"{self.n_atoms:5d} {0.0:12.6f} {0.0:12.6f} {0.0:12.6f}\n")
-            for i in range(3):
-                f.write(f"{int(self.n_grid[i]):5d} " + " ".join(f"{v:.17g}" for v in self.axis[i]) + "\n")
-            for z, p in zip(self.atomic_numbers, self.atom_pos):
-                f.write(f"{int(z):5d} {0.0:12.6f} {p[0]:.17g} {p[1]:.17g} {p[2]:.17g}\n")
-            vals = self.esp_cube.astype(np.float64).reshape(nx * ny, nz)
-            for row in vals:
-                for s in range(0, nz, 6):
-                    f.write(" ".join(f"{v:.10E}" for v in row[s:s + 6]) + "\n")
-
+        from .cube import cube_from_system, write_gauss_cube
+        write_gauss_cube(path, cube_from_system(self), comment=(f"{self.name} synthetic ESP",
+                                                               "exact Ewald potential plus noise"))
 
 # --------------------------------------------------------------------------- geometry helpers
 
