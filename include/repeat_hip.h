@@ -192,6 +192,18 @@ int rp_fit_timings(const rp_fit *fit, double *t3);
  */
 int rp_fit_reset(rp_fit *fit, const rp_system *sys, const rp_options *opt);
 
+/*
+ * ChargeCalculator.compute_qeq_charges (repeat.py:2538-2610): the QEq charges of the fit's atoms,
+ * from its Ewald setup.  chi (N): electronegativities, j_self (N): idempotentials 2 * hardness (the
+ * caller resolves the parameter tables as the reference does).  J_ij for i != j is the Ewald pair
+ * term of _ewald_pairwise (repeat.py:2612-2634): erfc(sqrt(alpha) r)/r over the full shift list for
+ * every image with r > 1e-10 (no cutoff) plus sum_k w_k cos(k . (R_i - R_j)); fit-type 0 uses
+ * 1/|R_i - R_j|.  Rows 1..N-1 of the system are J_{i-1} - J_i = chi_i - chi_{i-1}, row 0 is
+ * sum q = q_tot; LU with partial pivoting (RP_ESINGULAR on an exactly singular matrix, where the
+ * reference switches to lstsq).  q_out: N charges; j_out (N x N, nullable): the Coulomb matrix.
+ */
+int rp_fit_qeq(rp_fit *fit, const double *chi, const double *j_self, double q_tot, double *q_out, double *j_out);
+
 /* ---------------------------------------------------------------- serving
  * The reference fits one cube per process (repeat.py:2636-2789).  The serving
  * loop runs many such fits concurrently on one device: submitted requests wait

@@ -35,7 +35,7 @@ EXPORTED = (
     "rp_fit_phi", "rp_fit_timings", "rp_ctx_profile", "rp_ctx_kernel_stats", "rp_fit_refit",
     "rp_fit_reset", "rp_server_create", "rp_server_destroy", "rp_server_error", "rp_server_now",
     "rp_request_bytes", "rp_submit", "rp_poll", "rp_server_stats_get",
-    "rp_cube_read_header", "rp_cube_read", "rp_parse_values",
+    "rp_cube_read_header", "rp_cube_read", "rp_parse_values", "rp_fit_qeq",
 )
 KERNEL_KINDS = {"kspace": 0, "gram": 1, "filter": 2, "solve": 3, "phase": 4, "slice": 5, "reduce": 6}
 
@@ -147,6 +147,7 @@ def load(path: str | None = None):
     L.rp_cube_read_header.argtypes = [ctypes.c_char_p, ctypes.POINTER(RpCubeHeader)]
     L.rp_cube_read.argtypes = [ctypes.c_char_p, ctypes.POINTER(RpCubeHeader), ctypes.POINTER(ctypes.c_int32), dp, dp,
                                ctypes.c_int]
+    L.rp_fit_qeq.argtypes = [vp, dp, dp, ctypes.c_double, dp, dp]
     L.rp_parse_values.argtypes = [ctypes.c_char_p, ctypes.c_int64, dp, ctypes.c_int64, ctypes.c_int, i64p]
     for name in EXPORTED:
         if getattr(L, name).restype is None and name not in ("rp_last_error", "rp_server_error", "rp_server_now"):
@@ -385,6 +386,15 @@ class Fit:
         out = np.empty((len(g), self.n_atoms))
         check(load().rp_fit_phi(self.handle, len(g), ptr(g), ptr(out)), self.ctx.handle)
         return out
+
+    def qeq(self, chi, j_self, q_tot=0.0, want_j=False):
+        """QEq charges of the fit's atoms (and the N x N Coulomb matrix when ``want_j``)."""
+        chi = np.ascontiguousarray(chi, dtype=np.float64)
+        js = np.ascontiguousarray(j_self, dtype=np.float64)
+        q = np.empty(self.n_atoms)
+        J = np.empty((self.n_atoms, self.n_atoms)) if want_j else None
+        check(load().rp_fit_qeq(self.handle, ptr(chi), ptr(js), float(q_tot), ptr(q), ptr(J)), self.ctx.handle)
+        return (q, J) if want_j else q
 
     def timings(self):
         t = np.zeros(3)

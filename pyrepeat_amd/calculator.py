@@ -8,22 +8,27 @@ Same constructor arguments, method names and results as ``repeat.py``'s
 - ``fit_charges()`` streams the basis and solves the constrained least squares
   on the device, returning one charge per atom (``repeat.py:2184-2324``);
 - ``compute_stats(charges, need_v_coul)`` returns ``rms_error``, ``avrg_qm``,
-  ``avrg_coul`` and optionally ``V_coul`` (``repeat.py:2326-2441``).
+  ``avrg_coul`` and optionally ``V_coul`` (``repeat.py:2326-2441``);
+- ``compute_qeq_charges(param_set=None)`` is the QEq charge model
+  (``repeat.py:2538-2634``): Coulomb matrix and solve on the device.
 
 ``precision="fp64"`` is the plain fp64 path (the reference's ``--fp64``);
 ``precision="split"`` is the split form: the reciprocal sum factorized over the
-lattice axes in fp64, the real-space sum with fp64 displacements and fp32 erfc,
-the Gram of the basis from int8 fixed-point slices on MFMA with exact int32
+lattice axes in fp64, the real-space sum in fp64 (erfc as one polynomial on
+the argument range the cutoff allows), the Gram of the basis from int8 fixed-point slices on MFMA with exact int32
 accumulation, fp64 across chunks and in the solve.
 Every computation runs in ``librepeat_hip.so``; nothing falls back to the CPU.
 """
 from __future__ import annotations
+
+import time
 
 import numpy as np
 
 from . import _abi
 from .cube import read_data, v_pot_view
 from .params import constraint_arrays, read_qeq_params, read_resp_params, read_symmetry
+from .tables import ATOM_SYMBOLS, DEFAULT_QEQ_PARAMS
 
 PRECISIONS = {"fp64": _abi.RP_PREC_FP64, "split": _abi.RP_PREC_OZAKI}
 
@@ -130,6 +135,48 @@ class ChargeCalculator:
         ata, atb, sum_phi, sum_v = self._fit.normal_equations()
         v = self.V_pot
         return ata, atb, sum_phi, sum_v, float(v @ v), self.n_grid_points
+
+    # ---- QEq charge model (repeat.py:2538-2634)
+    def qeq_parameters(self, param_set=None):
+        """Per-atom (electronegativity, idempotential J_ii = 2 * hardness) as compute_qeq_charges
+        resolves them (repeat.py:2540-2574): without ``param_set`` the method is QEq on the default
+        table, with one it is EQeq (hydrogen then takes chi = (13.598 - 2.0)/2 eV, J = 15.598 eV);
+        --qeq-file entries override the table; unknown elements take (0.15, 0.50) with a warning."""
+        if param_set is None:
+            param_set, method = DEFAULT_QEQ_PARAMS, "QEq"
+        else:
+            method = "EQeq"
+        h_i0, h_i1 = -2.0, 13.598
+        ev_to_ha = 1.0 / 27.211386245988
+        chi = np.zeros(self.n_atoms)
+        jii = np.zeros(self.n_atoms)
+        for i, z in enumerate(self.cube["atom_index"]):
+            z = int(z)
+            sym = ATOM_SYMBOLS[z] if z < len(ATOM_SYMBOLS) else "X"
+            if sym == "H" and method == "EQeq":
+                chi[i] = 0.5 * (h_i1 + h_i0) * ev_to_ha
+                jii[i] = (h_i1 - h_i0) * ev_to_ha
+            elif self.qeq_data and sym in self.qeq_data:
+                chi[i] = self.qeq_data[sym]["electroneg"]
+                jii[i] = 2.0 * self.qeq_data[sym]["hardness"]
+            elif sym in param_set:
+                c, h = param_set[sym]
+                chi[i] = c
+                jii[i] = 2.0 * h
+            else:
+                chi[i] = 0.15
+                jii[i] = 0.50
+                print(f"Warning: No parameters for {sym}, using defaults")
+        return chi, jii, method
+
+    def compute_qeq_charges(self, param_set=None, return_matrix=False):
+        """QEq charges on the device (the Coulomb matrix and the N x N solve both run there)."""
+        chi, jii, method = self.qeq_parameters(param_set)
+        print(f"Computing {method} Coulomb matrix...")
+        t0 = time.time()
+        out = self._fit.qeq(chi, jii, self.q_tot, want_j=return_matrix)
+        print(f"  Coulomb matrix and solve in {time.time() - t0:.2f} seconds")
+        return out
 
     def timings(self):
         return self._fit.timings()

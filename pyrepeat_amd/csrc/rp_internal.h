@@ -254,4 +254,8 @@ int run_accumulate(rp_fit *fit);
 int run_solve(rp_fit *fit, const rp_constraints *con, double *q_out, double *solution_out, int32_t *used_fallback);
 int run_phi_matvec(rp_fit *fit, const double *q, double *v_coul);
 int run_phi_points(rp_fit *fit, int64_t m, const double *grid_xyz, double *phi_out);
+int run_qeq(rp_fit *fit, const double *chi, const double *j_self, double q_tot, double *q_out, double *j_out);
+// Solves the n x n system held row-major in K2 (leading dimension ld, right-hand side in column n) by
+// LU with partial pivoting into x; RP_ESINGULAR on an exactly zero pivot.
+int lu_solve_augmented(Ctx *ctx, double *K2, int ld, int n, int *piv, int *info, double *x, hipStream_t st);
 }  // namespace rp

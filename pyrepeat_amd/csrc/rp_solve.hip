@@ -551,6 +551,33 @@ __global__ void k_expand(const double *__restrict__ x, const int *__restrict__ s
 
 }  // namespace
 
+int lu_solve_augmented(Ctx *ctx, double *K2, int ld, int n, int *piv, int *info, double *x, hipStream_t st) {
+    const int ncols = n + 1;
+    RP_HIP(ctx, hipMemsetAsync(info, 0, sizeof(int), st));
+    for (int k0 = 0; k0 < n; k0 += NB) {
+        const int nb = std::min(NB, n - k0);
+        k_panel<<<1, 1024, 0, st>>>(K2, ld, n, k0, nb, piv, info);
+        k_swap_rest<<<ceil_div(ncols, 256), 256, 0, st>>>(K2, ld, ncols, k0, nb, piv);
+        const int right = ncols - (k0 + nb);
+        if (right > 0) {
+            k_trsm<<<ceil_div(right, 256), 256, 0, st>>>(K2, ld, ncols, k0, nb);
+            const int down = n - (k0 + nb);
+            if (down > 0) {
+                dim3 g(ceil_div(right, 64), ceil_div(down, 64));
+                k_update<<<g, 256, 0, st>>>(K2, ld, n, ncols, k0, nb);
+            }
+        }
+    }
+    int h_info = 0;
+    RP_HIP(ctx, hipMemcpyAsync(&h_info, info, sizeof(int), hipMemcpyDeviceToHost, st));
+    RP_HIP(ctx, hipStreamSynchronize(st));
+    if (h_info != 0)
+        return ctx->fail(RP_ESINGULAR, "system matrix is exactly singular (zero pivot at row " + std::to_string(h_info) + ")");
+    k_backsub<<<1, 1024, sizeof(double) * n, st>>>(K2, ld, n, x);
+    RP_HIP(ctx, hipGetLastError());
+    return RP_OK;
+}
+
 int run_solve(rp_fit *fit, const rp_constraints *con, double *q_out, double *solution_out, int32_t *used_fallback) {
     Ctx *ctx = fit->ctx;
     hipStream_t st = ctx->stream;
@@ -639,31 +666,14 @@ int run_solve(rp_fit *fit, const rp_constraints *con, double *q_out, double *sol
                                           con->q_tot, d_x.as<double>());
     } else {
         // indefinite or singular A: LU with partial pivoting on the bordered system
-        double *K2 = d_K2.as<double>();
-        for (int k0 = 0; k0 < n; k0 += NB) {
-            const int nb = std::min(NB, n - k0);
-            k_panel<<<1, 1024, 0, st>>>(K2, ld, n, k0, nb, d_piv.as<int>(), d_info.as<int>());
-            k_swap_rest<<<ceil_div(ncols, 256), 256, 0, st>>>(K2, ld, ncols, k0, nb, d_piv.as<int>());
-            const int right = ncols - (k0 + nb);
-            if (right > 0) {
-                k_trsm<<<ceil_div(right, 256), 256, 0, st>>>(K2, ld, ncols, k0, nb);
-                const int down = n - (k0 + nb);
-                if (down > 0) {
-                    dim3 g(ceil_div(right, 64), ceil_div(down, 64));
-                    k_update<<<g, 256, 0, st>>>(K2, ld, n, ncols, k0, nb);
-                }
-            }
-        }
-        int info = 0;
-        RP_HIP(ctx, hipMemcpyAsync(&info, d_info.p, sizeof(int), hipMemcpyDeviceToHost, st));
-        RP_HIP(ctx, hipStreamSynchronize(st));
-        if (info != 0) {
+        const int rc = lu_solve_augmented(ctx, d_K2.as<double>(), ld, n, d_piv.as<int>(), d_info.as<int>(),
+                                          d_x.as<double>(), st);
+        if (rc != RP_OK) {
             delete kts;
-            hipEventDestroy(e0);
-            hipEventDestroy(e1);
-            return ctx->fail(RP_ESINGULAR, "KKT matrix is exactly singular (zero pivot at row " + std::to_string(info) + ")");
+            (void)hipEventDestroy(e0);
+            (void)hipEventDestroy(e1);
+            return rc;
         }
-        k_backsub<<<1, 1024, sizeof(double) * n, st>>>(K2, ld, n, d_x.as<double>());
     }
     if (used_fallback) *used_fallback = 0;
     k_expand<<<ceil_div(N, 256), 256, 0, st>>>(d_x.as<double>(), d_exp.as<int>(), N, d_q.as<double>());
@@ -677,8 +687,8 @@ int run_solve(rp_fit *fit, const rp_constraints *con, double *q_out, double *sol
     float ms = 0.f;
     RP_HIP(ctx, hipEventElapsedTime(&ms, e0, e1));
     fit->t_solve_ms = ms;
-    hipEventDestroy(e0);
-    hipEventDestroy(e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
     return RP_OK;
 }
 

@@ -2,7 +2,7 @@
 
     python -m pyrepeat_amd system.cube [--fit-type 1] [--vdw-factor 1.0] [--vdw-max 1000]
         [--cutoff 20] [--total-charge 0] [--symm-file F] [--resp-file F] [--qeq-file F]
-        [--stats] [--output F] [--fp64] [--precision fp64|split]
+        [--stats] [--output F] [--fp64] [--precision fp64|split] [--charge repeat|qeq]
 
 Every fit runs on the GPU through ``librepeat_hip.so``.  As with the reference's
 ``--gpu``, the default precision is the split (mixed) form and ``--fp64``
@@ -52,9 +52,6 @@ def main(argv=None):
     if (not args.cube_file) or (not os.path.exists(args.cube_file)):
         print(f"Error: Cube file '{args.cube_file}' not found!")
         return 1
-    if args.charge == "qeq":
-        print("Error: --charge qeq (the QEq charge model) is not part of this build; use --qeq-file restraints")
-        return 2
     from .calculator import ChargeCalculator
     precision = args.precision or ("fp64" if args.fp64 else "split")
     device = 0
@@ -69,6 +66,26 @@ def main(argv=None):
                             precision=precision, context=ctx)
     print(f"Grid Points: {int(calc._fit.info.n_total)}, Effective Atoms: {int(calc._fit.info.n_eff)}")
     print(f"  Kept {calc.n_grid_points} grid points; read + filter {time.time() - t0:.2f} s ({precision})")
+    if args.charge == "qeq":
+        # repeat.py:2735-2756
+        print("\n========================================")
+        print("Using QEq (Charge Equilibration) method")
+        print("========================================")
+        charges = calc.compute_qeq_charges()
+        print("\nQEq Charges:")
+        for i, q in enumerate(charges):
+            print(f"{i + 1:4d} {element_symbol(calc.cube['atom_index'][i]):2s} {q:12.6f}")
+        print(f"\nTotal charge: {np.sum(charges):.6f}")
+        if args.output:
+            with open(args.output, "w") as f:
+                for i, q in enumerate(charges):
+                    pos = calc.cube["atom_pos"][i]
+                    f.write(f"{i + 1:4d} {element_symbol(calc.cube['atom_index'][i]):2s} {pos[0]:12.6f} {pos[1]:12.6f} "
+                            f"{pos[2]:12.6f} {q:12.6f}\n")
+            print(f"QEq charges written to {args.output}")
+        calc.close()
+        ctx.close()
+        return 0
     charges = calc.fit_charges()
     stats = None
     if args.stats or args.output:
