@@ -45,7 +45,7 @@ extern "C" {
 
 #define RP_PREC_FP64 0   /* plain fp64: fp64 MFMA k-space GEMM, fp64 real space, fp64 Gram */
 #define RP_PREC_OZAKI 1  /* split form: reciprocal sum factorized over the lattice axes (fp64),
-                            real space with fp64 displacements and fp32 erfc, Gram from
+                            real space in fp64 (erfc as one polynomial), Gram from
                             int8 fixed-point slices on MFMA with exact int32 accumulation,
                             fp64 across chunks and in the KKT solve */
 
