@@ -128,11 +128,11 @@ struct SepPlan {
 };
 // Per accumulate: the k-index box, T1 for every plane, and the line table of the kept points.
 int sep_prepare(rp_fit *fit, FitDevice &dv, int64_t rows, SepPlan &plan);
-// Phi rows [r0, r0 + valid) of one chunk (k-space + real space + augmented columns), row-major
-// [rows][NP] fp64, with the column maxima of |Phi| (bit patterns) into colmax.
+// Phi rows [r0, r0 + valid) of one chunk (k-space + real space + augmented columns), column-major
+// [NP][rows] fp64, with the column maxima of |Phi| (bit patterns) into colmax.
 void sep_phi_chunk(rp_fit *fit, FitDevice &dv, const SepPlan &plan, int chunk, int64_t r0, int64_t valid,
-                   double *phi, unsigned long long *colmax, hipStream_t st);
-// Row-major Phi chunk -> int8 slices [4][NP][rows] with per-column power-of-two scales.
+                   int64_t rows, double *phi, unsigned long long *colmax, hipStream_t st);
+// Column-major Phi chunk -> int8 slices [4][NP][rows] with per-column power-of-two scales.
 void sep_slice_chunk(const double *phi, int NP, int64_t rows, int64_t valid, const unsigned long long *colmax,
                      int8_t *slices, double *scale_out, hipStream_t st);
 __global__ void k_symmetrize(double *gram, int NP, int tile);

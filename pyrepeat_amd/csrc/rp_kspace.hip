@@ -214,7 +214,8 @@ struct SepArgs {
     int N, NP;
     const double2 *ex, *ey;
     EpiArgs e;
-    double *phi;                // [rows][NP], chunk-local rows
+    double *phi;                // [NP][rows]: column-major, chunk-local rows
+    int64_t rows;               // leading dimension of phi
     unsigned long long *colmax;
     int items, items_per_xcd;   // line groups of the chunk; line groups per XCD
 };
@@ -294,10 +295,9 @@ __global__ __launch_bounds__(SEP_THREADS) void k_sep_phi(SepArgs p) {
     int li = 0, base = lkey[0] * nx, j = lj[0], m = lm[0];
     int64_t r = rs + s;
     const int *idx = e.idx + r;
-    double *out = p.phi + (r - p.r0) * p.NP + a;
-    const int64_t out_step = (int64_t)SH * p.NP;
+    double *out = p.phi + (int64_t)a * p.rows + (r - p.r0);
     int f_next = r < rend ? *idx : 0;
-    for (; r < rend; r += SH, idx += SH, out += out_step) {
+    for (; r < rend; r += SH, idx += SH, out += SH) {
         if (r >= lrow[li + 1]) {
             while (r >= lrow[li + 1]) ++li;
             base = lkey[li] * nx;
@@ -379,53 +379,55 @@ __global__ __launch_bounds__(SEP_THREADS) void k_sep_phi(SepArgs p) {
     if (s == 0 && cmax > 0.0 && a < p.NP) atomicMax(p.colmax + a, (unsigned long long)__double_as_longlong(cmax));
 }
 
-// Row-major Phi tile (128 rows x 32 columns) -> slices [P][NP][rows] of x / S_col, S_col the power
-// of two with max |Phi_col| / S_col <= 127/128; rows at or past `valid` are zero.  Reads are 256-byte
-// row segments, writes 128-byte runs of one column's slice.
+// Column-major Phi -> slices [P][NP][rows] of x / S_col, S_col the power of two with
+// max |Phi_col| / S_col <= 127/128; rows at or past `valid` are zero.  Both sides are streams: a
+// thread takes 16 consecutive rows of one column (128-byte read, one 16-byte write per slice).
 template <int P>
-__global__ __launch_bounds__(256) void k_slice_rows(const double *__restrict__ phi, int NP, int64_t rows, int64_t valid,
+__global__ __launch_bounds__(256) void k_slice_cols(const double *__restrict__ phi, int64_t rows, int64_t valid,
                                                     const unsigned long long *__restrict__ colmax,
                                                     int8_t *__restrict__ out, double *__restrict__ scale_out) {
-    __shared__ double tile[128][33];
-    const int t = threadIdx.x;
-    const int64_t rt = (int64_t)blockIdx.x * 128;
-    const int c0 = blockIdx.y * 32;
-#pragma unroll
-    for (int e = 0; e < 16; ++e) {
-        const int q = e * 256 + t, row = q >> 5, col = q & 31;
-        tile[row][col] = (rt + row < valid) ? phi[(rt + row) * NP + c0 + col] : 0.0;
-    }
-    __syncthreads();
-    const int col = t >> 3, part = t & 7;    // 8 threads per column, 16 rows each
-    const int a = c0 + col;
+    const int a = blockIdx.y;
+    const int64_t r0 = ((int64_t)blockIdx.x * 256 + threadIdx.x) * 16;
     const double mx = __longlong_as_double((long long)colmax[a]);
     double S = 1.0;
     if (mx > 0.0) S = ldexp(1.0, (int)ceil(log2(mx * (128.0 / 127.0))));
     while (mx / S > 127.0 / 128.0) S *= 2.0;
-    if (blockIdx.x == 0 && part == 0) scale_out[a] = S;
+    if (blockIdx.x == 0 && threadIdx.x == 0) scale_out[a] = S;
+    if (r0 >= rows) return;
     const double inv = 1.0 / S;
+    const double *src = phi + (int64_t)a * rows + r0;
+    double v[16];
+    if (r0 + 16 <= valid) {
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            const double2 t = reinterpret_cast<const double2 *>(src)[q];
+            v[2 * q] = t.x;
+            v[2 * q + 1] = t.y;
+        }
+    } else {
+#pragma unroll
+        for (int q = 0; q < 16; ++q) v[q] = r0 + q < valid ? src[q] : 0.0;
+    }
     uint32_t w[P][4];
 #pragma unroll
     for (int b = 0; b < 4; ++b)
 #pragma unroll
         for (int s = 0; s < P; ++s) w[s][b] = 0u;
 #pragma unroll
-    for (int rr = 0; rr < 16; ++rr) {
-        double r = tile[part * 16 + rr][col] * inv;
+    for (int q = 0; q < 16; ++q) {
+        double r = v[q] * inv;
 #pragma unroll
         for (int s = 0; s < P; ++s) {
             const double tv = r * 128.0;
             const double c = rint(tv);
             r = tv - c;
-            w[s][rr >> 2] |= (uint32_t)(uint8_t)(int8_t)(int)c << (8 * (rr & 3));
+            w[s][q >> 2] |= (uint32_t)(uint8_t)(int8_t)(int)c << (8 * (q & 3));
         }
     }
-    const int64_t plane = (int64_t)NP * rows;
+    const int64_t plane = (int64_t)gridDim.y * rows;
 #pragma unroll
-    for (int s = 0; s < P; ++s) {
-        uint4 v = make_uint4(w[s][0], w[s][1], w[s][2], w[s][3]);
-        *reinterpret_cast<uint4 *>(out + s * plane + (int64_t)a * rows + rt + part * 16) = v;
-    }
+    for (int s = 0; s < P; ++s)
+        *reinterpret_cast<uint4 *>(out + s * plane + (int64_t)a * rows + r0) = make_uint4(w[s][0], w[s][1], w[s][2], w[s][3]);
 }
 
 }  // namespace
@@ -479,9 +481,10 @@ int sep_prepare(rp_fit *fit, FitDevice &dv, int64_t rows, SepPlan &plan) {
     return RP_OK;
 }
 
-void sep_phi_chunk(rp_fit *fit, FitDevice &dv, const SepPlan &plan, int chunk, int64_t r0, int64_t valid, double *phi,
-                   unsigned long long *colmax, hipStream_t st) {
+void sep_phi_chunk(rp_fit *fit, FitDevice &dv, const SepPlan &plan, int chunk, int64_t r0, int64_t valid, int64_t rows,
+                   double *phi, unsigned long long *colmax, hipStream_t st) {
     SepArgs a{};
+    a.rows = rows;
     a.line_row = dv.line_row.as<int>();
     a.line_key = dv.line_key.as<int>();
     a.l_first = plan.first_line[chunk];
@@ -504,8 +507,8 @@ void sep_phi_chunk(rp_fit *fit, FitDevice &dv, const SepPlan &plan, int chunk, i
 
 void sep_slice_chunk(const double *phi, int NP, int64_t rows, int64_t valid, const unsigned long long *colmax,
                      int8_t *slices, double *scale_out, hipStream_t st) {
-    dim3 g((unsigned)(rows / 128), NP / 32);
-    k_slice_rows<4><<<g, 256, 0, st>>>(phi, NP, rows, valid, colmax, slices, scale_out);
+    dim3 g((unsigned)ceil_div(rows, 256 * 16), (unsigned)NP);
+    k_slice_cols<4><<<g, 256, 0, st>>>(phi, rows, valid, colmax, slices, scale_out);
 }
 
 }  // namespace rp

@@ -243,7 +243,7 @@ int accumulate_split(rp_fit *fit, FitDevice &dv) {
         k_zero_u64<<<ceil_div(NP, 256), 256, 0, st>>>(d_colmax.as<unsigned long long>(), NP);
         {
             KTimer kt(ctx, KT_KSPACE);
-            sep_phi_chunk(fit, dv, plan, chunk, r0, valid, d_phi.as<double>(), d_colmax.as<unsigned long long>(), st);
+            sep_phi_chunk(fit, dv, plan, chunk, r0, valid, rows, d_phi.as<double>(), d_colmax.as<unsigned long long>(), st);
         }
         {
             KTimer kts(ctx, KT_SLICE);

@@ -220,29 +220,46 @@ __global__ __launch_bounds__(256) void k_update(double *__restrict__ A, int ld, 
 
 constexpr int CB = 64;
 
+// A double from one lane of the wave (lane uniform).
+__device__ __forceinline__ double readlane_f64(double v, int lane) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)b, lane);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), lane);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+constexpr int PB = 8;   // pivots per step of the diagonal block
+
 // Diagonal block of the blocked Cholesky (64 x 64, 256 threads): factor L, its inverse X = L^-1
 // (written to Dinv, row-major, identity past the block) and the forward-substitution step of the two
 // right-hand sides, z_k = X r_k (rows k0.. of rw, rb, in place).
 // The block lives in registers: thread (tr, tc) holds rows 4 tr.. and columns 4 tc.. of the Schur
-// complement S and of W, the right-looking inverse (W = I at the start).  Pivot step j needs only
-// column j of S and row j of W, which their owners publish to LDS at the end of step j - 1; after one
-// barrier every thread forms p_j = sqrt(S_jj), L_rj = S_rj / p_j, X_j = W_j / p_j and updates its
-// blocks, S_rc -= L_rj L_cj (r, c > j) and W_r -= L_rj X_j (r > j).  Column j of S and row j of W are
-// final after step j and stay unscaled in the registers; the pivots are applied once at the end
-// (L_rc = S_rc / p_c, L_cc = p_c, X_r = W_r / p_r).  64 steps, one barrier each.
-// A pivot at or below 1e-13 of the original diagonal entry counts as a failure: the system is then
-// left to the pivoted LU, which reports exact singularity as the reference's LAPACK call does.
+// complement S and of W, the right-looking inverse (W = I at the start).  Step J takes the 8 pivots
+// j = 8J .. 8J+7 in two phases:
+//   A (wave 0, after the step's first barrier): from columns j..j+7 of S and rows j..j+7 of W, which
+//     their owners published to LDS, one lane per row forms L[r][j..j+7] (the 8 pivots one after
+//     another, the pivot rows broadcast by readlane), then one lane per column forms
+//     X[j..j+7][c] = L_jj^-1 W[j..j+7][c];
+//   B (all threads, after the second barrier): the rank-8 update S -= L L^T, W -= L X of the blocks
+//     below and right of the pivot block, the finished L and X entries into their owners' registers,
+//     and the next 8 columns of S and rows of W published.
+// Ownership is aligned with the 8-pivot blocks (two 4-wide thread blocks), so no entry is masked.
+// 8 steps, two barriers each.  A pivot at or below 1e-13 of the original diagonal entry counts as a
+// failure: the system is then left to the pivoted LU, which reports exact singularity as the
+// reference's LAPACK call does.
 __global__ __launch_bounds__(256) void k_potrf_inv(double *__restrict__ A, int ld, int n, int k0,
                                                    const double *__restrict__ orig, int *__restrict__ info,
                                                    double *__restrict__ Dinv, double *__restrict__ rw,
                                                    double *__restrict__ rb) {
-    __shared__ double col[2][CB], xrow[2][CB];
-    __shared__ double od[CB];        // original diagonal entries for the pivot test
-    __shared__ double pv[CB], pinv[CB];
+    __shared__ double colS[CB][PB + 1];    // S[r][j + q]
+    __shared__ double xr[PB][CB];          // W[j + q][c]
+    __shared__ double lbT[PB][CB];         // L[r][j + q], transposed
+    __shared__ double xb[PB][CB];          // X[j + q][c]
+    __shared__ double od[CB];              // original diagonal entries for the pivot test
     __shared__ double Xs[CB][CB + 1];
     __shared__ double zw[CB], zb[CB];
     const int nb = min(CB, n - k0);
-    const int t = threadIdx.x, tr = t >> 4, tc = t & 15;
+    const int t = threadIdx.x, tr = t >> 4, tc = t & 15, lane = t & 63, wave = t >> 6;
     const bool lower = tc <= tr;
     double a[4][4], x[4][4];
 #pragma unroll
@@ -257,89 +274,135 @@ __global__ __launch_bounds__(256) void k_potrf_inv(double *__restrict__ A, int l
         zw[t] = t < nb ? rw[k0 + t] : 0.0;
         zb[t] = t < nb ? rb[k0 + t] : 0.0;
         od[t] = t < nb ? orig[(int64_t)(k0 + t) * ld + k0 + t] : 1.0;
-        // W is lower triangular: the entries of a published row past the owners' blocks stay zero
-        xrow[1][t] = 0.0;
     }
+    // W is lower triangular: row entries right of the owners' blocks stay zero
+    for (int e = t; e < PB * CB; e += 256) xr[e / CB][e % CB] = 0.0;
     __syncthreads();
-    // column 0 of S and row 0 of W
-    if (tc == 0) {
+    // columns 0..7 of S and rows 0..7 of W
+    if (lower && tc < 2) {
 #pragma unroll
-        for (int u = 0; u < 4; ++u) col[0][4 * tr + u] = a[u][0];
-    }
-    if (tr == 0) {
+        for (int u = 0; u < 4; ++u)
 #pragma unroll
-        for (int v = 0; v < 4; ++v) xrow[0][4 * tc + v] = x[0][v];
+            for (int v = 0; v < 4; ++v) colS[4 * tr + u][4 * tc + v] = a[u][v];
     }
-    __syncthreads();
+    if (lower && tr < 2) {
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) xr[4 * tr + u][4 * tc + v] = x[u][v];
+    }
     int bad = 0;
-    for (int j = 0; j < CB; ++j) {
-        const int buf = j & 1;
-        const double d = col[buf][j];
-        double inv = 1.0, piv = 1.0;
-        if (j < nb) {
-            if (!(d > 1e-13 * fabs(od[j]))) {
-                if (!bad) bad = k0 + j + 1;
-            } else {
-                inv = rsqrt(d);
-                piv = d * inv;
-            }
-        }
-        if (t == 0) {
-            pv[j] = piv;
-            pinv[j] = inv;
-        }
-        if (lower) {
-            // masked multipliers: L_rj is zero for rows r <= j and L_cj for columns c <= j, and W's
-            // row j is zero past column j, so the update is branch-free
-            double lr[4], lc[4], xj[4];
+    for (int J = 0; J < CB / PB; ++J) {
+        const int j = J * PB;
+        __syncthreads();
+        // ---- phase A: wave 0
+        if (wave == 0) {
+            double l[PB], Ljj[PB][PB], inv[PB];
 #pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                const double l = col[buf][4 * tr + u] * inv;
-                lr[u] = 4 * tr + u > j ? l : 0.0;
-            }
+            for (int q = 0; q < PB; ++q) {
+                double acc = colS[lane][q];
 #pragma unroll
-            for (int v = 0; v < 4; ++v) {
-                const double l = col[buf][4 * tc + v] * inv;
-                lc[v] = 4 * tc + v > j ? l : 0.0;
-                xj[v] = xrow[buf][4 * tc + v] * inv;
-            }
-#pragma unroll
-            for (int u = 0; u < 4; ++u)
-#pragma unroll
-                for (int v = 0; v < 4; ++v) {
-                    a[u][v] = fma(-lr[u], lc[v], a[u][v]);
-                    x[u][v] = fma(-lr[u], xj[v], x[u][v]);
+                for (int p = 0; p < q; ++p) {
+                    Ljj[q][p] = readlane_f64(l[p], j + q);
+                    acc = fma(-l[p], Ljj[q][p], acc);
                 }
-            // publish column j + 1 of S and row j + 1 of W for the next step; rows above j + 1 of
-            // the column are never read unmasked
-            const int nj = j + 1, njb = nj >> 2, nq = nj & 3;
-            if (tc == njb) {
-#pragma unroll
-                for (int u = 0; u < 4; ++u) {
-                    const double v = nq == 0 ? a[u][0] : nq == 1 ? a[u][1] : nq == 2 ? a[u][2] : a[u][3];
-                    col[buf ^ 1][4 * tr + u] = v;
+                const double d = readlane_f64(acc, j + q);
+                double piv = 1.0;
+                if (j + q < nb) {
+                    if (!(d > 1e-13 * fabs(od[j + q]))) {
+                        if (!bad) bad = k0 + j + q + 1;
+                    } else {
+                        piv = sqrt(d);
+                    }
                 }
+                inv[q] = 1.0 / piv;
+                l[q] = lane > j + q ? acc * inv[q] : (lane == j + q ? piv : 0.0);
             }
-            if (tr == njb) {
 #pragma unroll
-                for (int v = 0; v < 4; ++v) {
-                    const double w = nq == 0 ? x[0][v] : nq == 1 ? x[1][v] : nq == 2 ? x[2][v] : x[3][v];
-                    xrow[buf ^ 1][4 * tc + v] = w;
-                }
+            for (int q = 0; q < PB; ++q) lbT[q][lane] = l[q];
+            double xv[PB];
+#pragma unroll
+            for (int q = 0; q < PB; ++q) {
+                double acc = xr[q][lane];
+#pragma unroll
+                for (int p = 0; p < q; ++p) acc = fma(-Ljj[q][p], xv[p], acc);
+                xv[q] = acc * inv[q];
+                xb[q][lane] = xv[q];
             }
         }
         __syncthreads();
+        // ---- phase B: all threads
+        if (lower && tr >= 2 * J) {
+            if (tr >= 2 * J + 2) {
+                // rows below the pivot block: W -= L X, S -= L L^T right of it, L columns into place
+                const bool upd = tc >= 2 * J + 2, put = tc >= 2 * J && tc < 2 * J + 2;
+#pragma unroll
+                for (int q = 0; q < PB; ++q) {
+                    const double2 l01 = *reinterpret_cast<const double2 *>(&lbT[q][4 * tr]);
+                    const double2 l23 = *reinterpret_cast<const double2 *>(&lbT[q][4 * tr + 2]);
+                    const double lr[4] = {l01.x, l01.y, l23.x, l23.y};
+                    const double2 x01 = *reinterpret_cast<const double2 *>(&xb[q][4 * tc]);
+                    const double2 x23 = *reinterpret_cast<const double2 *>(&xb[q][4 * tc + 2]);
+                    const double xj[4] = {x01.x, x01.y, x23.x, x23.y};
+#pragma unroll
+                    for (int u = 0; u < 4; ++u)
+#pragma unroll
+                        for (int v = 0; v < 4; ++v) x[u][v] = fma(-lr[u], xj[v], x[u][v]);
+                    if (upd) {
+                        const double2 c01 = *reinterpret_cast<const double2 *>(&lbT[q][4 * tc]);
+                        const double2 c23 = *reinterpret_cast<const double2 *>(&lbT[q][4 * tc + 2]);
+                        const double lc[4] = {c01.x, c01.y, c23.x, c23.y};
+#pragma unroll
+                        for (int u = 0; u < 4; ++u)
+#pragma unroll
+                            for (int v = 0; v < 4; ++v) a[u][v] = fma(-lr[u], lc[v], a[u][v]);
+                    }
+                }
+                if (put) {
+#pragma unroll
+                    for (int u = 0; u < 4; ++u)
+#pragma unroll
+                        for (int v = 0; v < 4; ++v) a[u][v] = lbT[4 * tc + v - j][4 * tr + u];
+                }
+            } else {
+                // rows of the pivot block: their X rows and the pivot block of L
+#pragma unroll
+                for (int u = 0; u < 4; ++u)
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) {
+                        const int r = 4 * tr + u, c = 4 * tc + v;
+                        x[u][v] = xb[r - j][c];
+                        if (tc >= 2 * J && c <= r) a[u][v] = lbT[c - j][r];
+                    }
+            }
+            // publish columns j+8.. of S and rows j+8.. of W for the next step
+            if (J + 1 < CB / PB) {
+                const int nj = j + PB;
+                if (tc >= 2 * J + 2 && tc < 2 * J + 4 && tr >= 2 * J + 2) {
+#pragma unroll
+                    for (int u = 0; u < 4; ++u)
+#pragma unroll
+                        for (int v = 0; v < 4; ++v) colS[4 * tr + u][4 * tc + v - nj] = a[u][v];
+                }
+                if (tr >= 2 * J + 2 && tr < 2 * J + 4) {
+#pragma unroll
+                    for (int u = 0; u < 4; ++u)
+#pragma unroll
+                        for (int v = 0; v < 4; ++v) xr[4 * tr + u - nj][4 * tc + v] = x[u][v];
+                }
+            }
+        }
     }
-    // ---- apply the pivots; outputs: factor, inverse, z_k = X r_k
+    // ---- outputs: factor, inverse, z_k = X r_k
 #pragma unroll
     for (int u = 0; u < 4; ++u)
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
             const int i = 4 * tr + u, c = 4 * tc + v;
-            const double xv = lower && c <= i ? x[u][v] * pinv[i] : 0.0;
+            const double xv = lower && c <= i ? x[u][v] : 0.0;
             Xs[i][c] = xv;
             Dinv[i * CB + c] = xv;
-            if (lower && i < nb && c <= i) A[(int64_t)(k0 + i) * ld + k0 + c] = c == i ? pv[c] : a[u][v] * pinv[c];
+            if (lower && i < nb && c <= i) A[(int64_t)(k0 + i) * ld + k0 + c] = a[u][v];
         }
     __syncthreads();
     if (t < nb) {
