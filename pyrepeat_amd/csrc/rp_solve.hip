@@ -247,17 +247,27 @@ constexpr int PB = 8;   // pivots per step of the diagonal block
 // 8 steps, two barriers each.  A pivot at or below 1e-13 of the original diagonal entry counts as a
 // failure: the system is then left to the pivoted LU, which reports exact singularity as the
 // reference's LAPACK call does.
-__global__ __launch_bounds__(256) void k_potrf_inv(double *__restrict__ A, int ld, int n, int k0,
-                                                   const double *__restrict__ orig, int *__restrict__ info,
-                                                   double *__restrict__ Dinv, double *__restrict__ rw,
-                                                   double *__restrict__ rb) {
-    __shared__ double colS[CB][PB + 1];    // S[r][j + q]
-    __shared__ double xr[PB][CB];          // W[j + q][c]
-    __shared__ double lbT[PB][CB];         // L[r][j + q], transposed
-    __shared__ double xb[PB][CB];          // X[j + q][c]
-    __shared__ double od[CB];              // original diagonal entries for the pivot test
-    __shared__ double Xs[CB][CB + 1];
-    __shared__ double zw[CB], zb[CB];
+struct PotrfLds {
+    double colS[CB][PB + 1];    // S[r][j + q]
+    double xr[PB][CB];          // W[j + q][c]
+    double lbT[PB][CB];         // L[r][j + q], transposed
+    double xb[PB][CB];          // X[j + q][c]
+    double od[CB];              // original diagonal entries for the pivot test
+    double Xs[CB][CB + 1];
+    double zw[CB], zb[CB];
+};
+
+__device__ __forceinline__ void potrf_block(double *__restrict__ A, int ld, int n, int k0, const double *__restrict__ orig,
+                                            int *__restrict__ info, double *__restrict__ Dinv, double *__restrict__ rw,
+                                            double *__restrict__ rb, PotrfLds &sm) {
+    auto &colS = sm.colS;
+    auto &xr = sm.xr;
+    auto &lbT = sm.lbT;
+    auto &xb = sm.xb;
+    auto &od = sm.od;
+    auto &Xs = sm.Xs;
+    auto &zw = sm.zw;
+    auto &zb = sm.zb;
     const int nb = min(CB, n - k0);
     const int t = threadIdx.x, tr = t >> 4, tc = t & 15, lane = t & 63, wave = t >> 6;
     const bool lower = tc <= tr;
@@ -417,6 +427,14 @@ __global__ __launch_bounds__(256) void k_potrf_inv(double *__restrict__ A, int l
     if (t == 0 && bad && info[0] == 0) info[0] = bad;
 }
 
+__global__ __launch_bounds__(256) void k_potrf_inv(double *__restrict__ A, int ld, int n, int k0,
+                                                   const double *__restrict__ orig, int *__restrict__ info,
+                                                   double *__restrict__ Dinv, double *__restrict__ rw,
+                                                   double *__restrict__ rb) {
+    __shared__ PotrfLds sm;
+    potrf_block(A, ld, n, k0, orig, info, Dinv, rw, rb, sm);
+}
+
 // Panel below the diagonal block: L21 = A21 L11^{-T} = A21 Dinv^T, 32 rows per workgroup, and the
 // forward-substitution update of those rows of the right-hand sides, r_i -= L21[i] . z_k.
 __global__ __launch_bounds__(256) void k_chol_trsm_inv(double *__restrict__ A, int ld, int n, int k0,
@@ -478,13 +496,17 @@ __global__ __launch_bounds__(256) void k_chol_trsm_inv(double *__restrict__ A, i
 }
 
 // A22 -= L21 L21^T on the lower triangle (64x64 tiles with tile row >= tile col).
-__global__ __launch_bounds__(256) void k_chol_update(double *__restrict__ A, int ld, int n, int k0) {
+struct UpdateLds {
+    double Ls[64][CB + 1];
+    double Us[CB][64];
+};
+
+// Trailing update of one 64 x 64 tile (ti, tj) below the panel of block column k0.
+__device__ __forceinline__ void update_tile(double *__restrict__ A, int ld, int n, int k0, int ti, int tj, UpdateLds &sm) {
     const int nb = min(CB, n - k0);
     const int r0 = k0 + nb;
-    const int ti = blockIdx.y, tj = blockIdx.x;
-    if (tj > ti) return;
-    __shared__ double Ls[64][CB + 1];
-    __shared__ double Us[CB][64];
+    auto &Ls = sm.Ls;
+    auto &Us = sm.Us;
     const int tr = r0 + ti * 64, tc = r0 + tj * 64;
     const int t = threadIdx.x;
     #pragma unroll
@@ -516,6 +538,26 @@ __global__ __launch_bounds__(256) void k_chol_update(double *__restrict__ A, int
             const int c = tc + tx + 16 * b;
             if (c < n && c <= r) A[(int64_t)r * ld + c] -= acc[a][b];
         }
+    }
+}
+
+// Trailing update after block column k0; the work-group of tile (0, 0), which is the next diagonal
+// block, then factors that block (potrf_block), so the serial factorisation of block k0 + 64 runs
+// while the other tiles of this update are still in flight.
+__global__ __launch_bounds__(256) void k_chol_update_potrf(double *__restrict__ A, int ld, int n, int k0,
+                                                           const double *__restrict__ orig, int *__restrict__ info,
+                                                           double *__restrict__ Dinv_next, double *__restrict__ rw,
+                                                           double *__restrict__ rb) {
+    __shared__ union {
+        UpdateLds up;
+        PotrfLds pf;
+    } sm;
+    const int ti = blockIdx.y, tj = blockIdx.x;
+    if (tj > ti) return;
+    update_tile(A, ld, n, k0, ti, tj, sm.up);
+    if (ti == 0 && tj == 0) {
+        __syncthreads();   // the tile's update is in A; the LDS is reused
+        potrf_block(A, ld, n, k0 + CB, orig, info, Dinv_next, rw, rb, sm.pf);
     }
 }
 
@@ -706,18 +748,20 @@ int run_solve(rp_fit *fit, const rp_constraints *con, double *q_out, double *sol
                                  hipMemcpyDeviceToDevice, st));
     RP_HIP(ctx, hipMemcpy2DAsync(d_bc.p, sizeof(double), d_K.as<double>() + n, sizeof(double) * ld, sizeof(double), nr,
                                  hipMemcpyDeviceToDevice, st));
+    // block 0 on its own; every later diagonal block is factored by the preceding trailing update
+    k_potrf_inv<<<1, 256, 0, st>>>(d_K.as<double>(), ld, nr, 0, d_K2.as<double>(), d_cinfo.as<int>(), d_dinv.as<double>(),
+                                   d_wc.as<double>(), d_bc.as<double>());
     for (int k0 = 0; k0 < nr; k0 += CB) {
         const int nb = std::min(CB, nr - k0);
         double *dinv = d_dinv.as<double>() + (int64_t)(k0 / CB) * CB * CB;
-        k_potrf_inv<<<1, 256, 0, st>>>(d_K.as<double>(), ld, nr, k0, d_K2.as<double>(), d_cinfo.as<int>(), dinv,
-                                               d_wc.as<double>(), d_bc.as<double>());
         const int below = nr - (k0 + nb);
         if (below > 0) {
             k_chol_trsm_inv<<<ceil_div(below, 32), 256, 0, st>>>(d_K.as<double>(), ld, nr, k0, dinv, d_wc.as<double>(),
                                                                  d_bc.as<double>());
             const int nt = (int)ceil_div(below, 64);
             dim3 g(nt, nt);
-            k_chol_update<<<g, 256, 0, st>>>(d_K.as<double>(), ld, nr, k0);
+            k_chol_update_potrf<<<g, 256, 0, st>>>(d_K.as<double>(), ld, nr, k0, d_K2.as<double>(), d_cinfo.as<int>(),
+                                                   dinv + CB * CB, d_wc.as<double>(), d_bc.as<double>());
         }
     }
     int cinfo = 0;
