@@ -279,7 +279,10 @@ int load_system(rp_fit *fit, const rp_system *sys, const rp_options *opt) {
 
 using namespace rp;
 
-rp_fit::~rp_fit() { delete dev; }
+rp_fit::~rp_fit() {
+    delete dev;
+    if (h_stage) (void)hipHostFree(h_stage);
+}
 
 static void destroy_ctx(rp_ctx *ctx) {
     (void)hipSetDevice(ctx->device);

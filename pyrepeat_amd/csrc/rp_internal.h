@@ -238,6 +238,9 @@ struct rp_fit {
     int32_t na = 0;       // augmented width N + 2
     // grow-only workspaces reused by every accumulate / solve / matvec of this fit
     rp::DBuf ws[32];
+    void *h_stage = nullptr;        // pinned host staging of the solve's constraint arrays (grow-only)
+    size_t h_stage_bytes = 0;
+    int split_tiles_np = 0;         // NP the split Gram's tile list in ws[9] was built for
     rp::FitDevice *dev = nullptr;   // persistent device copy of the Ewald setup
     ~rp_fit();
 };

@@ -132,6 +132,7 @@ struct GramEpi {
     const int2 *tiles;     // (tile row in units of TM, tile col in units of TN)
     int n_tiles;
     double *part;          // [split][tile][TM*TN]
+    int accumulate;        // 0: the chunk's partial overwrites the slot (first chunk), 1: adds to it
 };
 
 // All P^2 products of the P-slice operands, 2P - 1 exact groups s = p + q.
@@ -168,7 +169,9 @@ __global__ __launch_bounds__(NTHR) void k_oz_gram(GramOperands op, GramEpi ge, i
 #pragma unroll
         for (int g = 0; g < 2 * P - 1; ++g) v += (double)acc[g][r] * ldexp(1.0, -7 * (g + 2));
         const int rr = wm * 32 + M32::d_row(lane, r);
-        dst[rr * TN + c] += v * (ge.scale[m0 + rr] * sc);   // slot owned by this block across chunks
+        // slot owned by this block across chunks; the first chunk overwrites it
+        const double add = v * (ge.scale[m0 + rr] * sc);
+        dst[rr * TN + c] = ge.accumulate ? dst[rr * TN + c] + add : add;
     }
 }
 
@@ -225,10 +228,14 @@ int accumulate_split(rp_fit *fit, FitDevice &dv) {
     const int64_t group_row_max = 2LL * 127 * 64 + (int64_t)(P - 2) * 64 * 64;
     if (group_row_max * rps > 2147483647LL)
         return ctx->fail(RP_EINVAL, "split path: row split too long for exact int32 slice groups");
-    RP_CHECK(d_tiles.grow(ctx, sizeof(int2) * n_tiles));
-    RP_HIP(ctx, hipMemcpyAsync(d_tiles.p, tiles.data(), sizeof(int2) * n_tiles, hipMemcpyHostToDevice, st));
+    if (fit->split_tiles_np != NP || !d_tiles.p) {
+        fit->split_tiles_np = 0;
+        RP_CHECK(d_tiles.grow(ctx, sizeof(int2) * n_tiles));
+        RP_HIP(ctx, hipMemcpyAsync(d_tiles.p, tiles.data(), sizeof(int2) * n_tiles, hipMemcpyHostToDevice, st));
+        RP_HIP(ctx, hipStreamSynchronize(st));   // the host vector goes out of scope
+        fit->split_tiles_np = NP;
+    }
     RP_CHECK(d_part.grow(ctx, sizeof(double) * (size_t)splits * n_tiles * TM * TN));
-    RP_HIP(ctx, hipMemsetAsync(d_part.p, 0, sizeof(double) * (size_t)splits * n_tiles * TM * TN, st));
 
     const size_t lds_bytes = gram_lds_bytes<P>();
     RP_HIP(ctx, hipFuncSetAttribute(reinterpret_cast<const void *>(&k_oz_gram<P>),
@@ -251,6 +258,7 @@ int accumulate_split(rp_fit *fit, FitDevice &dv) {
                             d_slices.as<int8_t>(), d_scale.as<double>(), st);
         }
         KTimer ktg(ctx, KT_GRAM);
+        ge.accumulate = chunk > 0;
         k_oz_gram<P><<<n_tiles * splits, NTHR, lds_bytes, st>>>(go, ge, rps, splits);
     }
     {

@@ -137,16 +137,13 @@ uint64_t fit_bytes_bound(const Setup &s, int n_atoms, int esp_dtype, int precisi
     const uint64_t nr = n_reduced, n = nr + 1, ld = (uint64_t)round_up((int64_t)n + 1, 4);
     const uint64_t nsrc = std::max(n_src, 1);
     at(12, 8 * n * ld);
-    at(13, 4 * (nr + 1));
-    at(14, 4 * nsrc);
-    at(15, 8 * nsrc);
-    for (int i = 16; i <= 18; ++i) at(i, 8 * nr);
+    // packed constraint arrays: ints (red_ptr, red_src, expand_src), then doubles
+    at(13, (uint64_t)round_up((int64_t)(4 * ((nr + 1) + nsrc + N)), 8) + 8 * (nsrc + 3 * nr));
     at(19, 4 * n);
     at(20, 8 * n * ld);
     for (int i = 21; i <= 23; ++i) at(i, 8 * nr);
     at(24, 4);
     at(25, 8 * n);
-    at(26, 4 * N);
     at(27, 8 * N);
     at(28, 4);
     at(29, 8 * nr);

@@ -39,9 +39,12 @@ __global__ __launch_bounds__(256) void k_assemble(const double *__restrict__ gra
                                                   const int *__restrict__ red_ptr, const int *__restrict__ red_src,
                                                   const double *__restrict__ red_mult, const double *__restrict__ w,
                                                   const double *__restrict__ diag_add, const double *__restrict__ rhs_add,
-                                                  double q_tot, int nr, double *__restrict__ K, int ld) {
+                                                  double q_tot, int nr, double *__restrict__ K, double *__restrict__ K2,
+                                                  int ld, double *__restrict__ wc, double *__restrict__ bc,
+                                                  int *__restrict__ cinfo) {
     const int n = nr + 1;
     const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (e == 0) *cinfo = 0;
     if (e >= (int64_t)n * (n + 1)) return;
     const int I = (int)(e / (n + 1)), J = (int)(e % (n + 1));
     const double ng = gram[(int64_t)(N + 1) * NP + N + 1];
@@ -78,6 +81,9 @@ __global__ __launch_bounds__(256) void k_assemble(const double *__restrict__ gra
         }
     }
     K[(int64_t)I * ld + J] = val;
+    K2[(int64_t)I * ld + J] = val;   // kept for the LU fallback
+    if (I < nr && J == nr) wc[I] = val;
+    if (I < nr && J == n) bc[I] = val;
 }
 
 // Factor columns [k0, k0+nb) of rows [k0, n): pivot search, in-panel swap, scale, rank-1 update.
@@ -569,7 +575,8 @@ __global__ __launch_bounds__(256) void k_chol_update_potrf(double *__restrict__ 
 __global__ __launch_bounds__(1024) void k_chol_finish(const double *__restrict__ A, int ld, int n,
                                                       const double *__restrict__ Dinv, const double *__restrict__ zw,
                                                       const double *__restrict__ zb, double q_tot,
-                                                      double *__restrict__ x) {
+                                                      double *__restrict__ x, const int *__restrict__ cinfo) {
+    if (*cinfo != 0) return;   // a pivot failed: the host reruns the system through the LU
     __shared__ double red[2][1024];
     __shared__ double part[16][CB];
     __shared__ double Dk[CB][CB + 1];
@@ -627,11 +634,6 @@ __global__ __launch_bounds__(1024) void k_chol_finish(const double *__restrict__
         __syncthreads();
     }
     if (t == 0) x[n] = lambda;
-}
-
-__global__ void k_copy(const double *__restrict__ a, double *__restrict__ b, int64_t n) {
-    const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (e < n) b[e] = a[e];
 }
 
 // Back substitution U x = y (y = column n of the factored augmented matrix); x -> out.
@@ -698,29 +700,41 @@ int run_solve(rp_fit *fit, const rp_constraints *con, double *q_out, double *sol
     const int n = nr + 1;
     const int ncols = n + 1;
     const int ld = (int)round_up(ncols, 4);
-    DBuf &d_K = fit->ws[12], &d_ptr = fit->ws[13], &d_src = fit->ws[14], &d_mult = fit->ws[15], &d_w = fit->ws[16],
-         &d_diag = fit->ws[17], &d_rhs = fit->ws[18], &d_piv = fit->ws[19];
-    DBuf &d_info = fit->ws[24], &d_x = fit->ws[25], &d_exp = fit->ws[26], &d_q = fit->ws[27];
+    // the constraint arrays in one pinned block, one upload: ints (red_ptr, red_src, expand_src), then
+    // doubles (red_mult, weights, diag_add, rhs_add)
+    const size_t n_int = (size_t)(nr + 1) + nsrc + N;
+    const size_t off_d = round_up((int64_t)(4 * n_int), 8);
+    const size_t stage = off_d + 8 * ((size_t)nsrc + 3 * (size_t)nr);
+    if (fit->h_stage_bytes < stage) {
+        if (fit->h_stage) (void)hipHostFree(fit->h_stage);
+        fit->h_stage = nullptr;
+        fit->h_stage_bytes = 0;
+        RP_HIP(ctx, hipHostMalloc(&fit->h_stage, stage, hipHostMallocDefault));
+        fit->h_stage_bytes = stage;
+    }
+    {
+        int *hi = static_cast<int *>(fit->h_stage);
+        std::copy(con->red_ptr, con->red_ptr + nr + 1, hi);
+        std::copy(con->red_src, con->red_src + nsrc, hi + nr + 1);
+        std::copy(con->expand_src, con->expand_src + N, hi + nr + 1 + nsrc);
+        double *hd = reinterpret_cast<double *>(static_cast<char *>(fit->h_stage) + off_d);
+        std::copy(con->red_mult, con->red_mult + nsrc, hd);
+        std::copy(con->weights, con->weights + nr, hd + nsrc);
+        if (con->diag_add) std::copy(con->diag_add, con->diag_add + nr, hd + nsrc + nr);
+        if (con->rhs_add) std::copy(con->rhs_add, con->rhs_add + nr, hd + nsrc + 2 * nr);
+    }
+    DBuf &d_K = fit->ws[12], &d_con = fit->ws[13], &d_piv = fit->ws[19];
+    DBuf &d_info = fit->ws[24], &d_x = fit->ws[25], &d_q = fit->ws[27];
     RP_CHECK(d_K.grow(ctx, sizeof(double) * (size_t)n * ld));
-    RP_CHECK(d_ptr.grow(ctx, sizeof(int) * (nr + 1)));
-    RP_CHECK(d_src.grow(ctx, sizeof(int) * std::max(nsrc, 1)));
-    RP_CHECK(d_mult.grow(ctx, sizeof(double) * std::max(nsrc, 1)));
-    RP_CHECK(d_w.grow(ctx, sizeof(double) * nr));
-    RP_CHECK(d_diag.grow(ctx, sizeof(double) * nr));
-    RP_CHECK(d_rhs.grow(ctx, sizeof(double) * nr));
+    RP_CHECK(d_con.grow(ctx, stage));
     RP_CHECK(d_piv.grow(ctx, sizeof(int) * n));
     RP_CHECK(d_info.grow(ctx, sizeof(int)));
     RP_CHECK(d_x.grow(ctx, sizeof(double) * n));
-    RP_CHECK(d_exp.grow(ctx, sizeof(int) * N));
     RP_CHECK(d_q.grow(ctx, sizeof(double) * N));
-    RP_HIP(ctx, hipMemcpyAsync(d_ptr.p, con->red_ptr, sizeof(int) * (nr + 1), hipMemcpyHostToDevice, st));
-    RP_HIP(ctx, hipMemcpyAsync(d_src.p, con->red_src, sizeof(int) * nsrc, hipMemcpyHostToDevice, st));
-    RP_HIP(ctx, hipMemcpyAsync(d_mult.p, con->red_mult, sizeof(double) * nsrc, hipMemcpyHostToDevice, st));
-    RP_HIP(ctx, hipMemcpyAsync(d_w.p, con->weights, sizeof(double) * nr, hipMemcpyHostToDevice, st));
-    if (con->diag_add) RP_HIP(ctx, hipMemcpyAsync(d_diag.p, con->diag_add, sizeof(double) * nr, hipMemcpyHostToDevice, st));
-    if (con->rhs_add) RP_HIP(ctx, hipMemcpyAsync(d_rhs.p, con->rhs_add, sizeof(double) * nr, hipMemcpyHostToDevice, st));
-    RP_HIP(ctx, hipMemcpyAsync(d_exp.p, con->expand_src, sizeof(int) * N, hipMemcpyHostToDevice, st));
-    RP_HIP(ctx, hipMemsetAsync(d_info.p, 0, sizeof(int), st));
+    RP_HIP(ctx, hipMemcpyAsync(d_con.p, fit->h_stage, stage, hipMemcpyHostToDevice, st));
+    const int *dc_ptr = d_con.as<int>(), *dc_src = dc_ptr + nr + 1, *dc_exp = dc_src + nsrc;
+    const double *dc_d = reinterpret_cast<const double *>(static_cast<const char *>(d_con.p) + off_d);
+    const double *dc_mult = dc_d, *dc_w = dc_d + nsrc, *dc_diag = dc_w + nr, *dc_rhs = dc_diag + nr;
 
     hipEvent_t e0, e1;
     RP_HIP(ctx, hipEventCreate(&e0));
@@ -728,12 +742,6 @@ int run_solve(rp_fit *fit, const rp_constraints *con, double *q_out, double *sol
     RP_HIP(ctx, hipEventRecord(e0, st));
     KTimer *kts = new KTimer(ctx, KT_SOLVE);
     const int NPg = (int)round_up(N + 2, 128);
-    k_assemble<<<ceil_div((int64_t)n * ncols, 256), 256, 0, st>>>(fit->d_gram.as<double>(), NPg, N, d_ptr.as<int>(), d_src.as<int>(),
-                                                                  d_mult.as<double>(), d_w.as<double>(),
-                                                                  con->diag_add ? d_diag.as<double>() : nullptr,
-                                                                  con->rhs_add ? d_rhs.as<double>() : nullptr, con->q_tot, nr,
-                                                                  d_K.as<double>(), ld);
-    // ---- Cholesky of A (leading nr x nr block) + Schur step
     DBuf &d_K2 = fit->ws[20], &d_wc = fit->ws[23];
     DBuf &d_cinfo = fit->ws[28], &d_bc = fit->ws[29], &d_dinv = fit->ws[30];
     RP_CHECK(d_K2.grow(ctx, sizeof(double) * (size_t)n * ld));
@@ -741,13 +749,13 @@ int run_solve(rp_fit *fit, const rp_constraints *con, double *q_out, double *sol
     RP_CHECK(d_wc.grow(ctx, sizeof(double) * nr));
     RP_CHECK(d_bc.grow(ctx, sizeof(double) * nr));
     RP_CHECK(d_dinv.grow(ctx, sizeof(double) * CB * CB * (size_t)ceil_div(nr, CB)));
-    RP_HIP(ctx, hipMemsetAsync(d_cinfo.p, 0, sizeof(int), st));
-    k_copy<<<ceil_div((int64_t)n * ld, 256), 256, 0, st>>>(d_K.as<double>(), d_K2.as<double>(), (int64_t)n * ld);
-    // the w and b columns before factorisation
-    RP_HIP(ctx, hipMemcpy2DAsync(d_wc.p, sizeof(double), d_K.as<double>() + nr, sizeof(double) * ld, sizeof(double), nr,
-                                 hipMemcpyDeviceToDevice, st));
-    RP_HIP(ctx, hipMemcpy2DAsync(d_bc.p, sizeof(double), d_K.as<double>() + n, sizeof(double) * ld, sizeof(double), nr,
-                                 hipMemcpyDeviceToDevice, st));
+    // the bordered system, its copy for the LU fallback and the w and b columns of the Cholesky
+    k_assemble<<<ceil_div((int64_t)n * ncols, 256), 256, 0, st>>>(fit->d_gram.as<double>(), NPg, N, dc_ptr, dc_src, dc_mult,
+                                                                  dc_w, con->diag_add ? dc_diag : nullptr,
+                                                                  con->rhs_add ? dc_rhs : nullptr, con->q_tot, nr,
+                                                                  d_K.as<double>(), d_K2.as<double>(), ld, d_wc.as<double>(),
+                                                                  d_bc.as<double>(), d_cinfo.as<int>());
+    // ---- Cholesky of A (leading nr x nr block) + Schur step
     // block 0 on its own; every later diagonal block is factored by the preceding trailing update
     k_potrf_inv<<<1, 256, 0, st>>>(d_K.as<double>(), ld, nr, 0, d_K2.as<double>(), d_cinfo.as<int>(), d_dinv.as<double>(),
                                    d_wc.as<double>(), d_bc.as<double>());
@@ -764,14 +772,20 @@ int run_solve(rp_fit *fit, const rp_constraints *con, double *q_out, double *sol
                                                    dinv + CB * CB, d_wc.as<double>(), d_bc.as<double>());
         }
     }
+    // the multiplier and backward solve (skipped on the device when a pivot failed), expansion,
+    // results and the pivot flag in one round trip
+    k_chol_finish<<<1, 1024, 0, st>>>(d_K.as<double>(), ld, nr, d_dinv.as<double>(), d_wc.as<double>(), d_bc.as<double>(),
+                                      con->q_tot, d_x.as<double>(), d_cinfo.as<int>());
+    k_expand<<<ceil_div(N, 256), 256, 0, st>>>(d_x.as<double>(), dc_exp, N, d_q.as<double>());
+    RP_HIP(ctx, hipGetLastError());
+    RP_HIP(ctx, hipEventRecord(e1, st));
     int cinfo = 0;
     RP_HIP(ctx, hipMemcpyAsync(&cinfo, d_cinfo.p, sizeof(int), hipMemcpyDeviceToHost, st));
+    RP_HIP(ctx, hipMemcpyAsync(q_out, d_q.p, sizeof(double) * N, hipMemcpyDeviceToHost, st));
+    if (solution_out) RP_HIP(ctx, hipMemcpyAsync(solution_out, d_x.p, sizeof(double) * n, hipMemcpyDeviceToHost, st));
     RP_HIP(ctx, hipStreamSynchronize(st));
     fit->lu_fallback = cinfo != 0;
-    if (cinfo == 0) {
-        k_chol_finish<<<1, 1024, 0, st>>>(d_K.as<double>(), ld, nr, d_dinv.as<double>(), d_wc.as<double>(), d_bc.as<double>(),
-                                          con->q_tot, d_x.as<double>());
-    } else {
+    if (cinfo != 0) {
         // indefinite or singular A: LU with partial pivoting on the bordered system
         const int rc = lu_solve_augmented(ctx, d_K2.as<double>(), ld, n, d_piv.as<int>(), d_info.as<int>(),
                                           d_x.as<double>(), st);
@@ -781,15 +795,15 @@ int run_solve(rp_fit *fit, const rp_constraints *con, double *q_out, double *sol
             (void)hipEventDestroy(e1);
             return rc;
         }
+        k_expand<<<ceil_div(N, 256), 256, 0, st>>>(d_x.as<double>(), dc_exp, N, d_q.as<double>());
+        RP_HIP(ctx, hipGetLastError());
+        RP_HIP(ctx, hipEventRecord(e1, st));
+        RP_HIP(ctx, hipMemcpyAsync(q_out, d_q.p, sizeof(double) * N, hipMemcpyDeviceToHost, st));
+        if (solution_out) RP_HIP(ctx, hipMemcpyAsync(solution_out, d_x.p, sizeof(double) * n, hipMemcpyDeviceToHost, st));
+        RP_HIP(ctx, hipStreamSynchronize(st));
     }
     if (used_fallback) *used_fallback = 0;
-    k_expand<<<ceil_div(N, 256), 256, 0, st>>>(d_x.as<double>(), d_exp.as<int>(), N, d_q.as<double>());
     delete kts;
-    RP_HIP(ctx, hipGetLastError());
-    RP_HIP(ctx, hipEventRecord(e1, st));
-    RP_HIP(ctx, hipMemcpyAsync(q_out, d_q.p, sizeof(double) * N, hipMemcpyDeviceToHost, st));
-    if (solution_out) RP_HIP(ctx, hipMemcpyAsync(solution_out, d_x.p, sizeof(double) * n, hipMemcpyDeviceToHost, st));
-    RP_HIP(ctx, hipStreamSynchronize(st));
     ctx->collect();
     float ms = 0.f;
     RP_HIP(ctx, hipEventElapsedTime(&ms, e0, e1));
