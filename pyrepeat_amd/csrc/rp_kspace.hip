@@ -77,7 +77,7 @@ constexpr int SBOX = SH * SK;   // T1 entries per (plane, atom)
 constexpr int SEP_WAVES = 4;
 constexpr int SEP_ATOMS = 8 * SEP_WAVES;     // atoms per work-group, 8 per wave
 constexpr int SEP_THREADS = 64 * SEP_WAVES;
-constexpr int SEP_LINES = 8;
+constexpr int SEP_LINES = 5;   // lines per work-group: 23 KB of T2 rows, 6 work-groups per CU
 constexpr int T1_PLANES = 16;   // planes per T1 work-group
 
 __device__ __forceinline__ double2 cmac(double2 acc, double2 a, double2 b) {

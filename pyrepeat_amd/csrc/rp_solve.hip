@@ -777,6 +777,8 @@ int run_solve(rp_fit *fit, const rp_constraints *con, double *q_out, double *sol
     k_chol_finish<<<1, 1024, 0, st>>>(d_K.as<double>(), ld, nr, d_dinv.as<double>(), d_wc.as<double>(), d_bc.as<double>(),
                                       con->q_tot, d_x.as<double>(), d_cinfo.as<int>());
     k_expand<<<ceil_div(N, 256), 256, 0, st>>>(d_x.as<double>(), dc_exp, N, d_q.as<double>());
+    delete kts;   // records the end event before the synchronisation below
+    kts = nullptr;
     RP_HIP(ctx, hipGetLastError());
     RP_HIP(ctx, hipEventRecord(e1, st));
     int cinfo = 0;
@@ -790,7 +792,6 @@ int run_solve(rp_fit *fit, const rp_constraints *con, double *q_out, double *sol
         const int rc = lu_solve_augmented(ctx, d_K2.as<double>(), ld, n, d_piv.as<int>(), d_info.as<int>(),
                                           d_x.as<double>(), st);
         if (rc != RP_OK) {
-            delete kts;
             (void)hipEventDestroy(e0);
             (void)hipEventDestroy(e1);
             return rc;
@@ -803,7 +804,6 @@ int run_solve(rp_fit *fit, const rp_constraints *con, double *q_out, double *sol
         RP_HIP(ctx, hipStreamSynchronize(st));
     }
     if (used_fallback) *used_fallback = 0;
-    delete kts;
     ctx->collect();
     float ms = 0.f;
     RP_HIP(ctx, hipEventElapsedTime(&ms, e0, e1));
