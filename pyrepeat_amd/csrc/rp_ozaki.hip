@@ -125,6 +125,109 @@ __device__ __forceinline__ void gram_mainloop(const GramOperands &op, int64_t m0
     }
 }
 
+// Same sums with the operands brought in by direct global-to-LDS loads (global_load_lds_dwordx4):
+// no register staging, NSTAGE stage buffers, loads issued two stages ahead.  A stage holds, per
+// slice, TM (A) and TN (B) rows of TK = 64 bytes, unpadded; 16-byte quarter q of row r is stored at
+// quarter q ^ ((r >> 2) & 3) so the fragment reads of 16 consecutive rows fall on distinct banks.
+// One load instruction fills 1 KB (16 rows): lane l writes base + 16 l, i.e. row 16 c + l / 4,
+// stored quarter l % 4, and so fetches logical quarter (l % 4) ^ ((row >> 2) & 3).
+constexpr int NSTAGE = 3;
+typedef __attribute__((address_space(3))) void lds_void;
+
+template <int P>
+constexpr size_t gram_stage_bytes() { return (size_t)P * (TM + TN) * TK; }
+template <int P>
+constexpr size_t gram_lds_bytes_dma() { return NSTAGE * gram_stage_bytes<P>(); }   // 144 KiB (P = 4)
+
+// s_waitcnt with only the vector-memory count (gfx9 encoding: vmcnt bits [3:0] and [15:14],
+// expcnt [6:4] and lgkmcnt [11:8] left at their maxima).
+template <int N>
+__device__ __forceinline__ void wait_vmcnt() {
+    static_assert(N >= 0 && N < 64, "vmcnt");
+    __builtin_amdgcn_s_waitcnt((N & 15) | (7 << 4) | (15 << 8) | ((N >> 4) << 14));
+}
+
+// Work-group barrier for LDS traffic only: this wave's LDS reads and writes are complete, then
+// s_barrier.  __syncthreads() would also wait for every vector-memory operation in flight, i.e. for
+// the direct loads of the stages ahead.
+__device__ __forceinline__ void lds_barrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
+
+template <int P>
+__device__ __forceinline__ void gram_mainloop_dma(const GramOperands &op, int64_t m0, int64_t n0, int k0, int k1,
+                                                  ivec16 (&acc)[2 * P - 1], int8_t *lds) {
+    constexpr int WAVES = NTHR / 64;
+    constexpr int CHUNKS = P * (TM + TN) / 16;
+    constexpr int PER_WAVE = CHUNKS / WAVES;
+    static_assert(CHUNKS % WAVES == 0, "load chunks per wave");
+    constexpr size_t STAGE = gram_stage_bytes<P>();
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    const int wm = wave >> 2, wn = wave & 3;
+#pragma unroll
+    for (int g = 0; g < 2 * P - 1; ++g)
+#pragma unroll
+        for (int r = 0; r < 16; ++r) acc[g][r] = 0;
+    const int nk = (k1 - k0) / TK;
+    if (nk <= 0) return;
+    const int lrow = lane >> 2, qst = lane & 3;
+    auto issue = [&](int kt) {
+        int8_t *buf = lds + (size_t)(kt % NSTAGE) * STAGE;
+        const int kbase = k0 + kt * TK;
+#pragma unroll
+        for (int u = 0; u < PER_WAVE; ++u) {
+            const int ii = wave + WAVES * u;   // chunk of this load instruction (wave-uniform)
+            int p, row;
+            int64_t col0;
+            int8_t *dst;
+            if (ii < P * TM / 16) {
+                p = ii / (TM / 16);
+                const int rc = ii % (TM / 16);
+                row = 16 * rc + lrow;
+                col0 = m0;
+                dst = buf + (size_t)(p * TM + 16 * rc) * TK;
+            } else {
+                const int jj = ii - P * TM / 16;
+                p = jj / (TN / 16);
+                const int rc = jj % (TN / 16);
+                row = 16 * rc + lrow;
+                col0 = n0;
+                dst = buf + (size_t)P * TM * TK + (size_t)(p * TN + 16 * rc) * TK;
+            }
+            const int qlog = qst ^ ((row >> 2) & 3);
+            const int8_t *src = op.S + p * op.plane + (col0 + row) * op.ld + kbase + 16 * qlog;
+            __builtin_amdgcn_global_load_lds(src, (lds_void *)dst, 16, 0, 0);
+        }
+    };
+    issue(0);
+    if (nk > 1) issue(1);
+    const int ra = wm * 32 + (lane & 31), rb = wn * 32 + (lane & 31);
+    for (int kt = 0; kt < nk; ++kt) {
+        // this wave's loads of stage kt are done when at most the next stage's are outstanding;
+        // the barrier makes every wave's visible and frees the buffer the next issue overwrites
+        if (kt + 1 < nk) wait_vmcnt<PER_WAVE>();
+        else wait_vmcnt<0>();
+        lds_barrier();
+        if (kt + 2 < nk) issue(kt + 2);
+        const int8_t *As = lds + (size_t)(kt % NSTAGE) * STAGE;
+        const int8_t *Bs = As + (size_t)P * TM * TK;
+#pragma unroll
+        for (int h = 0; h < TK / 32; ++h) {
+            const int qa = (2 * h + (lane >> 5)) ^ ((ra >> 2) & 3);
+            const int qb = (2 * h + (lane >> 5)) ^ ((rb >> 2) & 3);
+            ivec4 af[P], bf[P];
+#pragma unroll
+            for (int p = 0; p < P; ++p) {
+                af[p] = *reinterpret_cast<const ivec4 *>(As + (size_t)(p * TM + ra) * TK + 16 * qa);
+                bf[p] = *reinterpret_cast<const ivec4 *>(Bs + (size_t)(p * TN + rb) * TK + 16 * qb);
+            }
+#pragma unroll
+            for (int p = 0; p < P; ++p)
+#pragma unroll
+                for (int q = 0; q < P; ++q)
+                    acc[p + q] = __builtin_amdgcn_mfma_i32_32x32x32_i8(af[p], bf[q], acc[p + q], 0, 0, 0);
+        }
+    }
+}
+
 // ---------------------------------------------------------------- Gram kernels
 
 struct GramEpi {
@@ -156,7 +259,7 @@ __global__ __launch_bounds__(NTHR) void k_oz_gram(GramOperands op, GramEpi ge, i
     const int64_t m0 = (int64_t)tile.x * TM, n0 = (int64_t)tile.y * TN;
     // this split covers k (= grid rows) [split * rps, (split+1) * rps)
     ivec16 acc[2 * P - 1];
-    gram_mainloop<P>(op, m0, n0, split * rows_per_split, (split + 1) * rows_per_split, acc, lds);
+    gram_mainloop_dma<P>(op, m0, n0, split * rows_per_split, (split + 1) * rows_per_split, acc, lds);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int wm = wave >> 2, wn = wave & 3;
     double *dst = ge.part + ((int64_t)split * ge.n_tiles + tix) * (TM * TN);
@@ -237,7 +340,7 @@ int accumulate_split(rp_fit *fit, FitDevice &dv) {
     }
     RP_CHECK(d_part.grow(ctx, sizeof(double) * (size_t)splits * n_tiles * TM * TN));
 
-    const size_t lds_bytes = gram_lds_bytes<P>();
+    const size_t lds_bytes = gram_lds_bytes_dma<P>();
     RP_HIP(ctx, hipFuncSetAttribute(reinterpret_cast<const void *>(&k_oz_gram<P>),
                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes));
     GramOperands go{d_slices.as<int8_t>(), (int64_t)NP * rows, rows};
