@@ -12,10 +12,10 @@
 // Use (fit_charges steps 1-3, repeat.py:2195-2262): the Gram [Phi|V|1]^T [Phi|V|1] of each chunk of
 // kept points, from Phi sliced per column (scale from the column maximum over the chunk), upper
 // tiles, 4 slices of each operand (16 products, 7 exact groups).  Phi itself comes from the
-// factorized reciprocal sum and the fp32 real-space sum (rp_kspace.hip).
+// factorized reciprocal sum and the fp64 real-space sum (rp_kspace.hip).
 //
 // MFMA: v_mfma_i32_32x32x32_i8, one 32x32 tile per wave, 8 waves per block (2 x 4)
-// -> 64x128 block tile, k-step 32, slices staged through LDS, double buffered.
+// -> 64x128 block tile, k-step 32, slices loaded straight into LDS, three stage buffers.
 // Seven exact int32 group accumulators per wave tile.
 #include <algorithm>
 #include <cmath>
@@ -28,7 +28,6 @@ namespace rp {
 namespace {
 
 constexpr int TM = 64, TN = 128, TK = 64;
-constexpr int TKP = TK + 16;   // padded LDS row (bytes)
 constexpr int NTHR = 512;
 
 // Slices [P][NP][rows] of the chunk's Phi (column-major per slice: rows contiguous).
@@ -38,95 +37,10 @@ struct GramOperands {
     int64_t ld;      // rows
 };
 
-template <int P>
-constexpr size_t gram_lds_bytes() { return 2 * P * (TM + TN) * TKP; }   // 120 KiB (P = 4), 150 KiB (P = 5)
-
 // Main loop: the exact int32 group sums I_s = sum_{p+q=s} A_p^T B_q (s = 0 .. 2P-2) of the block
 // tile (columns m0.. of A, n0.. of B) over k (= grid rows) [k0, k1), a multiple of TK.  Eight waves
-// as 2 x 4, one 32 x 32 tile each.  LDS rows are padded to TKP bytes so the 16-byte fragment reads
-// of 16 consecutive lanes fall on distinct banks.  Each stage holds TK = 64 k (two MFMA k-steps);
-// the next stage is loaded into registers while the current one is consumed.
-template <int P>
-__device__ __forceinline__ void gram_mainloop(const GramOperands &op, int64_t m0, int64_t n0, int k0, int k1,
-                                              ivec16 (&acc)[2 * P - 1], int8_t *lds) {
-    constexpr int NA = (256 * P + NTHR - 1) / NTHR, NB = (512 * P + NTHR - 1) / NTHR;
-    int8_t(*As)[P][TM][TKP] = reinterpret_cast<int8_t(*)[P][TM][TKP]>(lds);
-    int8_t(*Bs)[P][TN][TKP] = reinterpret_cast<int8_t(*)[P][TN][TKP]>(lds + 2 * P * TM * TKP);
-    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
-    const int wm = wave >> 2, wn = wave & 3;
-#pragma unroll
-    for (int g = 0; g < 2 * P - 1; ++g)
-#pragma unroll
-        for (int r = 0; r < 16; ++r) acc[g][r] = 0;
-    const int nk = (k1 - k0) / TK;
-    // staging, 16-byte chunks: A c in [0, 256 P): slice c/256, row (c%256)/4, quarter c%4;
-    //                          B c in [0, 512 P): slice c/512, row (c%512)/4, quarter c%4
-    ivec4 ra[NA], rb[NB];
-    auto gload = [&](int kbase) {
-#pragma unroll
-        for (int u = 0; u < NA; ++u) {
-            const int c = t + NTHR * u;
-            if (c < 256 * P) {
-                const int p = c >> 8, row = (c & 255) >> 2, qq = c & 3;
-                ra[u] = *reinterpret_cast<const ivec4 *>(op.S + p * op.plane + (m0 + row) * op.ld + kbase + 16 * qq);
-            }
-        }
-#pragma unroll
-        for (int u = 0; u < NB; ++u) {
-            const int c = t + NTHR * u;
-            if (c < 512 * P) {
-                const int p = c >> 9, row = (c & 511) >> 2, qq = c & 3;
-                rb[u] = *reinterpret_cast<const ivec4 *>(op.S + p * op.plane + (n0 + row) * op.ld + kbase + 16 * qq);
-            }
-        }
-    };
-    auto lstore = [&](int buf) {
-#pragma unroll
-        for (int u = 0; u < NA; ++u) {
-            const int c = t + NTHR * u;
-            if (c < 256 * P) {
-                const int p = c >> 8, row = (c & 255) >> 2, qq = c & 3;
-                *reinterpret_cast<ivec4 *>(&As[buf][p][row][16 * qq]) = ra[u];
-            }
-        }
-#pragma unroll
-        for (int u = 0; u < NB; ++u) {
-            const int c = t + NTHR * u;
-            if (c < 512 * P) {
-                const int p = c >> 9, row = (c & 511) >> 2, qq = c & 3;
-                *reinterpret_cast<ivec4 *>(&Bs[buf][p][row][16 * qq]) = rb[u];
-            }
-        }
-    };
-    if (nk <= 0) return;
-    gload(k0);
-    lstore(0);
-    __syncthreads();
-    for (int kt = 0; kt < nk; ++kt) {
-        const int cur = kt & 1;
-        const bool more = kt + 1 < nk;
-        if (more) gload(k0 + (kt + 1) * TK);
-#pragma unroll
-        for (int h = 0; h < TK / 32; ++h) {
-            ivec4 af[P], bf[P];
-#pragma unroll
-            for (int p = 0; p < P; ++p) {
-                af[p] = *reinterpret_cast<const ivec4 *>(&As[cur][p][wm * 32 + (lane & 31)][32 * h + 16 * (lane >> 5)]);
-                bf[p] = *reinterpret_cast<const ivec4 *>(&Bs[cur][p][wn * 32 + (lane & 31)][32 * h + 16 * (lane >> 5)]);
-            }
-#pragma unroll
-            for (int p = 0; p < P; ++p)
-#pragma unroll
-                for (int q = 0; q < P; ++q)
-                    acc[p + q] = __builtin_amdgcn_mfma_i32_32x32x32_i8(af[p], bf[q], acc[p + q], 0, 0, 0);
-        }
-        if (more) lstore(cur ^ 1);
-        __syncthreads();
-    }
-}
-
-// Same sums with the operands brought in by direct global-to-LDS loads (global_load_lds_dwordx4):
-// no register staging, NSTAGE stage buffers, loads issued two stages ahead.  A stage holds, per
+// as 2 x 4, one 32 x 32 tile each.  The operands come by direct global-to-LDS loads
+// (global_load_lds_dwordx4): no register staging, NSTAGE stage buffers, loads issued two stages ahead.  A stage holds, per
 // slice, TM (A) and TN (B) rows of TK = 64 bytes, unpadded; 16-byte quarter q of row r is stored at
 // quarter q ^ ((r >> 2) & 3) so the fragment reads of 16 consecutive rows fall on distinct banks.
 // One load instruction fills 1 KB (16 rows): lane l writes base + 16 l, i.e. row 16 c + l / 4,
