@@ -192,12 +192,19 @@ __global__ __launch_bounds__(NTHR) void k_oz_gram(GramOperands op, GramEpi ge, i
     }
 }
 
+// Sum of the row-split partials into the Gram: block (tile, part of REDUCE_PARTS), one entry per
+// thread, the splits summed in order.
+constexpr int REDUCE_PARTS = TM * TN / 1024;
 __global__ __launch_bounds__(256) void k_oz_reduce(const double *__restrict__ part, int n_splits, int n_tiles,
                                                    const int2 *__restrict__ tiles, double *__restrict__ gram, int NP) {
-    const int2 tile = tiles[blockIdx.x];
-    for (int e = threadIdx.x; e < TM * TN; e += 256) {
+    const int tix = blockIdx.x / REDUCE_PARTS;
+    const int2 tile = tiles[tix];
+    const int e0 = (blockIdx.x % REDUCE_PARTS) * 1024;
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+        const int e = e0 + u * 256 + threadIdx.x;
         double s = 0.0;
-        for (int sp = 0; sp < n_splits; ++sp) s += part[((int64_t)sp * n_tiles + blockIdx.x) * (TM * TN) + e];
+        for (int sp = 0; sp < n_splits; ++sp) s += part[((int64_t)sp * n_tiles + tix) * (TM * TN) + e];
         gram[(int64_t)(tile.x * TM + e / TN) * NP + tile.y * TN + (e % TN)] += s;
     }
 }
@@ -280,7 +287,7 @@ int accumulate_split(rp_fit *fit, FitDevice &dv) {
     }
     {
         KTimer ktr(ctx, KT_REDUCE);
-        k_oz_reduce<<<n_tiles, 256, 0, st>>>(d_part.as<double>(), splits, n_tiles, d_tiles.as<int2>(),
+        k_oz_reduce<<<n_tiles * REDUCE_PARTS, 256, 0, st>>>(d_part.as<double>(), splits, n_tiles, d_tiles.as<int2>(),
                                               fit->d_gram.as<double>(), NP);
     }
     RP_HIP(ctx, hipGetLastError());
