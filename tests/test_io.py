@@ -179,3 +179,35 @@ def test_locpot_name_falls_back_to_cube(tmp_path, capsys):
     c = read_data(str(p))
     assert "Failed to read as VASP file" in capsys.readouterr().out
     assert c["esp_order"] == _abi.RP_ORDER_CUBE and c["n_atoms"] == 2
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision", ["fp64", "split"])
+def test_locpot_fit_equals_cube_order_fit(tmp_path, gpu_ctx, oracle, precision):
+    """A LOCPOT goes to the device in its x-fastest order (RP_ORDER_FLAT); the fit must be bitwise the
+    fit of the same values in cube order, and match the oracle."""
+    from pyrepeat_amd.calculator import ChargeCalculator
+    s = synth.tiny(seed=31, n_atoms=6, n=14, L=11.0, triclinic=True)
+    nx, ny, nz = (int(v) for v in s.n_grid)
+    lat_ang = np.asarray(s.axis) * np.array([nx, ny, nz])[:, None] / ANG_TO_BOHR
+    frac = np.linalg.solve(np.asarray(s.axis).T * np.array([nx, ny, nz]), np.asarray(s.atom_pos).T).T % 1.0
+    syms = {1: "H", 6: "C", 7: "N", 8: "O"}
+    order = np.argsort(s.atomic_numbers, kind="stable")
+    types, counts = np.unique(np.asarray(s.atomic_numbers)[order], return_counts=True)
+    v_ev = np.asarray(s.esp_flat(), dtype=np.float64) / LOCPOT_FACTOR
+    path = tmp_path / "LOCPOT"
+    _write_locpot(path, lat_ang, 1.0, [syms[int(z)] for z in types], list(counts), frac[order], (nx, ny, nz), v_ev)
+    c_flat = read_vasp_potential(str(path))
+    c_cube = dict(c_flat)
+    c_cube["esp_cube"] = np.asarray(c_flat["esp_cube"]).reshape(nz, ny, nx).transpose(2, 1, 0).ravel()
+    c_cube["esp_order"] = _abi.RP_ORDER_CUBE
+    print(f"locpot fit {precision}", flush=True)
+    q = []
+    for c in (c_flat, c_cube):
+        calc = ChargeCalculator(cube=c, precision=precision, context=gpu_ctx)
+        q.append(calc.fit_charges())
+        calc.close()
+    assert np.array_equal(q[0], q[1])
+    r = oracle.fit(c_flat["n_grid"], c_flat["axis_vector"], c_flat["atom_index"], c_flat["atom_pos"],
+                   np.asarray(c_flat["esp_cube"]))
+    np.testing.assert_allclose(q[0], r.charges, atol=1e-9 if precision == "fp64" else 1e-5)
