@@ -120,6 +120,10 @@ int rp_ctx_hbm_in_use(const rp_ctx *ctx, uint64_t *bytes);
  * 6 reduction of the Gram's row-split partials.  total_ms sums the launches of that kind
  * since the last reset. */
 int rp_ctx_profile(rp_ctx *ctx, int enable);
+/* The same for the kinds whose bit is set (bit k = kind k) only.  Every timed launch adds two event
+ * records to the stream, each of which leaves the GPU idle for a few microseconds, so a throughput
+ * measurement times only the kinds it reports. */
+int rp_ctx_profile_kinds(rp_ctx *ctx, uint32_t kinds);
 int rp_ctx_kernel_stats(const rp_ctx *ctx, int kind, double *total_ms, int64_t *launches);
 
 /* ---------------------------------------------------------------- fit path */

@@ -32,7 +32,7 @@ EXPORTED = (
     "rp_ctx_create", "rp_ctx_destroy", "rp_last_error", "rp_ctx_hbm_in_use",
     "rp_fit_create", "rp_fit_destroy", "rp_fit_info_get", "rp_fit_grid", "rp_fit_ewald",
     "rp_fit_accumulate", "rp_fit_normal_equations", "rp_fit_solve", "rp_fit_phi_matvec",
-    "rp_fit_phi", "rp_fit_timings", "rp_ctx_profile", "rp_ctx_kernel_stats", "rp_fit_refit",
+    "rp_fit_phi", "rp_fit_timings", "rp_ctx_profile", "rp_ctx_profile_kinds", "rp_ctx_kernel_stats", "rp_fit_refit",
     "rp_fit_reset", "rp_server_create", "rp_server_destroy", "rp_server_error", "rp_server_now",
     "rp_request_bytes", "rp_submit", "rp_poll", "rp_server_stats_get",
     "rp_cube_read_header", "rp_cube_read", "rp_parse_values", "rp_fit_qeq",
@@ -131,6 +131,7 @@ def load(path: str | None = None):
     L.rp_fit_phi.argtypes = [vp, ctypes.c_int64, dp, dp]
     L.rp_fit_timings.argtypes = [vp, dp]
     L.rp_ctx_profile.argtypes = [vp, ctypes.c_int]
+    L.rp_ctx_profile_kinds.argtypes = [vp, ctypes.c_uint32]
     L.rp_ctx_kernel_stats.argtypes = [vp, ctypes.c_int, dp, i64p]
     L.rp_fit_refit.argtypes = [vp, ctypes.POINTER(RpConstraints), dp]
     L.rp_fit_reset.argtypes = [vp, ctypes.POINTER(RpSystem), ctypes.POINTER(RpOptions)]
@@ -184,8 +185,16 @@ class Context:
         check(load().rp_ctx_hbm_in_use(self.handle, ctypes.byref(v)), self.handle)
         return int(v.value)
 
-    def profile(self, enable=True):
-        check(load().rp_ctx_profile(self.handle, int(enable)), self.handle)
+    def profile(self, enable=True, kinds=None):
+        """Kernel timing with events on the stream; ``kinds`` (names of KERNEL_KINDS) limits it to
+        those kinds, which keeps the event records off the other launches."""
+        if kinds is None:
+            check(load().rp_ctx_profile(self.handle, int(enable)), self.handle)
+        else:
+            mask = 0
+            for k in kinds:
+                mask |= 1 << KERNEL_KINDS[k]
+            check(load().rp_ctx_profile_kinds(self.handle, mask if enable else 0), self.handle)
 
     def kernel_stats(self):
         out = {}

@@ -96,6 +96,7 @@ int prepare_device(rp_fit *fit, FitDevice &dv, int precision) {
     dv.NP = (int)round_up(N + 2, 128);
     dv.S = (int)(fit->shifts_pruned.size() / 3);
     const bool rebuild = !dv.ready || dv.shifts_host != fit->shifts_pruned;
+    if (!dv.ready) dv.sep_kbox_ok = false;
     dv.geom.nx = s.nx;
     dv.geom.ny = s.ny;
     dv.geom.nz = s.nz;
@@ -203,22 +204,20 @@ int run_accumulate(rp_fit *fit) {
     FitDevice &dv = *dvp;
     const int64_t g2 = (int64_t)dv.NP * dv.NP;
     RP_CHECK(fit->d_gram.grow(ctx, sizeof(double) * g2));
-    hipEvent_t e0, e1;
-    RP_HIP(ctx, hipEventCreate(&e0));
-    RP_HIP(ctx, hipEventCreate(&e1));
-    RP_HIP(ctx, hipEventRecord(e0, st));
+    if (!fit->ev_acc0) RP_HIP(ctx, hipEventCreate(&fit->ev_acc0));
+    if (!fit->ev_acc1) RP_HIP(ctx, hipEventCreate(&fit->ev_acc1));
+    fit->accumulated = false;
+    fit->acc_timing_pending = false;
+    RP_HIP(ctx, hipEventRecord(fit->ev_acc0, st));
     k_zero<<<ceil_div(g2, 256), 256, 0, st>>>(fit->d_gram.as<double>(), g2);
     int rc = fit->opt.precision == RP_PREC_FP64 ? accumulate_f64(fit, dv) : accumulate_split(fit, dv);
     if (rc != RP_OK) return rc;
     k_symmetrize<<<ceil_div(g2, 256), 256, 0, st>>>(fit->d_gram.as<double>(), dv.NP, 128);
     RP_HIP(ctx, hipGetLastError());
-    RP_HIP(ctx, hipEventRecord(e1, st));
-    RP_HIP(ctx, hipStreamSynchronize(st));
-    float ms = 0.f;
-    RP_HIP(ctx, hipEventElapsedTime(&ms, e0, e1));
-    fit->t_accum_ms = ms;
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
+    RP_HIP(ctx, hipEventRecord(fit->ev_acc1, st));
+    // no synchronisation: the Gram stays on the stream for the solve (or the next read-back), and
+    // the phase time is read when that synchronises
+    fit->acc_timing_pending = true;
     fit->na = fit->n_atoms + 2;
     fit->accumulated = true;
     return RP_OK;
@@ -281,7 +280,10 @@ using namespace rp;
 
 rp_fit::~rp_fit() {
     delete dev;
+    if (ev_acc0) (void)hipEventDestroy(ev_acc0);
+    if (ev_acc1) (void)hipEventDestroy(ev_acc1);
     if (h_stage) (void)hipHostFree(h_stage);
+    if (h_small) (void)hipHostFree(h_small);
 }
 
 static void destroy_ctx(rp_ctx *ctx) {
@@ -345,9 +347,11 @@ int rp_ctx_hbm_in_use(const rp_ctx *ctx, uint64_t *bytes) {
     return RP_OK;
 }
 
-int rp_ctx_profile(rp_ctx *ctx, int enable) {
+int rp_ctx_profile(rp_ctx *ctx, int enable) { return rp_ctx_profile_kinds(ctx, enable ? (1u << KT_N) - 1u : 0u); }
+
+int rp_ctx_profile_kinds(rp_ctx *ctx, uint32_t kinds) {
     if (!ctx) return RP_EINVAL;
-    ctx->profile = enable != 0;
+    ctx->profile = kinds & ((1u << KT_N) - 1u);
     for (int k = 0; k < KT_N; ++k) {
         ctx->kt_ms[k] = 0.0;
         ctx->kt_n[k] = 0;
@@ -507,6 +511,7 @@ int rp_fit_phi(rp_fit *fit, int64_t m, const double *grid_xyz, double *phi_out) 
 
 int rp_fit_timings(const rp_fit *fit, double *t3) {
     if (!fit || !t3) return RP_EINVAL;
+    const_cast<rp_fit *>(fit)->resolve_timings(true);
     t3[0] = fit->t_filter_ms;
     t3[1] = fit->t_accum_ms;
     t3[2] = fit->t_solve_ms;

@@ -200,13 +200,16 @@ __global__ __launch_bounds__(1024) void k_mean(const double *__restrict__ block_
 }
 
 // V -= mean; also the AABB of the kept grid points (for shift pruning) via per-block partials.
+// The grid covers the largest possible count (one block per filter block); m is read on the device.
 __global__ __launch_bounds__(kBlock) void k_center_and_bounds(int nx, int ny, const double *__restrict__ ax_,
                                                                const int *__restrict__ idx, double *__restrict__ v,
-                                                               int64_t m, const double *__restrict__ mean,
+                                                               const int64_t *__restrict__ m_dev,
+                                                               const double *__restrict__ mean,
                                                                double *__restrict__ bounds_part) {
     __shared__ double ax[9];
     if (threadIdx.x < 9) ax[threadIdx.x] = ax_[threadIdx.x];
     __syncthreads();
+    const int64_t m = *m_dev;
     const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
     if (t < m) {
@@ -244,6 +247,39 @@ __global__ __launch_bounds__(kBlock) void k_center_and_bounds(int nx, int ny, co
             bounds_part[6 * blockIdx.x + c] = a;
             bounds_part[6 * blockIdx.x + 3 + c] = b;
         }
+    }
+}
+
+// The filter's results for the host, written into mapped pinned memory so one synchronisation
+// returns them: out[0] the kept count, out[1] mean(V), out[2..4] / out[5..7] the AABB of the kept points.
+__global__ __launch_bounds__(1024) void k_filter_publish(const double *__restrict__ bounds_part, int64_t n_blocks,
+                                                         const int64_t *__restrict__ total,
+                                                         const double *__restrict__ mean, double *__restrict__ out) {
+    __shared__ double red[16][6];
+    double r[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
+    for (int64_t b = threadIdx.x; b < n_blocks; b += 1024)
+        for (int c = 0; c < 3; ++c) {
+            r[c] = fmin(r[c], bounds_part[6 * b + c]);
+            r[3 + c] = fmax(r[3 + c], bounds_part[6 * b + 3 + c]);
+        }
+    for (int c = 0; c < 3; ++c)
+        for (int off = 32; off > 0; off >>= 1) {
+            r[c] = fmin(r[c], __shfl_xor(r[c], off, 64));
+            r[3 + c] = fmax(r[3 + c], __shfl_xor(r[3 + c], off, 64));
+        }
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0)
+        for (int c = 0; c < 6; ++c) red[w][c] = r[c];
+    __syncthreads();
+    if (threadIdx.x < 6) {
+        const int c = threadIdx.x;
+        double a = red[0][c];
+        for (int g = 1; g < 16; ++g) a = c < 3 ? fmin(a, red[g][c]) : fmax(a, red[g][c]);
+        out[2 + c] = a;
+    }
+    if (threadIdx.x == 0) {
+        out[0] = (double)total[0];
+        out[1] = mean[0];
     }
 }
 
@@ -369,6 +405,9 @@ int upload_filter(rp_fit *fit, const rp_system *sys) {
     RP_CHECK(fs.mean.grow(ctx, sizeof(double)));
     RP_CHECK(fs.axis.grow(ctx, sizeof(double) * 9));
     RP_CHECK(fit->d_mask.grow(ctx, (size_t)n_total));
+    RP_CHECK(fit->d_idx.grow(ctx, sizeof(int) * n_total));
+    RP_CHECK(fit->d_v.grow(ctx, sizeof(double) * n_total));
+    RP_CHECK(fs.bounds.grow(ctx, sizeof(double) * 6 * nb));
     const size_t esp_bytes = (size_t)n_total * (sys->esp_dtype == RP_F32 ? 4 : 8);
     RP_CHECK(fs.esp.grow(ctx, esp_bytes));
     hipStream_t st = ctx->stream;
@@ -400,18 +439,9 @@ int run_filter(rp_fit *fit) {
                                                    fit->d_mask.as<uint8_t>(), fs.cnt.as<int>());
     }
     k_scan_counts<<<1, 1024, 0, st>>>(fs.cnt.as<int>(), n_blocks, fs.off.as<int64_t>());
-    int64_t n_keep = 0;
-    RP_HIP(ctx, hipMemcpyAsync(&n_keep, fs.off.as<int64_t>() + n_blocks, sizeof(int64_t), hipMemcpyDeviceToHost, st));
-    RP_HIP(ctx, hipStreamSynchronize(st));
-    if (n_keep == 0) {
-        fit->n_keep = 0;
-        (void)hipEventDestroy(e0);
-        (void)hipEventDestroy(e1);
-        return ctx->fail(RP_EEMPTY, "the vdW filter kept no grid point");
-    }
-    fit->n_keep = n_keep;
-    RP_CHECK(fit->d_idx.grow(ctx, sizeof(int) * n_keep));
-    RP_CHECK(fit->d_v.grow(ctx, sizeof(double) * n_keep));
+    // compaction, mean and bounds run before the host knows the count: the kept-point buffers hold
+    // n_total entries and the grids cover every filter block
+    const int64_t *d_count = fs.off.as<int64_t>() + n_blocks;
     if (fs.esp_dtype == RP_F32)
         k_compact<float><<<n_blocks, kBlock, 0, st>>>(s.nx, s.ny, s.nz, fs.esp_order, fit->d_mask.as<uint8_t>(),
                                                       fs.off.as<int64_t>(), fs.esp.as<float>(), fit->d_idx.as<int>(),
@@ -420,16 +450,15 @@ int run_filter(rp_fit *fit) {
         k_compact<double><<<n_blocks, kBlock, 0, st>>>(s.nx, s.ny, s.nz, fs.esp_order, fit->d_mask.as<uint8_t>(),
                                                        fs.off.as<int64_t>(), fs.esp.as<double>(), fit->d_idx.as<int>(),
                                                        fit->d_v.as<double>(), fs.vsum.as<double>());
-    k_mean<<<1, 1024, 0, st>>>(fs.vsum.as<double>(), n_blocks, fs.off.as<int64_t>() + n_blocks, fs.mean.as<double>());
-    const int64_t cb = ceil_div(n_keep, kBlock);
-    RP_CHECK(fs.bounds.grow(ctx, sizeof(double) * 6 * cb));
-    k_center_and_bounds<<<cb, kBlock, 0, st>>>(s.nx, s.ny, fs.axis.as<double>(), fit->d_idx.as<int>(), fit->d_v.as<double>(),
-                                                n_keep, fs.mean.as<double>(), fs.bounds.as<double>());
+    k_mean<<<1, 1024, 0, st>>>(fs.vsum.as<double>(), n_blocks, d_count, fs.mean.as<double>());
+    k_center_and_bounds<<<n_blocks, kBlock, 0, st>>>(s.nx, s.ny, fs.axis.as<double>(), fit->d_idx.as<int>(),
+                                                      fit->d_v.as<double>(), d_count, fs.mean.as<double>(),
+                                                      fs.bounds.as<double>());
+    RP_CHECK(fit->host_slots(ctx));
+    k_filter_publish<<<1, 1024, 0, st>>>(fs.bounds.as<double>(), n_blocks, d_count, fs.mean.as<double>(),
+                                         fit->h_small_dev);
     RP_HIP(ctx, hipGetLastError());
     RP_HIP(ctx, hipEventRecord(e1, st));
-    std::vector<double> bounds(6 * cb);
-    RP_HIP(ctx, hipMemcpyAsync(bounds.data(), fs.bounds.p, sizeof(double) * 6 * cb, hipMemcpyDeviceToHost, st));
-    RP_HIP(ctx, hipMemcpyAsync(&fit->v_mean, fs.mean.p, sizeof(double), hipMemcpyDeviceToHost, st));
     RP_HIP(ctx, hipStreamSynchronize(st));
     ctx->collect();
     float ms = 0.f;
@@ -437,15 +466,12 @@ int run_filter(rp_fit *fit) {
     fit->t_filter_ms = ms;
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
-    double gmin[3], gmax[3];
-    for (int c = 0; c < 3; ++c) {
-        gmin[c] = bounds[c];
-        gmax[c] = bounds[3 + c];
-        for (int64_t b = 1; b < cb; ++b) {
-            gmin[c] = std::min(gmin[c], bounds[6 * b + c]);
-            gmax[c] = std::max(gmax[c], bounds[6 * b + 3 + c]);
-        }
-    }
+    const double *res = fit->h_small;
+    const int64_t n_keep = (int64_t)res[0];
+    fit->n_keep = n_keep;
+    if (n_keep == 0) return ctx->fail(RP_EEMPTY, "the vdW filter kept no grid point");
+    fit->v_mean = res[1];
+    const double gmin[3] = {res[2], res[3], res[4]}, gmax[3] = {res[5], res[6], res[7]};
     if (s.fit_type == 1)
         fit->shifts_pruned = prune_shifts(s.shifts, gmin, gmax, fit->atom_xyz, s.r_cutoff);
     else

@@ -114,6 +114,7 @@ struct FitDevice {
     // split form: factorized reciprocal sum (k-index box, per-plane partial sums) and the
     // table of grid lines holding kept points
     DBuf sep_kbox, sep_t1, line_row, line_key, line_cnt, line_off, chunk_lines;
+    bool sep_kbox_ok = false;   // sep_kbox holds the index box of the current k-set
 };
 
 int prepare_device(rp_fit *fit, FitDevice &dv, int precision);

@@ -45,7 +45,7 @@ struct Ctx {
     uint64_t in_use = 0;
     std::string err;
     hipDeviceProp_t prop{};
-    bool profile = false;
+    uint32_t profile = 0;  // bit k: time launches of kind k
     int n_fits = 0;        // live fits on this context
     bool closing = false;  // rp_ctx_destroy was called while fits were alive
     double kt_ms[KT_N] = {};
@@ -61,7 +61,12 @@ struct Ctx {
     }
     // Adds the elapsed times of recorded launches (call after the stream is synchronised).
     void collect() {
+        size_t keep = 0;
         for (auto &p : pending) {
+            if (hipEventQuery(p.b) == hipErrorNotReady) {
+                pending[keep++] = p;
+                continue;
+            }
             float ms = 0.f;
             if (hipEventElapsedTime(&ms, p.a, p.b) == hipSuccess) {
                 kt_ms[p.kind] += ms;
@@ -70,7 +75,7 @@ struct Ctx {
             (void)hipEventDestroy(p.a);
             (void)hipEventDestroy(p.b);
         }
-        pending.clear();
+        pending.resize(keep);
     }
 };
 
@@ -80,7 +85,7 @@ struct KTimer {
     int kind;
     hipEvent_t a = nullptr, b = nullptr;
     KTimer(Ctx *ctx, int k) : c(ctx), kind(k) {
-        if (c->profile && hipEventCreate(&a) == hipSuccess && hipEventCreate(&b) == hipSuccess)
+        if ((c->profile >> kind & 1u) && hipEventCreate(&a) == hipSuccess && hipEventCreate(&b) == hipSuccess)
             (void)hipEventRecord(a, c->stream);
         else
             a = b = nullptr;
@@ -227,6 +232,18 @@ struct rp_fit {
     bool lu_fallback = false;     // last solve used LU (A not positive definite)
     int64_t kspace_products = 0;  // int8 MFMA products of a k-space GEMM (0: the split form factorizes it)
     double t_filter_ms = 0, t_accum_ms = 0, t_solve_ms = 0;
+    // the accumulate phase is timed by two events that are read at the next synchronisation
+    hipEvent_t ev_acc0 = nullptr, ev_acc1 = nullptr;
+    bool acc_timing_pending = false;
+    void resolve_timings(bool wait) {
+        if (!acc_timing_pending) return;
+        if (wait) (void)hipEventSynchronize(ev_acc1);
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, ev_acc0, ev_acc1) == hipSuccess) {
+            t_accum_ms = ms;
+            acc_timing_pending = false;
+        }
+    }
 
     // device state
     rp::FilterState filt;
@@ -238,7 +255,25 @@ struct rp_fit {
     int32_t na = 0;       // augmented width N + 2
     // grow-only workspaces reused by every accumulate / solve / matvec of this fit
     rp::DBuf ws[32];
-    void *h_stage = nullptr;        // pinned host staging of the solve's constraint arrays (grow-only)
+    void *h_stage = nullptr;        // mapped pinned block: the solve's constraint arrays, then its results
+    void *h_stage_dev = nullptr;    // its device address
+    // mapped pinned slots that kernels write results into for the host (filter count, mean and
+    // bounds; chunk line ranges): a synchronisation returns them without a copy
+    double *h_small = nullptr, *h_small_dev = nullptr;
+    size_t h_small_bytes = 0;
+    int host_slots(rp::Ctx *ctx, size_t bytes = 4096) {
+        if (h_small_bytes >= bytes) return 0;
+        if (h_small) (void)hipHostFree(h_small);
+        h_small = h_small_dev = nullptr;
+        h_small_bytes = 0;
+        void *p = nullptr, *d = nullptr;
+        if (hipHostMalloc(&p, bytes, hipHostMallocMapped) != hipSuccess) return ctx->fail(RP_ENOMEM, "hipHostMalloc failed");
+        h_small = static_cast<double *>(p);
+        h_small_bytes = bytes;
+        if (hipHostGetDevicePointer(&d, p, 0) != hipSuccess) return ctx->fail(RP_EHIP, "hipHostGetDevicePointer failed");
+        h_small_dev = static_cast<double *>(d);
+        return 0;
+    }
     size_t h_stage_bytes = 0;
     int split_tiles_np = 0;         // NP the split Gram's tile list in ws[9] was built for
     rp::FitDevice *dev = nullptr;   // persistent device copy of the Ewald setup

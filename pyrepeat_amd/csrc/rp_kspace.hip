@@ -432,6 +432,11 @@ __global__ __launch_bounds__(256) void k_slice_cols(const double *__restrict__ p
 
 }  // namespace
 
+__global__ __launch_bounds__(256) void k_ints_to_host(const int *__restrict__ src, int n, int *__restrict__ dst) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i < n) dst[i] = src[i];
+}
+
 int sep_prepare(rp_fit *fit, FitDevice &dv, int64_t rows, SepPlan &plan) {
     Ctx *ctx = fit->ctx;
     hipStream_t st = ctx->stream;
@@ -439,15 +444,19 @@ int sep_prepare(rp_fit *fit, FitDevice &dv, int64_t rows, SepPlan &plan) {
     const int N = fit->n_atoms, K = dv.K;
     const int64_t M = fit->n_keep;
     if (K > 0) {
-        std::vector<int> kbox(SH * SK * SK, -1);
-        for (int p = 0; p < K; ++p) {
-            const int h = s.khkl[3 * p], k = s.khkl[3 * p + 1] + SOFF, l = s.khkl[3 * p + 2] + SOFF;
-            if (h < 0 || h > HMAX || k < 0 || k >= SK || l < 0 || l >= SK)
-                return ctx->fail(RP_EINVAL, "k-vector index outside the factorized box");
-            kbox[(h * SK + k) * SK + l] = p;
+        if (!dv.sep_kbox_ok) {   // the k-set's index box, once per Ewald setup
+            std::vector<int> kbox(SH * SK * SK, -1);
+            for (int p = 0; p < K; ++p) {
+                const int h = s.khkl[3 * p], k = s.khkl[3 * p + 1] + SOFF, l = s.khkl[3 * p + 2] + SOFF;
+                if (h < 0 || h > HMAX || k < 0 || k >= SK || l < 0 || l >= SK)
+                    return ctx->fail(RP_EINVAL, "k-vector index outside the factorized box");
+                kbox[(h * SK + k) * SK + l] = p;
+            }
+            RP_CHECK(dv.sep_kbox.grow(ctx, sizeof(int) * kbox.size()));
+            RP_HIP(ctx, hipMemcpyAsync(dv.sep_kbox.p, kbox.data(), sizeof(int) * kbox.size(), hipMemcpyHostToDevice, st));
+            RP_HIP(ctx, hipStreamSynchronize(st));   // the host vector goes out of scope
+            dv.sep_kbox_ok = true;
         }
-        RP_CHECK(dv.sep_kbox.grow(ctx, sizeof(int) * kbox.size()));
-        RP_HIP(ctx, hipMemcpyAsync(dv.sep_kbox.p, kbox.data(), sizeof(int) * kbox.size(), hipMemcpyHostToDevice, st));
         RP_CHECK(dv.sep_t1.grow(ctx, sizeof(double2) * (size_t)s.nz * N * SBOX));
         dim3 g(N, (unsigned)ceil_div(s.nz, T1_PLANES));
         k_sep_t1<<<g, 128, 0, st>>>(N, s.nz, dv.sep_kbox.as<int>(), dv.kvecs.as<double>(), dv.kcoef.as<double>(),
@@ -468,9 +477,13 @@ int sep_prepare(rp_fit *fit, FitDevice &dv, int64_t rows, SepPlan &plan) {
     k_chunk_lines<<<ceil_div(n_chunks, 256), 256, 0, st>>>(dv.line_row.as<int>(), dv.line_off.as<int64_t>() + nb, M, rows,
                                                            n_chunks, dv.chunk_lines.as<int>());
     RP_HIP(ctx, hipGetLastError());
-    std::vector<int> cl(2 * n_chunks);
-    RP_HIP(ctx, hipMemcpyAsync(cl.data(), dv.chunk_lines.p, sizeof(int) * cl.size(), hipMemcpyDeviceToHost, st));
+    // the line ranges into the mapped host slots (after the filter's 8 doubles)
+    RP_CHECK(fit->host_slots(ctx, std::max<size_t>(4096, 64 + sizeof(int) * 2 * (size_t)n_chunks)));
+    k_ints_to_host<<<ceil_div(2 * n_chunks, 256), 256, 0, st>>>(dv.chunk_lines.as<int>(), 2 * n_chunks,
+                                                               reinterpret_cast<int *>(fit->h_small_dev + 8));
+    RP_HIP(ctx, hipGetLastError());
     RP_HIP(ctx, hipStreamSynchronize(st));
+    const int *cl = reinterpret_cast<const int *>(fit->h_small + 8);
     plan.first_line.resize(n_chunks);
     plan.end_line.resize(n_chunks);
     for (int c = 0; c < n_chunks; ++c) {

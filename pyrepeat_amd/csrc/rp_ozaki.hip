@@ -297,8 +297,6 @@ int accumulate_split(rp_fit *fit, FitDevice &dv) {
                                               fit->d_gram.as<double>(), NP);
     }
     RP_HIP(ctx, hipGetLastError());
-    RP_HIP(ctx, hipStreamSynchronize(st));
-    ctx->collect();
     return RP_OK;
 }
 

@@ -138,12 +138,13 @@ uint64_t fit_bytes_bound(const Setup &s, int n_atoms, int esp_dtype, int precisi
     const uint64_t nsrc = std::max(n_src, 1);
     at(12, 8 * n * ld);
     // packed constraint arrays: ints (red_ptr, red_src, expand_src), then doubles
-    at(13, (uint64_t)round_up((int64_t)(4 * ((nr + 1) + nsrc + N)), 8) + 8 * (nsrc + 3 * nr));
+    at(13, (uint64_t)round_up((int64_t)(round_up((int64_t)(4 * ((nr + 1) + nsrc + N)), 8) + 8 * (nsrc + 3 * nr)), 16));
     at(19, 4 * n);
     at(20, 8 * n * ld);
     for (int i = 21; i <= 23; ++i) at(i, 8 * nr);
     at(24, 4);
     at(25, 8 * n);
+    at(26, 8 * nr);
     at(27, 8 * N);
     at(28, 4);
     at(29, 8 * nr);

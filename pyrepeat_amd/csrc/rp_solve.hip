@@ -236,6 +236,17 @@ __device__ __forceinline__ double readlane_f64(double v, int lane) {
 
 constexpr int PB = 8;   // pivots per step of the diagonal block
 
+// 1/sqrt(d) for d > 0: the hardware estimate (v_rsq_f64, about 2^-23 relative) and two Newton
+// steps y <- y (3 - d y^2) / 2, each of which squares the relative error, so the result is within a
+// few ulp.  It replaces a correctly rounded sqrt and a division on the serial pivot chain.
+__device__ __forceinline__ double rsqrt_newton(double d) {
+    double y = __builtin_amdgcn_rsq(d);
+    const double h = 0.5 * d;
+    y = y * fma(-h * y, y, 1.5);
+    y = y * fma(-h * y, y, 1.5);
+    return y;
+}
+
 // Diagonal block of the blocked Cholesky (64 x 64, 256 threads): factor L, its inverse X = L^-1
 // (written to Dinv, row-major, identity past the block) and the forward-substitution step of the two
 // right-hand sides, z_k = X r_k (rows k0.. of rw, rb, in place).
@@ -313,37 +324,39 @@ __device__ __forceinline__ void potrf_block(double *__restrict__ A, int ld, int 
         __syncthreads();
         // ---- phase A: wave 0
         if (wave == 0) {
-            double l[PB], Ljj[PB][PB], inv[PB];
+            // right-looking over the 8 pivots: only one multiply-add per pivot lies on the chain to
+            // the next pivot; the subtraction order per entry is the same as a left-looking sum
+            double c[PB], l[PB], Ljj[PB][PB], inv[PB];
+#pragma unroll
+            for (int q = 0; q < PB; ++q) c[q] = colS[lane][q];
 #pragma unroll
             for (int q = 0; q < PB; ++q) {
-                double acc = colS[lane][q];
+                const double d = readlane_f64(c[q], j + q);
+                const bool live = j + q < nb;
+                const bool ok = d > 1e-13 * fabs(od[j + q]);
+                if (live && !ok && !bad) bad = k0 + j + q + 1;
+                const double dd = live && ok ? d : 1.0;   // dead or failed pivots stay 1
+                const double ip = rsqrt_newton(dd);
+                const double piv = dd * ip;
+                inv[q] = ip;
+                l[q] = lane > j + q ? c[q] * ip : (lane == j + q ? piv : 0.0);
 #pragma unroll
-                for (int p = 0; p < q; ++p) {
-                    Ljj[q][p] = readlane_f64(l[p], j + q);
-                    acc = fma(-l[p], Ljj[q][p], acc);
+                for (int p = q + 1; p < PB; ++p) {
+                    Ljj[p][q] = readlane_f64(l[q], j + p);
+                    c[p] = fma(-l[q], Ljj[p][q], c[p]);
                 }
-                const double d = readlane_f64(acc, j + q);
-                double piv = 1.0;
-                if (j + q < nb) {
-                    if (!(d > 1e-13 * fabs(od[j + q]))) {
-                        if (!bad) bad = k0 + j + q + 1;
-                    } else {
-                        piv = sqrt(d);
-                    }
-                }
-                inv[q] = 1.0 / piv;
-                l[q] = lane > j + q ? acc * inv[q] : (lane == j + q ? piv : 0.0);
             }
 #pragma unroll
             for (int q = 0; q < PB; ++q) lbT[q][lane] = l[q];
-            double xv[PB];
+            double x[PB];
+#pragma unroll
+            for (int q = 0; q < PB; ++q) x[q] = xr[q][lane];
 #pragma unroll
             for (int q = 0; q < PB; ++q) {
-                double acc = xr[q][lane];
+                const double xv = x[q] * inv[q];
+                xb[q][lane] = xv;
 #pragma unroll
-                for (int p = 0; p < q; ++p) acc = fma(-Ljj[q][p], xv[p], acc);
-                xv[q] = acc * inv[q];
-                xb[q][lane] = xv[q];
+                for (int p = q + 1; p < PB; ++p) x[p] = fma(-Ljj[p][q], xv, x[p]);
             }
         }
         __syncthreads();
@@ -443,6 +456,8 @@ __global__ __launch_bounds__(256) void k_potrf_inv(double *__restrict__ A, int l
 
 // Panel below the diagonal block: L21 = A21 L11^{-T} = A21 Dinv^T, 32 rows per workgroup, and the
 // forward-substitution update of those rows of the right-hand sides, r_i -= L21[i] . z_k.
+// L21 goes in place (the trailing update reads it there) and, transposed, into the upper triangle
+// (A[k0 + c][r] = L21[r][c]), where the backward substitution reads it.
 __global__ __launch_bounds__(256) void k_chol_trsm_inv(double *__restrict__ A, int ld, int n, int k0,
                                                        const double *__restrict__ Dinv, double *__restrict__ rw,
                                                        double *__restrict__ rb) {
@@ -489,6 +504,10 @@ __global__ __launch_bounds__(256) void k_chol_trsm_inv(double *__restrict__ A, i
         }
     }
     __syncthreads();
+    for (int e = t; e < 32 * CB; e += 256) {
+        const int c = e / 32, i = e % 32;
+        if (r0 + i < n && c < nb) A[(int64_t)(k0 + c) * ld + r0 + i] = As[i][c];
+    }
     if (t < 64) {
         const int i = t & 31, which = t >> 5;
         const int r = r0 + i;
@@ -567,73 +586,202 @@ __global__ __launch_bounds__(256) void k_chol_update_potrf(double *__restrict__ 
     }
 }
 
+// One step of the blocked Cholesky after diagonal block k0 was factored (its inverse X in Dinv),
+// with the panel fused into the trailing update.  Work-group (ti, tj), tj <= ti, owns tile (ti, tj)
+// of the trailing lower triangle:
+//   1. L_i = A21_i X^T and L_j = A21_j X^T, the panel rows of tiles ti and tj, from the lower-triangle
+//      panel A21, which every work-group of the step reads and none writes;
+//   2. A22_ij -= L_i L_j^T on the lower triangle;
+//   3. a diagonal work-group (ti == tj) stores its L rows transposed in the upper triangle,
+//      A[k0 + c][r] = L[r][c], and applies the forward substitution r_i -= L_i z_k to its rows of
+//      the right-hand sides (z_k: rows k0.. of rw, rb, left there by the block's factorisation);
+//   4. work-group (0, 0) then factors the next diagonal block.
+// Each panel row block is computed by every work-group of its tile row and column.  That costs
+// nothing while the grid is at most one work-group per CU, which is when the host picks this kernel;
+// it saves the separate panel launch on the serial chain of diagonal blocks.
+struct StepLds {
+    double X[CB][CB + 1];
+    double Li[64][CB + 1];
+    double Lj[64][CB + 1];
+    double z[2][CB];
+};
+constexpr size_t STEP_LDS = sizeof(StepLds) > sizeof(PotrfLds) ? sizeof(StepLds) : sizeof(PotrfLds);
+
+__global__ __launch_bounds__(256) void k_chol_step(double *__restrict__ A, int ld, int n, int k0,
+                                                   const double *__restrict__ Dinv, const double *__restrict__ orig,
+                                                   int *__restrict__ info, double *__restrict__ Dinv_next,
+                                                   double *__restrict__ rw, double *__restrict__ rb) {
+    extern __shared__ __align__(16) char step_smem[];
+    const int ti = blockIdx.y, tj = blockIdx.x;
+    if (tj > ti) return;
+    StepLds &sm = *reinterpret_cast<StepLds *>(step_smem);
+    const int nb = min(CB, n - k0);
+    const int r0 = k0 + nb;
+    const int ri = r0 + ti * 64, rj = r0 + tj * 64;
+    const bool diag = ti == tj;
+    const int t = threadIdx.x, ty = t / 16, tx = t % 16;
+    for (int e = t; e < CB * CB; e += 256) sm.X[e / CB][e % CB] = Dinv[e];
+    for (int e = t; e < 64 * CB; e += 256) {
+        const int r = e / CB, q = e % CB;
+        sm.Li[r][q] = (ri + r < n && q < nb) ? A[(int64_t)(ri + r) * ld + k0 + q] : 0.0;
+        if (!diag) sm.Lj[r][q] = (rj + r < n && q < nb) ? A[(int64_t)(rj + r) * ld + k0 + q] : 0.0;
+    }
+    if (diag && t < CB) {
+        sm.z[0][t] = t < nb ? rw[k0 + t] : 0.0;
+        sm.z[1][t] = t < nb ? rb[k0 + t] : 0.0;
+    }
+    __syncthreads();
+    // 1. panel rows: thread (ty, tx) forms rows 4 ty.. and columns tx + 16 b
+    double li[4][4] = {}, lj[4][4] = {};
+    for (int q = 0; q < CB; ++q) {
+        double xv[4];
+#pragma unroll
+        for (int b = 0; b < 4; ++b) xv[b] = sm.X[tx + 16 * b][q];
+#pragma unroll
+        for (int a = 0; a < 4; ++a) {
+            const double v = sm.Li[ty * 4 + a][q];
+#pragma unroll
+            for (int b = 0; b < 4; ++b) li[a][b] = fma(v, xv[b], li[a][b]);
+        }
+        if (!diag) {
+#pragma unroll
+            for (int a = 0; a < 4; ++a) {
+                const double v = sm.Lj[ty * 4 + a][q];
+#pragma unroll
+                for (int b = 0; b < 4; ++b) lj[a][b] = fma(v, xv[b], lj[a][b]);
+            }
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            const int c = tx + 16 * b;
+            sm.Li[ty * 4 + a][c] = c < nb ? li[a][b] : 0.0;
+            if (!diag) sm.Lj[ty * 4 + a][c] = c < nb ? lj[a][b] : 0.0;
+        }
+    __syncthreads();
+    // 2. trailing update of the tile
+    double (*Lc)[CB + 1] = diag ? sm.Li : sm.Lj;
+    double acc[4][4] = {};
+    for (int q = 0; q < nb; ++q) {
+        double u[4];
+#pragma unroll
+        for (int b = 0; b < 4; ++b) u[b] = Lc[tx + 16 * b][q];
+#pragma unroll
+        for (int a = 0; a < 4; ++a) {
+            const double v = sm.Li[ty * 4 + a][q];
+#pragma unroll
+            for (int b = 0; b < 4; ++b) acc[a][b] = fma(v, u[b], acc[a][b]);
+        }
+    }
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+        const int r = ri + ty * 4 + a;
+        if (r >= n) continue;
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            const int c = rj + tx + 16 * b;
+            if (c < n && c <= r) A[(int64_t)r * ld + c] -= acc[a][b];
+        }
+    }
+    if (!diag) return;
+    // 3. the tile row's panel, transposed, and the forward substitution of its right-hand-side rows
+    for (int e = t; e < 64 * CB; e += 256) {
+        const int c = e / 64, r = e % 64;
+        if (ri + r < n && c < nb) A[(int64_t)(k0 + c) * ld + ri + r] = sm.Li[r][c];
+    }
+    if (t < 128) {
+        const int i = t & 63, which = t >> 6;
+        if (ri + i < n) {
+            double s = 0.0;
+            for (int c = 0; c < nb; ++c) s = fma(sm.Li[i][c], sm.z[which][c], s);
+            double *v = which ? rb : rw;
+            v[ri + i] -= s;
+        }
+    }
+    // 4. the next diagonal block
+    if (ti == 0) {
+        __syncthreads();   // the tile and its right-hand-side rows are in memory; the LDS is reused
+        potrf_block(A, ld, n, k0 + CB, orig, info, Dinv_next, rw, rb, *reinterpret_cast<PotrfLds *>(step_smem));
+    }
+}
+
 // After the factorisation (which leaves z_w = L^-1 w, z_b = L^-1 b in zw, zb): the Lagrange
-// multiplier lambda = (w.A^-1 b - q_tot) / (w.A^-1 w) = (z_w.z_b - q_tot) / (z_w.z_w), then
-// q = A^-1 (b - lambda w) = L^-T (z_b - lambda z_w), blocked with the diagonal-block inverses:
-//     y_k = D_k^-T (u_k - L_{>k,k}^T y_{>k}).
-// One workgroup of 1024 threads; y is kept in x.
-__global__ __launch_bounds__(1024) void k_chol_finish(const double *__restrict__ A, int ld, int n,
-                                                      const double *__restrict__ Dinv, const double *__restrict__ zw,
-                                                      const double *__restrict__ zb, double q_tot,
-                                                      double *__restrict__ x, const int *__restrict__ cinfo) {
+// multiplier lambda = (w.A^-1 b - q_tot) / (w.A^-1 w) = (z_w.z_b - q_tot) / (z_w.z_w), and the
+// right-hand side u = z_b - lambda z_w of the backward substitution q = L^-T u.  x[n] = lambda.
+__global__ __launch_bounds__(1024) void k_chol_lambda(const double *__restrict__ zw, const double *__restrict__ zb, int n,
+                                                      double q_tot, double *__restrict__ u, double *__restrict__ x,
+                                                      const int *__restrict__ cinfo) {
     if (*cinfo != 0) return;   // a pivot failed: the host reruns the system through the LU
-    __shared__ double red[2][1024];
-    __shared__ double part[16][CB];
-    __shared__ double Dk[CB][CB + 1];
-    __shared__ double u[CB];
-    const int t = threadIdx.x, lane = t & 63, grp = t >> 6;
+    __shared__ double red[2][16];
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
     double pw = 0.0, pb = 0.0;
     for (int r = t; r < n; r += 1024) {
-        pw += zw[r] * zw[r];
-        pb += zw[r] * zb[r];
+        pw = fma(zw[r], zw[r], pw);
+        pb = fma(zw[r], zb[r], pb);
     }
-    red[0][t] = pw;
-    red[1][t] = pb;
-    __syncthreads();
-    for (int s = 512; s > 0; s >>= 1) {
-        if (t < s) {
-            red[0][t] += red[0][t + s];
-            red[1][t] += red[1][t + s];
-        }
-        __syncthreads();
-    }
-    const double lambda = (red[1][0] - q_tot) / red[0][0];
-    for (int r = t; r < n; r += 1024) x[r] = zb[r] - lambda * zw[r];
-    __syncthreads();
-    const int nblk = (n + CB - 1) / CB;
-    for (int kb = nblk - 1; kb >= 0; --kb) {
-        const int j0 = kb * CB, j1 = min(n, j0 + CB), nb = j1 - j0;
-        #pragma unroll
-        for (int e = t; e < CB * CB; e += 1024) Dk[e / CB][e % CB] = Dinv[(int64_t)kb * CB * CB + e];
-        // (L_{>k,k}^T y_{>k})_c: lane = column c, 16 groups of rows
-        double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
-        if (lane < nb) {
-            int r = j1 + grp;
-            for (; r + 48 < n; r += 64) {
-                s0 += A[(int64_t)r * ld + j0 + lane] * x[r];
-                s1 += A[(int64_t)(r + 16) * ld + j0 + lane] * x[r + 16];
-                s2 += A[(int64_t)(r + 32) * ld + j0 + lane] * x[r + 32];
-                s3 += A[(int64_t)(r + 48) * ld + j0 + lane] * x[r + 48];
-            }
-            for (; r < n; r += 16) s0 += A[(int64_t)r * ld + j0 + lane] * x[r];
-        }
-        part[grp][lane] = (s0 + s1) + (s2 + s3);
-        __syncthreads();
-        if (t < nb) {
-            double acc = 0.0;
 #pragma unroll
-            for (int g = 0; g < 16; ++g) acc += part[g][t];
-            u[t] = x[j0 + t] - acc;
-        }
-        __syncthreads();
-        if (t < nb) {
-            double y = 0.0;
-            for (int i = t; i < nb; ++i) y += Dk[i][t] * u[i];
-            x[j0 + t] = y;
-        }
-        __syncthreads();
+    for (int o = 32; o > 0; o >>= 1) {
+        pw += __shfl_xor(pw, o);
+        pb += __shfl_xor(pb, o);
     }
+    if (lane == 0) {
+        red[0][wave] = pw;
+        red[1][wave] = pb;
+    }
+    __syncthreads();
+    double sw = 0.0, sb = 0.0;
+#pragma unroll
+    for (int g = 0; g < 16; ++g) {
+        sw += red[0][g];
+        sb += red[1][g];
+    }
+    const double lambda = (sb - q_tot) / sw;
+    for (int r = t; r < n; r += 1024) u[r] = zb[r] - lambda * zw[r];
     if (t == 0) x[n] = lambda;
+}
+
+// Block kb of the backward substitution L^T y = u, right-looking: u_k already holds every
+// contribution of the blocks after k, so y_k = D_k^T u_k (D_k = L_kk^-1, the diagonal-block inverse),
+// and then u_j -= (L_kj)^T y_k for every block j < k.  (L_kj)^T is the upper-triangle block in the
+// rows of block j and the columns of block k, where the factorisation stored the panels transposed.
+// Work-group g updates u_g (g < kb); every work-group forms y_k, work-group 0 stores it in x.  Its
+// block of L is loaded first, so the loads overlap the small triangular product.
+__global__ __launch_bounds__(256) void k_backsub_block(const double *__restrict__ A, int ld, int n, int kb,
+                                                       const double *__restrict__ Dinv, double *__restrict__ u,
+                                                       double *__restrict__ x, const int *__restrict__ cinfo) {
+    if (*cinfo != 0) return;
+    __shared__ double Dk[CB][CB + 1];
+    __shared__ double uk[CB], yk[CB];
+    const int t = threadIdx.x, g = blockIdx.x;
+    const int j0 = kb * CB, nb = min(CB, n - j0);
+    const int r = t >> 2, q = (t & 3) * 16;   // row g*64 + r (always < j0 <= n), columns j0 + q ..
+    double ub[16];
+    if (kb > 0) {
+        const double *row = A + (int64_t)(g * CB + r) * ld + j0 + q;
+#pragma unroll
+        for (int i = 0; i < 16; ++i) ub[i] = q + i < nb ? row[i] : 0.0;
+    }
+    for (int e = t; e < CB * CB; e += 256) Dk[e / CB][e % CB] = Dinv[(int64_t)kb * CB * CB + e];
+    if (t < CB) uk[t] = t < nb ? u[j0 + t] : 0.0;
+    __syncthreads();
+    if (t < CB) {
+        double y = 0.0;
+        for (int i = t; i < nb; ++i) y = fma(Dk[i][t], uk[i], y);
+        yk[t] = y;
+    }
+    __syncthreads();
+    if (g == 0 && t < nb) x[j0 + t] = yk[t];
+    if (kb > 0) {
+        double s = 0.0;
+#pragma unroll
+        for (int i = 0; i < 16; ++i) s = fma(ub[i], yk[q + i], s);
+        s += __shfl_xor(s, 1);
+        s += __shfl_xor(s, 2);
+        if ((t & 3) == 0) u[g * CB + r] -= s;
+    }
 }
 
 // Back substitution U x = y (y = column n of the factored augmented matrix); x -> out.
@@ -654,6 +802,22 @@ __global__ __launch_bounds__(1024) void k_backsub(const double *__restrict__ A, 
 __global__ void k_expand(const double *__restrict__ x, const int *__restrict__ src, int N, double *__restrict__ q) {
     const int a = blockIdx.x * 256 + threadIdx.x;
     if (a < N) q[a] = src[a] >= 0 ? x[src[a]] : 0.0;
+}
+
+// Copy of the constraint block from mapped pinned host memory (a few KB; the kernel's loads cross
+// the bus once, which starts sooner than a copy-engine transfer).
+__global__ __launch_bounds__(256) void k_fetch_host(const uint4 *__restrict__ src, uint4 *__restrict__ dst, int64_t n16) {
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n16; i += (int64_t)gridDim.x * 256) dst[i] = src[i];
+}
+
+// The results into mapped pinned host memory: out[0] the pivot flag, out[1..N] the expanded charges,
+// out[1+N..1+N+n) the reduced solution and the multiplier; one synchronisation then returns them.
+__global__ __launch_bounds__(256) void k_publish(const double *__restrict__ x, const int *__restrict__ src, int N, int n,
+                                                 const int *__restrict__ cinfo, double *__restrict__ out) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i == 0) out[0] = (double)*cinfo;
+    if (i < N) out[1 + i] = src[i] >= 0 ? x[src[i]] : 0.0;
+    if (i < n) out[1 + N + i] = x[i];
 }
 
 }  // namespace
@@ -702,15 +866,19 @@ int run_solve(rp_fit *fit, const rp_constraints *con, double *q_out, double *sol
     const int ld = (int)round_up(ncols, 4);
     // the constraint arrays in one pinned block, one upload: ints (red_ptr, red_src, expand_src), then
     // doubles (red_mult, weights, diag_add, rhs_add)
+    // The same mapped pinned block then receives the results (k_publish).
     const size_t n_int = (size_t)(nr + 1) + nsrc + N;
     const size_t off_d = round_up((int64_t)(4 * n_int), 8);
-    const size_t stage = off_d + 8 * ((size_t)nsrc + 3 * (size_t)nr);
-    if (fit->h_stage_bytes < stage) {
+    const size_t stage = round_up((int64_t)(off_d + 8 * ((size_t)nsrc + 3 * (size_t)nr)), 16);
+    const size_t out_bytes = sizeof(double) * (1 + (size_t)N + n);
+    if (fit->h_stage_bytes < stage + out_bytes) {
         if (fit->h_stage) (void)hipHostFree(fit->h_stage);
         fit->h_stage = nullptr;
+        fit->h_stage_dev = nullptr;
         fit->h_stage_bytes = 0;
-        RP_HIP(ctx, hipHostMalloc(&fit->h_stage, stage, hipHostMallocDefault));
-        fit->h_stage_bytes = stage;
+        RP_HIP(ctx, hipHostMalloc(&fit->h_stage, stage + out_bytes, hipHostMallocMapped));
+        fit->h_stage_bytes = stage + out_bytes;
+        RP_HIP(ctx, hipHostGetDevicePointer(&fit->h_stage_dev, fit->h_stage, 0));
     }
     {
         int *hi = static_cast<int *>(fit->h_stage);
@@ -731,7 +899,10 @@ int run_solve(rp_fit *fit, const rp_constraints *con, double *q_out, double *sol
     RP_CHECK(d_info.grow(ctx, sizeof(int)));
     RP_CHECK(d_x.grow(ctx, sizeof(double) * n));
     RP_CHECK(d_q.grow(ctx, sizeof(double) * N));
-    RP_HIP(ctx, hipMemcpyAsync(d_con.p, fit->h_stage, stage, hipMemcpyHostToDevice, st));
+    k_fetch_host<<<(unsigned)std::min<int64_t>(ceil_div((int64_t)(stage / 16), 256), 64), 256, 0, st>>>(
+        static_cast<const uint4 *>(fit->h_stage_dev), d_con.as<uint4>(), (int64_t)(stage / 16));
+    double *h_out = reinterpret_cast<double *>(static_cast<char *>(fit->h_stage) + stage);
+    double *h_out_dev = reinterpret_cast<double *>(static_cast<char *>(fit->h_stage_dev) + stage);
     const int *dc_ptr = d_con.as<int>(), *dc_src = dc_ptr + nr + 1, *dc_exp = dc_src + nsrc;
     const double *dc_d = reinterpret_cast<const double *>(static_cast<const char *>(d_con.p) + off_d);
     const double *dc_mult = dc_d, *dc_w = dc_d + nsrc, *dc_diag = dc_w + nr, *dc_rhs = dc_diag + nr;
@@ -759,33 +930,50 @@ int run_solve(rp_fit *fit, const rp_constraints *con, double *q_out, double *sol
     // block 0 on its own; every later diagonal block is factored by the preceding trailing update
     k_potrf_inv<<<1, 256, 0, st>>>(d_K.as<double>(), ld, nr, 0, d_K2.as<double>(), d_cinfo.as<int>(), d_dinv.as<double>(),
                                    d_wc.as<double>(), d_bc.as<double>());
+    // steps whose trailing triangle has at most one 64 x 64 tile per CU fuse the panel into the update
+    const int n_cu = std::max(1, ctx->prop.multiProcessorCount);
+    RP_HIP(ctx, hipFuncSetAttribute(reinterpret_cast<const void *>(&k_chol_step),
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)STEP_LDS));
     for (int k0 = 0; k0 < nr; k0 += CB) {
         const int nb = std::min(CB, nr - k0);
         double *dinv = d_dinv.as<double>() + (int64_t)(k0 / CB) * CB * CB;
         const int below = nr - (k0 + nb);
         if (below > 0) {
-            k_chol_trsm_inv<<<ceil_div(below, 32), 256, 0, st>>>(d_K.as<double>(), ld, nr, k0, dinv, d_wc.as<double>(),
-                                                                 d_bc.as<double>());
             const int nt = (int)ceil_div(below, 64);
             dim3 g(nt, nt);
-            k_chol_update_potrf<<<g, 256, 0, st>>>(d_K.as<double>(), ld, nr, k0, d_K2.as<double>(), d_cinfo.as<int>(),
-                                                   dinv + CB * CB, d_wc.as<double>(), d_bc.as<double>());
+            if ((int64_t)nt * (nt + 1) / 2 <= n_cu) {
+                k_chol_step<<<g, 256, STEP_LDS, st>>>(d_K.as<double>(), ld, nr, k0, dinv, d_K2.as<double>(),
+                                                      d_cinfo.as<int>(), dinv + CB * CB, d_wc.as<double>(),
+                                                      d_bc.as<double>());
+            } else {
+                k_chol_trsm_inv<<<ceil_div(below, 32), 256, 0, st>>>(d_K.as<double>(), ld, nr, k0, dinv,
+                                                                     d_wc.as<double>(), d_bc.as<double>());
+                k_chol_update_potrf<<<g, 256, 0, st>>>(d_K.as<double>(), ld, nr, k0, d_K2.as<double>(),
+                                                       d_cinfo.as<int>(), dinv + CB * CB, d_wc.as<double>(),
+                                                       d_bc.as<double>());
+            }
         }
     }
-    // the multiplier and backward solve (skipped on the device when a pivot failed), expansion,
-    // results and the pivot flag in one round trip
-    k_chol_finish<<<1, 1024, 0, st>>>(d_K.as<double>(), ld, nr, d_dinv.as<double>(), d_wc.as<double>(), d_bc.as<double>(),
-                                      con->q_tot, d_x.as<double>(), d_cinfo.as<int>());
-    k_expand<<<ceil_div(N, 256), 256, 0, st>>>(d_x.as<double>(), dc_exp, N, d_q.as<double>());
+    // the multiplier and the backward solve, one launch per diagonal block (skipped on the device
+    // when a pivot failed); then expansion, results and the pivot flag in one round trip
+    DBuf &d_u = fit->ws[26];
+    RP_CHECK(d_u.grow(ctx, sizeof(double) * nr));
+    k_chol_lambda<<<1, 1024, 0, st>>>(d_wc.as<double>(), d_bc.as<double>(), nr, con->q_tot, d_u.as<double>(),
+                                      d_x.as<double>(), d_cinfo.as<int>());
+    for (int kb = (int)ceil_div(nr, CB) - 1; kb >= 0; --kb)
+        k_backsub_block<<<std::max(kb, 1), 256, 0, st>>>(d_K.as<double>(), ld, nr, kb, d_dinv.as<double>(),
+                                                          d_u.as<double>(), d_x.as<double>(), d_cinfo.as<int>());
+    k_publish<<<ceil_div(N + 1, 256), 256, 0, st>>>(d_x.as<double>(), dc_exp, N, n, d_cinfo.as<int>(), h_out_dev);
     delete kts;   // records the end event before the synchronisation below
     kts = nullptr;
     RP_HIP(ctx, hipGetLastError());
     RP_HIP(ctx, hipEventRecord(e1, st));
-    int cinfo = 0;
-    RP_HIP(ctx, hipMemcpyAsync(&cinfo, d_cinfo.p, sizeof(int), hipMemcpyDeviceToHost, st));
-    RP_HIP(ctx, hipMemcpyAsync(q_out, d_q.p, sizeof(double) * N, hipMemcpyDeviceToHost, st));
-    if (solution_out) RP_HIP(ctx, hipMemcpyAsync(solution_out, d_x.p, sizeof(double) * n, hipMemcpyDeviceToHost, st));
     RP_HIP(ctx, hipStreamSynchronize(st));
+    const int cinfo = (int)h_out[0];
+    if (cinfo == 0) {
+        std::copy(h_out + 1, h_out + 1 + N, q_out);
+        if (solution_out) std::copy(h_out + 1 + N, h_out + 1 + N + n, solution_out);
+    }
     fit->lu_fallback = cinfo != 0;
     if (cinfo != 0) {
         // indefinite or singular A: LU with partial pivoting on the bordered system
@@ -805,6 +993,7 @@ int run_solve(rp_fit *fit, const rp_constraints *con, double *q_out, double *sol
     }
     if (used_fallback) *used_fallback = 0;
     ctx->collect();
+    fit->resolve_timings(false);
     float ms = 0.f;
     RP_HIP(ctx, hipEventElapsedTime(&ms, e0, e1));
     fit->t_solve_ms = ms;
