@@ -115,6 +115,13 @@ struct FitDevice {
     // table of grid lines holding kept points
     DBuf sep_kbox, sep_t1, line_row, line_key, line_cnt, line_off, chunk_lines;
     bool sep_kbox_ok = false;   // sep_kbox holds the index box of the current k-set
+    hipEvent_t ev_plan = nullptr;   // marks the chunk line ranges on the stream
+    FitDevice() = default;
+    FitDevice(const FitDevice &) = delete;
+    FitDevice &operator=(const FitDevice &) = delete;
+    ~FitDevice() {
+        if (ev_plan) (void)hipEventDestroy(ev_plan);
+    }
 };
 
 int prepare_device(rp_fit *fit, FitDevice &dv, int precision);

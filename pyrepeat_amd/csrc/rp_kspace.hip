@@ -458,9 +458,6 @@ int sep_prepare(rp_fit *fit, FitDevice &dv, int64_t rows, SepPlan &plan) {
             dv.sep_kbox_ok = true;
         }
         RP_CHECK(dv.sep_t1.grow(ctx, sizeof(double2) * (size_t)s.nz * N * SBOX));
-        dim3 g(N, (unsigned)ceil_div(s.nz, T1_PLANES));
-        k_sep_t1<<<g, 128, 0, st>>>(N, s.nz, dv.sep_kbox.as<int>(), dv.kvecs.as<double>(), dv.kcoef.as<double>(),
-                                    fit->d_atoms.as<double>(), dv.ez.as<double2>(), dv.sep_t1.as<double2>());
     }
     // line table of the kept points
     const int64_t nb = ceil_div(M, 256);
@@ -482,7 +479,17 @@ int sep_prepare(rp_fit *fit, FitDevice &dv, int64_t rows, SepPlan &plan) {
     k_ints_to_host<<<ceil_div(2 * n_chunks, 256), 256, 0, st>>>(dv.chunk_lines.as<int>(), 2 * n_chunks,
                                                                reinterpret_cast<int *>(fit->h_small_dev + 8));
     RP_HIP(ctx, hipGetLastError());
-    RP_HIP(ctx, hipStreamSynchronize(st));
+    // the T1 tables go on the stream behind the line ranges, so the GPU computes them while the
+    // host waits for the ranges and plans the chunks
+    if (!dv.ev_plan) RP_HIP(ctx, hipEventCreateWithFlags(&dv.ev_plan, hipEventDisableTiming));
+    RP_HIP(ctx, hipEventRecord(dv.ev_plan, st));
+    if (K > 0) {
+        dim3 g(N, (unsigned)ceil_div(s.nz, T1_PLANES));
+        k_sep_t1<<<g, 128, 0, st>>>(N, s.nz, dv.sep_kbox.as<int>(), dv.kvecs.as<double>(), dv.kcoef.as<double>(),
+                                    fit->d_atoms.as<double>(), dv.ez.as<double2>(), dv.sep_t1.as<double2>());
+        RP_HIP(ctx, hipGetLastError());
+    }
+    RP_HIP(ctx, hipEventSynchronize(dv.ev_plan));
     const int *cl = reinterpret_cast<const int *>(fit->h_small + 8);
     plan.first_line.resize(n_chunks);
     plan.end_line.resize(n_chunks);
