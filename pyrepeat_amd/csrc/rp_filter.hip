@@ -200,7 +200,9 @@ __global__ __launch_bounds__(1024) void k_mean(const double *__restrict__ block_
 }
 
 // V -= mean; also the AABB of the kept grid points (for shift pruning) via per-block partials.
-// The grid covers the largest possible count (one block per filter block); m is read on the device.
+// A fixed grid strides over the kept points (the count m is read on the device), one bounds partial
+// per block.
+constexpr int kBoundBlocks = 1024;
 __global__ __launch_bounds__(kBlock) void k_center_and_bounds(int nx, int ny, const double *__restrict__ ax_,
                                                                const int *__restrict__ idx, double *__restrict__ v,
                                                                const int64_t *__restrict__ m_dev,
@@ -210,10 +212,10 @@ __global__ __launch_bounds__(kBlock) void k_center_and_bounds(int nx, int ny, co
     if (threadIdx.x < 9) ax[threadIdx.x] = ax_[threadIdx.x];
     __syncthreads();
     const int64_t m = *m_dev;
-    const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    const double mu = mean[0];
     double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
-    if (t < m) {
-        v[t] = v[t] - mean[0];
+    for (int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x; t < m; t += (int64_t)gridDim.x * kBlock) {
+        v[t] = v[t] - mu;
         const int f = idx[t];
         const int layer = nx * ny;
         const int k = f / layer;
@@ -222,7 +224,10 @@ __global__ __launch_bounds__(kBlock) void k_center_and_bounds(int nx, int ny, co
         const int i = rem - j * nx;
         double g[3];
         grid_xyz(ax, i, j, k, g[0], g[1], g[2]);
-        for (int c = 0; c < 3; ++c) lo[c] = hi[c] = g[c];
+        for (int c = 0; c < 3; ++c) {
+            lo[c] = fmin(lo[c], g[c]);
+            hi[c] = fmax(hi[c], g[c]);
+        }
     }
     for (int c = 0; c < 3; ++c) {
         for (int off = 32; off > 0; off >>= 1) {
@@ -451,11 +456,12 @@ int run_filter(rp_fit *fit) {
                                                        fs.off.as<int64_t>(), fs.esp.as<double>(), fit->d_idx.as<int>(),
                                                        fit->d_v.as<double>(), fs.vsum.as<double>());
     k_mean<<<1, 1024, 0, st>>>(fs.vsum.as<double>(), n_blocks, d_count, fs.mean.as<double>());
-    k_center_and_bounds<<<n_blocks, kBlock, 0, st>>>(s.nx, s.ny, fs.axis.as<double>(), fit->d_idx.as<int>(),
+    const int bound_blocks = (int)std::min<int64_t>(n_blocks, kBoundBlocks);
+    k_center_and_bounds<<<bound_blocks, kBlock, 0, st>>>(s.nx, s.ny, fs.axis.as<double>(), fit->d_idx.as<int>(),
                                                       fit->d_v.as<double>(), d_count, fs.mean.as<double>(),
                                                       fs.bounds.as<double>());
     RP_CHECK(fit->host_slots(ctx));
-    k_filter_publish<<<1, 1024, 0, st>>>(fs.bounds.as<double>(), n_blocks, d_count, fs.mean.as<double>(),
+    k_filter_publish<<<1, 1024, 0, st>>>(fs.bounds.as<double>(), bound_blocks, d_count, fs.mean.as<double>(),
                                          fit->h_small_dev);
     RP_HIP(ctx, hipGetLastError());
     RP_HIP(ctx, hipEventRecord(e1, st));

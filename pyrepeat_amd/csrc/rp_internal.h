@@ -276,6 +276,7 @@ struct rp_fit {
     }
     size_t h_stage_bytes = 0;
     int split_tiles_np = 0;         // NP the split Gram's tile list in ws[9] was built for
+    uint32_t bs_base = 0, bs_epoch = 0;   // backward substitution: tickets taken so far, solve epoch
     rp::FitDevice *dev = nullptr;   // persistent device copy of the Ewald setup
     ~rp_fit();
 };

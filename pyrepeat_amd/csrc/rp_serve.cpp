@@ -144,9 +144,9 @@ uint64_t fit_bytes_bound(const Setup &s, int n_atoms, int esp_dtype, int precisi
     for (int i = 21; i <= 23; ++i) at(i, 8 * nr);
     at(24, 4);
     at(25, 8 * n);
-    at(26, 8 * nr);
     at(27, 8 * N);
-    at(28, 4);
+    at(28, 8);
+    at(31, 4 * (1 + (uint64_t)ceil_div((int64_t)nr, 64)));
     at(29, 8 * nr);
     at(30, 8 * 64 * 64 * (uint64_t)ceil_div((int64_t)nr, 64));
     uint64_t b = filter_bytes_bound(s, esp_dtype) + sep;

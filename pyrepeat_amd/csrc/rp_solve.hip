@@ -44,7 +44,10 @@ __global__ __launch_bounds__(256) void k_assemble(const double *__restrict__ gra
                                                   int *__restrict__ cinfo) {
     const int n = nr + 1;
     const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (e == 0) *cinfo = 0;
+    if (e == 0) {
+        cinfo[0] = 0;
+        cinfo[1] = 0;
+    }
     if (e >= (int64_t)n * (n + 1)) return;
     const int I = (int)(e / (n + 1)), J = (int)(e % (n + 1));
     const double ng = gram[(int64_t)(N + 1) * NP + N + 1];
@@ -710,15 +713,47 @@ __global__ __launch_bounds__(256) void k_chol_step(double *__restrict__ A, int l
 
 // After the factorisation (which leaves z_w = L^-1 w, z_b = L^-1 b in zw, zb): the Lagrange
 // multiplier lambda = (w.A^-1 b - q_tot) / (w.A^-1 w) = (z_w.z_b - q_tot) / (z_w.z_w), and the
-// right-hand side u = z_b - lambda z_w of the backward substitution q = L^-T u.  x[n] = lambda.
-__global__ __launch_bounds__(1024) void k_chol_lambda(const double *__restrict__ zw, const double *__restrict__ zb, int n,
-                                                      double q_tot, double *__restrict__ u, double *__restrict__ x,
-                                                      const int *__restrict__ cinfo) {
-    if (*cinfo != 0) return;   // a pivot failed: the host reruns the system through the LU
-    __shared__ double red[2][16];
+// backward substitution L^T y = u with u = z_b - lambda z_w; q = y, x[n] = lambda.
+//
+// One launch, one work-group per diagonal block.  A work-group takes a ticket when it starts; ticket
+// i owns block kb = nblk - 1 - i, so every block it needs (the ones after kb) belongs to a work-group
+// that started before it, and each of those depends only on earlier starters again: the chain always
+// progresses, whatever the dispatch order.  Work-group kb
+//   - forms lambda itself (two dot products of length n) and u_kb;
+//   - for each later block k, oldest first, loads its part of the transposed panel (the upper-triangle
+//     block in the rows of block kb and the columns of block k, contiguous rows) before it waits for
+//     block k's flag, then subtracts U_{kb,k} y_k;
+//   - forms y_kb = D_kb^T u_kb (D_kb = L_kb,kb^-1) and publishes it: the values, a release fence at
+//     agent scope, then the flag set to this solve's epoch.
+// A wait that sees no progress for BACKSUB_SPIN polls records the failure in *status and the
+// work-group leaves, so the kernel always ends.
+constexpr unsigned BACKSUB_SPIN = 1u << 22;
+
+__global__ __launch_bounds__(256) void k_backsub(const double *__restrict__ A, int ld, int n,
+                                                 const double *__restrict__ Dinv, const double *__restrict__ zw,
+                                                 const double *__restrict__ zb, double q_tot, double *__restrict__ x,
+                                                 unsigned *__restrict__ sync, unsigned base, unsigned epoch,
+                                                 int *__restrict__ status, const int *__restrict__ cinfo) {
+    __shared__ double Dk[CB][CB + 1];
+    __shared__ double uk[CB], yk[CB];
+    __shared__ double red[2][4];
+    __shared__ unsigned sh_ticket;
+    __shared__ int sh_abort;
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    unsigned *counter = sync, *flags = sync + 1;
+    if (t == 0) {
+        sh_ticket = atomicAdd(counter, 1u) - base;
+        sh_abort = 0;
+    }
+    __syncthreads();
+    if (*cinfo != 0) return;   // a pivot failed: the host reruns the system through the LU
+    const int nblk = (n + CB - 1) / CB;
+    const int kb = nblk - 1 - (int)sh_ticket;
+    const int j0 = kb * CB, nb = min(CB, n - j0);
+    for (int e = t; e < CB * CB; e += 256) Dk[e / CB][e % CB] = Dinv[(int64_t)kb * CB * CB + e];
+    // lambda
     double pw = 0.0, pb = 0.0;
-    for (int r = t; r < n; r += 1024) {
+    for (int r = t; r < n; r += 256) {
         pw = fma(zw[r], zw[r], pw);
         pb = fma(zw[r], zb[r], pb);
     }
@@ -732,56 +767,57 @@ __global__ __launch_bounds__(1024) void k_chol_lambda(const double *__restrict__
         red[1][wave] = pb;
     }
     __syncthreads();
-    double sw = 0.0, sb = 0.0;
+    const double lambda = (((red[1][0] + red[1][1]) + (red[1][2] + red[1][3])) - q_tot) /
+                          ((red[0][0] + red[0][1]) + (red[0][2] + red[0][3]));
+    if (t < CB) uk[t] = t < nb ? zb[j0 + t] - lambda * zw[j0 + t] : 0.0;
+    if (kb == nblk - 1 && t == 0) x[n] = lambda;
+    // contributions of the later blocks: thread (row c = t / 4, columns q .. q + 15 of block k)
+    const int c = t >> 2, q = (t & 3) * 16;
+    const bool crow = c < nb;
+    for (int k = nblk - 1; k > kb; --k) {
+        const int k0 = k * CB, kn = min(CB, n - k0);
+        double ub[16];
+        const double *row = A + (int64_t)(j0 + c) * ld + k0 + q;
 #pragma unroll
-    for (int g = 0; g < 16; ++g) {
-        sw += red[0][g];
-        sb += red[1][g];
-    }
-    const double lambda = (sb - q_tot) / sw;
-    for (int r = t; r < n; r += 1024) u[r] = zb[r] - lambda * zw[r];
-    if (t == 0) x[n] = lambda;
-}
-
-// Block kb of the backward substitution L^T y = u, right-looking: u_k already holds every
-// contribution of the blocks after k, so y_k = D_k^T u_k (D_k = L_kk^-1, the diagonal-block inverse),
-// and then u_j -= (L_kj)^T y_k for every block j < k.  (L_kj)^T is the upper-triangle block in the
-// rows of block j and the columns of block k, where the factorisation stored the panels transposed.
-// Work-group g updates u_g (g < kb); every work-group forms y_k, work-group 0 stores it in x.  Its
-// block of L is loaded first, so the loads overlap the small triangular product.
-__global__ __launch_bounds__(256) void k_backsub_block(const double *__restrict__ A, int ld, int n, int kb,
-                                                       const double *__restrict__ Dinv, double *__restrict__ u,
-                                                       double *__restrict__ x, const int *__restrict__ cinfo) {
-    if (*cinfo != 0) return;
-    __shared__ double Dk[CB][CB + 1];
-    __shared__ double uk[CB], yk[CB];
-    const int t = threadIdx.x, g = blockIdx.x;
-    const int j0 = kb * CB, nb = min(CB, n - j0);
-    const int r = t >> 2, q = (t & 3) * 16;   // row g*64 + r (always < j0 <= n), columns j0 + q ..
-    double ub[16];
-    if (kb > 0) {
-        const double *row = A + (int64_t)(g * CB + r) * ld + j0 + q;
+        for (int i = 0; i < 16; ++i) ub[i] = crow && q + i < kn ? row[i] : 0.0;
+        if (t == 0) {
+            unsigned spins = 0;
+            while (__hip_atomic_load(flags + k, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) != epoch) {
+                if (++spins > BACKSUB_SPIN) {
+                    sh_abort = 1;
+                    atomicExch(status, 1);
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(2);
+            }
+        }
+        __syncthreads();
+        if (sh_abort) return;
+        if (t < CB) yk[t] = t < kn ? __hip_atomic_load(x + k0 + t, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0.0;
+        __syncthreads();
+        double sacc = 0.0;
 #pragma unroll
-        for (int i = 0; i < 16; ++i) ub[i] = q + i < nb ? row[i] : 0.0;
+        for (int i = 0; i < 16; ++i) sacc = fma(ub[i], yk[q + i], sacc);
+        sacc += __shfl_xor(sacc, 1);
+        sacc += __shfl_xor(sacc, 2);
+        if ((t & 3) == 0 && crow) uk[c] -= sacc;
+        __syncthreads();
     }
-    for (int e = t; e < CB * CB; e += 256) Dk[e / CB][e % CB] = Dinv[(int64_t)kb * CB * CB + e];
-    if (t < CB) uk[t] = t < nb ? u[j0 + t] : 0.0;
+    // y_kb and its flag
     __syncthreads();
-    if (t < CB) {
+    if (t < nb) {
         double y = 0.0;
         for (int i = t; i < nb; ++i) y = fma(Dk[i][t], uk[i], y);
-        yk[t] = y;
+        __hip_atomic_store(x + j0 + t, y, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
+    __threadfence();
     __syncthreads();
-    if (g == 0 && t < nb) x[j0 + t] = yk[t];
-    if (kb > 0) {
-        double s = 0.0;
-#pragma unroll
-        for (int i = 0; i < 16; ++i) s = fma(ub[i], yk[q + i], s);
-        s += __shfl_xor(s, 1);
-        s += __shfl_xor(s, 2);
-        if ((t & 3) == 0) u[g * CB + r] -= s;
-    }
+    if (t == 0) __hip_atomic_store(flags + kb, epoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__global__ void k_fill_u32(unsigned *p, int n, unsigned v) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i < n) p[i] = v;
 }
 
 // Back substitution U x = y (y = column n of the factored augmented matrix); x -> out.
@@ -810,12 +846,13 @@ __global__ __launch_bounds__(256) void k_fetch_host(const uint4 *__restrict__ sr
     for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n16; i += (int64_t)gridDim.x * 256) dst[i] = src[i];
 }
 
-// The results into mapped pinned host memory: out[0] the pivot flag, out[1..N] the expanded charges,
+// The results into mapped pinned host memory: out[0] the pivot flag (-1: a backward-substitution wait
+// gave up, which the host also answers with the LU), out[1..N] the expanded charges,
 // out[1+N..1+N+n) the reduced solution and the multiplier; one synchronisation then returns them.
 __global__ __launch_bounds__(256) void k_publish(const double *__restrict__ x, const int *__restrict__ src, int N, int n,
                                                  const int *__restrict__ cinfo, double *__restrict__ out) {
     const int i = blockIdx.x * 256 + threadIdx.x;
-    if (i == 0) out[0] = (double)*cinfo;
+    if (i == 0) out[0] = cinfo[0] != 0 ? (double)cinfo[0] : (cinfo[1] != 0 ? -1.0 : 0.0);
     if (i < N) out[1 + i] = src[i] >= 0 ? x[src[i]] : 0.0;
     if (i < n) out[1 + N + i] = x[i];
 }
@@ -916,7 +953,7 @@ int run_solve(rp_fit *fit, const rp_constraints *con, double *q_out, double *sol
     DBuf &d_K2 = fit->ws[20], &d_wc = fit->ws[23];
     DBuf &d_cinfo = fit->ws[28], &d_bc = fit->ws[29], &d_dinv = fit->ws[30];
     RP_CHECK(d_K2.grow(ctx, sizeof(double) * (size_t)n * ld));
-    RP_CHECK(d_cinfo.grow(ctx, sizeof(int)));
+    RP_CHECK(d_cinfo.grow(ctx, 2 * sizeof(int)));   // pivot flag, backward-substitution wait status
     RP_CHECK(d_wc.grow(ctx, sizeof(double) * nr));
     RP_CHECK(d_bc.grow(ctx, sizeof(double) * nr));
     RP_CHECK(d_dinv.grow(ctx, sizeof(double) * CB * CB * (size_t)ceil_div(nr, CB)));
@@ -956,13 +993,24 @@ int run_solve(rp_fit *fit, const rp_constraints *con, double *q_out, double *sol
     }
     // the multiplier and the backward solve, one launch per diagonal block (skipped on the device
     // when a pivot failed); then expansion, results and the pivot flag in one round trip
-    DBuf &d_u = fit->ws[26];
-    RP_CHECK(d_u.grow(ctx, sizeof(double) * nr));
-    k_chol_lambda<<<1, 1024, 0, st>>>(d_wc.as<double>(), d_bc.as<double>(), nr, con->q_tot, d_u.as<double>(),
-                                      d_x.as<double>(), d_cinfo.as<int>());
-    for (int kb = (int)ceil_div(nr, CB) - 1; kb >= 0; --kb)
-        k_backsub_block<<<std::max(kb, 1), 256, 0, st>>>(d_K.as<double>(), ld, nr, kb, d_dinv.as<double>(),
-                                                          d_u.as<double>(), d_x.as<double>(), d_cinfo.as<int>());
+    // ws[31]: the ticket counter and one flag per block, cleared when (re)allocated; flags compare
+    // with a per-solve epoch, so they are never cleared between solves
+    DBuf &d_sync = fit->ws[31];
+    const int nblk = (int)ceil_div(nr, CB);
+    const void *sync_before = d_sync.p;
+    const size_t sync_bytes_before = d_sync.bytes;
+    RP_CHECK(d_sync.grow(ctx, sizeof(unsigned) * (size_t)(1 + nblk)));
+    if (d_sync.p != sync_before || d_sync.bytes != sync_bytes_before) {
+        const int words = (int)(d_sync.bytes / sizeof(unsigned));
+        k_fill_u32<<<ceil_div(words, 256), 256, 0, st>>>(d_sync.as<unsigned>(), words, 0u);
+        fit->bs_base = 0;
+        fit->bs_epoch = 0;
+    }
+    ++fit->bs_epoch;
+    k_backsub<<<nblk, 256, 0, st>>>(d_K.as<double>(), ld, nr, d_dinv.as<double>(), d_wc.as<double>(),
+                                    d_bc.as<double>(), con->q_tot, d_x.as<double>(), d_sync.as<unsigned>(),
+                                    fit->bs_base, fit->bs_epoch, d_cinfo.as<int>() + 1, d_cinfo.as<int>());
+    fit->bs_base += (unsigned)nblk;
     k_publish<<<ceil_div(N + 1, 256), 256, 0, st>>>(d_x.as<double>(), dc_exp, N, n, d_cinfo.as<int>(), h_out_dev);
     delete kts;   // records the end event before the synchronisation below
     kts = nullptr;
