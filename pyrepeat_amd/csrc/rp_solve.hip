@@ -28,6 +28,7 @@
 #include <cmath>
 
 #include "rp_fit_common.h"
+#include "rp_mfma.h"
 
 namespace rp {
 
@@ -603,13 +604,16 @@ __global__ __launch_bounds__(256) void k_chol_update_potrf(double *__restrict__ 
 // nothing while the grid is at most one work-group per CU, which is when the host picks this kernel;
 // it saves the separate panel launch on the serial chain of diagonal blocks.
 struct StepLds {
-    double X[CB][CB + 1];
-    double Li[64][CB + 1];
-    double Lj[64][CB + 1];
+    double XT[CB][CB + 1];    // X^T: XT[q][c] = X[c][q]
+    double LiT[CB][CB + 1];   // A21 rows of tile ti, then their L rows, transposed: LiT[q][r]
+    double LjT[CB][CB + 1];
     double z[2][CB];
 };
 constexpr size_t STEP_LDS = sizeof(StepLds) > sizeof(PotrfLds) ? sizeof(StepLds) : sizeof(PotrfLds);
 
+// Both 64 x 64 x 64 products run on fp64 MFMA (16 x 16 x 4): wave w owns the 32 x 32 quadrant
+// (w / 2, w % 2) as 2 x 2 tiles; the operands sit k-major in LDS, so each k group of 16 lanes reads
+// 128 contiguous bytes.
 __global__ __launch_bounds__(256) void k_chol_step(double *__restrict__ A, int ld, int n, int k0,
                                                    const double *__restrict__ Dinv, const double *__restrict__ orig,
                                                    int *__restrict__ info, double *__restrict__ Dinv_next,
@@ -622,92 +626,145 @@ __global__ __launch_bounds__(256) void k_chol_step(double *__restrict__ A, int l
     const int r0 = k0 + nb;
     const int ri = r0 + ti * 64, rj = r0 + tj * 64;
     const bool diag = ti == tj;
-    const int t = threadIdx.x, ty = t / 16, tx = t % 16;
-    for (int e = t; e < CB * CB; e += 256) sm.X[e / CB][e % CB] = Dinv[e];
-    for (int e = t; e < 64 * CB; e += 256) {
-        const int r = e / CB, q = e % CB;
-        sm.Li[r][q] = (ri + r < n && q < nb) ? A[(int64_t)(ri + r) * ld + k0 + q] : 0.0;
-        if (!diag) sm.Lj[r][q] = (rj + r < n && q < nb) ? A[(int64_t)(rj + r) * ld + k0 + q] : 0.0;
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    const int wr = wave >> 1, wc = wave & 1, lr = lane & 15, lk = lane >> 4;
+    // every global read of the step is issued up front, into registers: the block inverse, the panel
+    // rows of both tiles and this lane's entries of the tile to update (no other work-group writes
+    // them in this step), then the LDS is filled k-major
+    double xv[16], av[16], bv[16], a22[2][2][4];
+#pragma unroll
+    for (int i = 0; i < 16; ++i) xv[i] = Dinv[t + 256 * i];
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+        const int e = t + 256 * i, r = e / CB, q = e % CB;
+        av[i] = (ri + r < n && q < nb) ? A[(int64_t)(ri + r) * ld + k0 + q] : 0.0;
+        bv[i] = (!diag && rj + r < n && q < nb) ? A[(int64_t)(rj + r) * ld + k0 + q] : 0.0;
     }
-    if (diag && t < CB) {
-        sm.z[0][t] = t < nb ? rw[k0 + t] : 0.0;
-        sm.z[1][t] = t < nb ? rb[k0 + t] : 0.0;
+#pragma unroll
+    for (int mi = 0; mi < 2; ++mi)
+#pragma unroll
+        for (int nj = 0; nj < 2; ++nj)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int r = ri + wr * 32 + mi * 16 + F64x16::d_row(lane, v);
+                const int c = rj + wc * 32 + nj * 16 + F64x16::d_col(lane);
+                a22[mi][nj][v] = (r < n && c < n && c <= r) ? A[(int64_t)r * ld + c] : 0.0;
+            }
+    double zv = 0.0;
+    if (diag && t < 2 * CB) zv = (t & 63) < nb ? ((t < CB) ? rw : rb)[k0 + (t & 63)] : 0.0;
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+        const int e = t + 256 * i, r = e / CB, q = e % CB;
+        sm.XT[q][r] = xv[i];
+        sm.LiT[q][r] = av[i];
+        if (!diag) sm.LjT[q][r] = bv[i];
     }
+    if (diag && t < 2 * CB) sm.z[t >> 6][t & 63] = zv;
     __syncthreads();
-    // 1. panel rows: thread (ty, tx) forms rows 4 ty.. and columns tx + 16 b
-    double li[4][4] = {}, lj[4][4] = {};
-    for (int q = 0; q < CB; ++q) {
-        double xv[4];
+    // 1. panel rows: L[r][c] = sum_q A21[r][q] X[c][q]
+    dvec4 ci[2][2], cj[2][2];
 #pragma unroll
-        for (int b = 0; b < 4; ++b) xv[b] = sm.X[tx + 16 * b][q];
+    for (int mi = 0; mi < 2; ++mi)
 #pragma unroll
-        for (int a = 0; a < 4; ++a) {
-            const double v = sm.Li[ty * 4 + a][q];
+        for (int nj = 0; nj < 2; ++nj) {
+            ci[mi][nj] = dvec4{0.0, 0.0, 0.0, 0.0};
+            cj[mi][nj] = dvec4{0.0, 0.0, 0.0, 0.0};
+        }
+    for (int q0 = 0; q0 < CB; q0 += 4) {
+        const int q = q0 + lk;
+        const double b0 = sm.XT[q][wc * 32 + lr], b1 = sm.XT[q][wc * 32 + 16 + lr];
 #pragma unroll
-            for (int b = 0; b < 4; ++b) li[a][b] = fma(v, xv[b], li[a][b]);
+        for (int mi = 0; mi < 2; ++mi) {
+            const double a = sm.LiT[q][wr * 32 + mi * 16 + lr];
+            ci[mi][0] = F64x16::mma(a, b0, ci[mi][0]);
+            ci[mi][1] = F64x16::mma(a, b1, ci[mi][1]);
         }
         if (!diag) {
 #pragma unroll
-            for (int a = 0; a < 4; ++a) {
-                const double v = sm.Lj[ty * 4 + a][q];
-#pragma unroll
-                for (int b = 0; b < 4; ++b) lj[a][b] = fma(v, xv[b], lj[a][b]);
+            for (int mi = 0; mi < 2; ++mi) {
+                const double a = sm.LjT[q][wr * 32 + mi * 16 + lr];
+                cj[mi][0] = F64x16::mma(a, b0, cj[mi][0]);
+                cj[mi][1] = F64x16::mma(a, b1, cj[mi][1]);
             }
         }
     }
     __syncthreads();
 #pragma unroll
-    for (int a = 0; a < 4; ++a)
+    for (int mi = 0; mi < 2; ++mi)
 #pragma unroll
-        for (int b = 0; b < 4; ++b) {
-            const int c = tx + 16 * b;
-            sm.Li[ty * 4 + a][c] = c < nb ? li[a][b] : 0.0;
-            if (!diag) sm.Lj[ty * 4 + a][c] = c < nb ? lj[a][b] : 0.0;
-        }
+        for (int nj = 0; nj < 2; ++nj)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int r = wr * 32 + mi * 16 + F64x16::d_row(lane, v), c = wc * 32 + nj * 16 + F64x16::d_col(lane);
+                sm.LiT[c][r] = c < nb ? ci[mi][nj][v] : 0.0;
+                if (!diag) sm.LjT[c][r] = c < nb ? cj[mi][nj][v] : 0.0;
+            }
     __syncthreads();
-    // 2. trailing update of the tile
-    double (*Lc)[CB + 1] = diag ? sm.Li : sm.Lj;
-    double acc[4][4] = {};
-    for (int q = 0; q < nb; ++q) {
-        double u[4];
+    // 2. trailing update of the tile: A22[r][c] -= sum_q L_i[r][q] L_j[c][q]
+    double (*Lc)[CB + 1] = diag ? sm.LiT : sm.LjT;
+    dvec4 cu[2][2];
 #pragma unroll
-        for (int b = 0; b < 4; ++b) u[b] = Lc[tx + 16 * b][q];
+    for (int mi = 0; mi < 2; ++mi)
 #pragma unroll
-        for (int a = 0; a < 4; ++a) {
-            const double v = sm.Li[ty * 4 + a][q];
+        for (int nj = 0; nj < 2; ++nj) cu[mi][nj] = dvec4{0.0, 0.0, 0.0, 0.0};
+    for (int q0 = 0; q0 < CB; q0 += 4) {
+        const int q = q0 + lk;
+        const double b0 = Lc[q][wc * 32 + lr], b1 = Lc[q][wc * 32 + 16 + lr];
 #pragma unroll
-            for (int b = 0; b < 4; ++b) acc[a][b] = fma(v, u[b], acc[a][b]);
+        for (int mi = 0; mi < 2; ++mi) {
+            const double a = sm.LiT[q][wr * 32 + mi * 16 + lr];
+            cu[mi][0] = F64x16::mma(a, b0, cu[mi][0]);
+            cu[mi][1] = F64x16::mma(a, b1, cu[mi][1]);
         }
     }
 #pragma unroll
-    for (int a = 0; a < 4; ++a) {
-        const int r = ri + ty * 4 + a;
-        if (r >= n) continue;
+    for (int mi = 0; mi < 2; ++mi)
 #pragma unroll
-        for (int b = 0; b < 4; ++b) {
-            const int c = rj + tx + 16 * b;
-            if (c < n && c <= r) A[(int64_t)r * ld + c] -= acc[a][b];
-        }
-    }
+        for (int nj = 0; nj < 2; ++nj)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int r = ri + wr * 32 + mi * 16 + F64x16::d_row(lane, v);
+                const int c = rj + wc * 32 + nj * 16 + F64x16::d_col(lane);
+                if (r < n && c < n && c <= r) A[(int64_t)r * ld + c] = a22[mi][nj][v] - cu[mi][nj][v];
+            }
     if (!diag) return;
-    // 3. the tile row's panel, transposed, and the forward substitution of its right-hand-side rows
+    // 3. the forward substitution of the tile row's right-hand-side entries, r_i -= L_i z_k: four
+    // lanes per dot product, 16 terms each
+    {
+        const int row = t >> 2, part = t & 3;
+#pragma unroll
+        for (int w = 0; w < 2; ++w) {
+            double s = 0.0;
+#pragma unroll
+            for (int c = part * 16; c < part * 16 + 16; ++c) s = fma(sm.LiT[c][row], sm.z[w][c], s);
+            s += __shfl_xor(s, 1);
+            s += __shfl_xor(s, 2);
+            if (part == 0 && ri + row < n) (w ? rb : rw)[ri + row] -= s;
+        }
+    }
+    // 4. the next diagonal block (the rows it reads are in memory after the barrier); the panel,
+    // transposed, goes out afterwards: only the backward substitution reads it
+    if (ti == 0) {
+        __syncthreads();
+        // the LDS is reused by the factorisation: keep this tile's panel rows in registers
+        double lt[16];
+#pragma unroll
+        for (int i = 0; i < 16; ++i) {
+            const int e = t + 256 * i, c = e / 64, r = e % 64;
+            lt[i] = sm.LiT[c][r];
+        }
+        __syncthreads();
+        potrf_block(A, ld, n, k0 + CB, orig, info, Dinv_next, rw, rb, *reinterpret_cast<PotrfLds *>(step_smem));
+#pragma unroll
+        for (int i = 0; i < 16; ++i) {
+            const int e = t + 256 * i, c = e / 64, r = e % 64;
+            if (ri + r < n && c < nb) A[(int64_t)(k0 + c) * ld + ri + r] = lt[i];
+        }
+        return;
+    }
     for (int e = t; e < 64 * CB; e += 256) {
         const int c = e / 64, r = e % 64;
-        if (ri + r < n && c < nb) A[(int64_t)(k0 + c) * ld + ri + r] = sm.Li[r][c];
-    }
-    if (t < 128) {
-        const int i = t & 63, which = t >> 6;
-        if (ri + i < n) {
-            double s = 0.0;
-            for (int c = 0; c < nb; ++c) s = fma(sm.Li[i][c], sm.z[which][c], s);
-            double *v = which ? rb : rw;
-            v[ri + i] -= s;
-        }
-    }
-    // 4. the next diagonal block
-    if (ti == 0) {
-        __syncthreads();   // the tile and its right-hand-side rows are in memory; the LDS is reused
-        potrf_block(A, ld, n, k0 + CB, orig, info, Dinv_next, rw, rb, *reinterpret_cast<PotrfLds *>(step_smem));
+        if (ri + r < n && c < nb) A[(int64_t)(k0 + c) * ld + ri + r] = sm.LiT[c][r];
     }
 }
 
