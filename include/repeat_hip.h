@@ -162,7 +162,9 @@ int rp_fit_normal_equations(const rp_fit *fit, double *ata, double *atb, double 
  * partial pivoting, expansion to all atoms.  q_out: N charges; solution_out
  * (N'+1, nullable): reduced charges and the Lagrange multiplier.
  * Returns RP_ESINGULAR if an LU pivot is exactly zero (the reference then
- * falls back to lstsq); *used_fallback is set to 0 otherwise.
+ * falls back to lstsq).  *used_fallback (nullable) is 1 when the system went
+ * through the LU with partial pivoting (a non-positive Cholesky pivot, or a
+ * backward-substitution wait that gave up), 0 when the Cholesky solved it.
  */
 int rp_fit_solve(rp_fit *fit, const rp_constraints *con, double *q_out, double *solution_out,
                  int32_t *used_fallback);

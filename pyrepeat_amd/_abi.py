@@ -374,6 +374,7 @@ class Fit:
         sol = np.empty(nr + 1)
         fb = ctypes.c_int32(0)
         check(load().rp_fit_solve(self.handle, ctypes.byref(c), ptr(q), ptr(sol), ctypes.byref(fb)), self.ctx.handle)
+        self.last_solve_used_lu = bool(fb.value)
         return q, sol
 
     def refit(self, con, c_prepared=None):

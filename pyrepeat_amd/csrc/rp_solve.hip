@@ -26,6 +26,7 @@
 // along, so it holds L^-1 P b at the end; a back substitution finishes.
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 
 #include "rp_fit_common.h"
 #include "rp_mfma.h"
@@ -782,15 +783,16 @@ __global__ __launch_bounds__(256) void k_chol_step(double *__restrict__ A, int l
 //     block k's flag, then subtracts U_{kb,k} y_k;
 //   - forms y_kb = D_kb^T u_kb (D_kb = L_kb,kb^-1) and publishes it: the values, a release fence at
 //     agent scope, then the flag set to this solve's epoch.
-// A wait that sees no progress for BACKSUB_SPIN polls records the failure in *status and the
-// work-group leaves, so the kernel always ends.
-constexpr unsigned BACKSUB_SPIN = 1u << 22;
+// A wait that sees no progress for spin_limit polls records the failure in *status and the
+// work-group leaves, so the kernel always ends; the host then solves the system by the LU.
+constexpr unsigned BACKSUB_SPIN = 1u << 22;   // ~0.3 s of polling; RP_BACKSUB_SPIN overrides it
 
 __global__ __launch_bounds__(256) void k_backsub(const double *__restrict__ A, int ld, int n,
                                                  const double *__restrict__ Dinv, const double *__restrict__ zw,
                                                  const double *__restrict__ zb, double q_tot, double *__restrict__ x,
                                                  unsigned *__restrict__ sync, unsigned base, unsigned epoch,
-                                                 int *__restrict__ status, const int *__restrict__ cinfo) {
+                                                 unsigned spin_limit, int *__restrict__ status,
+                                                 const int *__restrict__ cinfo) {
     __shared__ double Dk[CB][CB + 1];
     __shared__ double uk[CB], yk[CB];
     __shared__ double red[2][4];
@@ -838,13 +840,14 @@ __global__ __launch_bounds__(256) void k_backsub(const double *__restrict__ A, i
 #pragma unroll
         for (int i = 0; i < 16; ++i) ub[i] = crow && q + i < kn ? row[i] : 0.0;
         if (t == 0) {
-            unsigned spins = 0;
-            while (__hip_atomic_load(flags + k, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) != epoch) {
-                if (++spins > BACKSUB_SPIN) {
+            // at most spin_limit polls (none when it is 0)
+            for (unsigned spins = 0;; ++spins) {
+                if (spins >= spin_limit) {
                     sh_abort = 1;
                     atomicExch(status, 1);
                     break;
                 }
+                if (__hip_atomic_load(flags + k, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == epoch) break;
                 __builtin_amdgcn_s_sleep(2);
             }
         }
@@ -1064,9 +1067,14 @@ int run_solve(rp_fit *fit, const rp_constraints *con, double *q_out, double *sol
         fit->bs_epoch = 0;
     }
     ++fit->bs_epoch;
+    static const unsigned spin_limit = [] {
+        const char *v = std::getenv("RP_BACKSUB_SPIN");
+        return v ? (unsigned)std::strtoul(v, nullptr, 10) : BACKSUB_SPIN;
+    }();
     k_backsub<<<nblk, 256, 0, st>>>(d_K.as<double>(), ld, nr, d_dinv.as<double>(), d_wc.as<double>(),
                                     d_bc.as<double>(), con->q_tot, d_x.as<double>(), d_sync.as<unsigned>(),
-                                    fit->bs_base, fit->bs_epoch, d_cinfo.as<int>() + 1, d_cinfo.as<int>());
+                                    fit->bs_base, fit->bs_epoch, spin_limit, d_cinfo.as<int>() + 1,
+                                    d_cinfo.as<int>());
     fit->bs_base += (unsigned)nblk;
     k_publish<<<ceil_div(N + 1, 256), 256, 0, st>>>(d_x.as<double>(), dc_exp, N, n, d_cinfo.as<int>(), h_out_dev);
     delete kts;   // records the end event before the synchronisation below
@@ -1096,7 +1104,7 @@ int run_solve(rp_fit *fit, const rp_constraints *con, double *q_out, double *sol
         if (solution_out) RP_HIP(ctx, hipMemcpyAsync(solution_out, d_x.p, sizeof(double) * n, hipMemcpyDeviceToHost, st));
         RP_HIP(ctx, hipStreamSynchronize(st));
     }
-    if (used_fallback) *used_fallback = 0;
+    if (used_fallback) *used_fallback = fit->lu_fallback ? 1 : 0;
     ctx->collect();
     fit->resolve_timings(false);
     float ms = 0.f;

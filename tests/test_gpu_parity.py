@@ -13,6 +13,8 @@ from pyrepeat_amd import _abi, synth
 from pyrepeat_amd.calculator import ChargeCalculator
 from pyrepeat_amd.cube import cube_from_system
 
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
 pytestmark = pytest.mark.gpu
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 
@@ -201,6 +203,41 @@ def test_singular_kkt_reports(gpu_ctx):
     calc.close()
 
 
+def test_backsub_wait_limit_falls_back_to_lu(gpu_ctx, tmp_path):
+    """The one-launch backward substitution waits on flags with a bounded poll count.  With the
+    limit forced to zero from the host (RP_BACKSUB_SPIN=0, a fresh process), every wait gives up
+    without polling; the solve must notice and answer through the LU, with the Cholesky's charges."""
+    import subprocess
+    import sys
+    script = tmp_path / "spin.py"
+    script.write_text(
+        "import sys, numpy as np\n"
+        f"sys.path.insert(0, {str(ROOT)!r})\n"
+        "from pyrepeat_amd import _abi, params, synth\n"
+        "s = synth.crystal200()\n"
+        "ctx = _abi.Context(0, 0)\n"
+        "f = _abi.Fit(ctx, s.n_grid, s.axis, s.atomic_numbers, s.atom_pos, s.esp_cube, _abi.RP_ORDER_CUBE, 1,\n"
+        "             _abi.RP_PREC_FP64)\n"
+        "f.accumulate()\n"
+        "q, _ = f.solve(params.constraint_arrays(s.n_atoms, s.atomic_numbers, 0.0, None, None, None))\n"
+        "np.save(sys.argv[1], q)\n"
+        "print('lu', int(f.last_solve_used_lu))\n")
+    out = {}
+    for spin in ("0", None):
+        env = dict(os.environ)
+        env.pop("RP_BACKSUB_SPIN", None)
+        if spin is not None:
+            env["RP_BACKSUB_SPIN"] = spin
+        path = tmp_path / f"q_{spin}.npy"
+        print(f"backsub wait limit {spin}", flush=True)
+        r = subprocess.run([sys.executable, str(script), str(path)], env=env, capture_output=True, text=True,
+                           timeout=240)
+        assert r.returncode == 0, r.stderr[-2000:]
+        out[spin] = (np.load(path), r.stdout.split()[-1])
+    assert out["0"][1] == "1" and out[None][1] == "0"
+    assert np.max(np.abs(out["0"][0] - out[None][0])) <= 1e-8
+
+
 def test_deterministic_bitwise(gpu_ctx):
     s = synth.crystal200()
     qs = []
@@ -212,7 +249,7 @@ def test_deterministic_bitwise(gpu_ctx):
 
 
 # ---------------------------------------------------------------- split precision path
-# (factorized fp64 reciprocal sum, fp32 real space, int8-sliced Gram with exact int32 groups)
+# (factorized fp64 reciprocal sum, fp64 real space, int8-sliced Gram with exact int32 groups)
 
 def test_fit_type0_split_matches_oracle(oracle, gpu_ctx):
     """Bare 1/r basis (no reciprocal part) through the split path."""
