@@ -439,14 +439,24 @@ __device__ __forceinline__ void potrf_block(double *__restrict__ A, int ld, int 
             if (lower && i < nb && c <= i) A[(int64_t)(k0 + i) * ld + k0 + c] = a[u][v];
         }
     __syncthreads();
-    if (t < nb) {
+    {
+        // z = X r for both right-hand sides: four lanes per row, 16 terms each (X is lower
+        // triangular, zero above the diagonal)
+        const int row = t >> 2, part = t & 3;
         double sa = 0.0, sb = 0.0;
-        for (int c = 0; c <= t; ++c) {
-            sa += Xs[t][c] * zw[c];
-            sb += Xs[t][c] * zb[c];
+#pragma unroll
+        for (int c = part * 16; c < part * 16 + 16; ++c) {
+            sa = fma(Xs[row][c], zw[c], sa);
+            sb = fma(Xs[row][c], zb[c], sb);
         }
-        rw[k0 + t] = sa;
-        rb[k0 + t] = sb;
+        sa += __shfl_xor(sa, 1);
+        sb += __shfl_xor(sb, 1);
+        sa += __shfl_xor(sa, 2);
+        sb += __shfl_xor(sb, 2);
+        if (part == 0 && row < nb) {
+            rw[k0 + row] = sa;
+            rb[k0 + row] = sb;
+        }
     }
     if (t == 0 && bad && info[0] == 0) info[0] = bad;
 }
