@@ -817,6 +817,10 @@ __global__ __launch_bounds__(256) void k_backsub(const double *__restrict__ A, i
     __syncthreads();
     if (*cinfo != 0) return;   // a pivot failed: the host reruns the system through the LU
     const int nblk = (n + CB - 1) / CB;
+    if (sh_ticket >= (unsigned)nblk) {   // tickets out of step with the host's count: give up, LU answers
+        if (t == 0) atomicExch(status, 1);
+        return;
+    }
     const int kb = nblk - 1 - (int)sh_ticket;
     const int j0 = kb * CB, nb = min(CB, n - j0);
     for (int e = t; e < CB * CB; e += 256) Dk[e / CB][e % CB] = Dinv[(int64_t)kb * CB * CB + e];
@@ -1098,8 +1102,16 @@ int run_solve(rp_fit *fit, const rp_constraints *con, double *q_out, double *sol
         if (solution_out) std::copy(h_out + 1 + N, h_out + 1 + N + n, solution_out);
     }
     fit->lu_fallback = cinfo != 0;
+    if (cinfo < 0) {
+        // a backward-substitution wait gave up: start the tickets and flags afresh for the next solve
+        const int words = (int)(d_sync.bytes / sizeof(unsigned));
+        k_fill_u32<<<ceil_div(words, 256), 256, 0, st>>>(d_sync.as<unsigned>(), words, 0u);
+        fit->bs_base = 0;
+        fit->bs_epoch = 0;
+    }
     if (cinfo != 0) {
-        // indefinite or singular A: LU with partial pivoting on the bordered system
+        // indefinite or singular A (or a backward-substitution wait that gave up): LU with partial
+        // pivoting on the bordered system
         const int rc = lu_solve_augmented(ctx, d_K2.as<double>(), ld, n, d_piv.as<int>(), d_info.as<int>(),
                                           d_x.as<double>(), st);
         if (rc != RP_OK) {
