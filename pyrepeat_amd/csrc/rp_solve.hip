@@ -854,14 +854,18 @@ __global__ __launch_bounds__(256) void k_backsub(const double *__restrict__ A, i
 #pragma unroll
         for (int i = 0; i < 16; ++i) ub[i] = crow && q + i < kn ? row[i] : 0.0;
         if (t == 0) {
-            // at most spin_limit polls (none when it is 0)
+            // at most spin_limit polls (none when it is 0); the polls are relaxed and one acquire
+            // fence follows the flag, so the wait does not invalidate the cache on every poll
             for (unsigned spins = 0;; ++spins) {
                 if (spins >= spin_limit) {
                     sh_abort = 1;
                     atomicExch(status, 1);
                     break;
                 }
-                if (__hip_atomic_load(flags + k, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == epoch) break;
+                if (__hip_atomic_load(flags + k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == epoch) {
+                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+                    break;
+                }
                 __builtin_amdgcn_s_sleep(2);
             }
         }
