@@ -257,37 +257,44 @@ __device__ __forceinline__ double rsqrt_newton(double d) {
 // right-hand sides, z_k = X r_k (rows k0.. of rw, rb, in place).
 // The block lives in registers: thread (tr, tc) holds rows 4 tr.. and columns 4 tc.. of the Schur
 // complement S and of W, the right-looking inverse (W = I at the start).  Step J takes the 8 pivots
-// j = 8J .. 8J+7 in two phases:
-//   A (wave 0, after the step's first barrier): from columns j..j+7 of S and rows j..j+7 of W, which
-//     their owners published to LDS, one lane per row forms L[r][j..j+7] (the 8 pivots one after
-//     another, the pivot rows broadcast by readlane), then one lane per column forms
-//     X[j..j+7][c] = L_jj^-1 W[j..j+7][c];
-//   B (all threads, after the second barrier): the rank-8 update S -= L L^T, W -= L X of the blocks
-//     below and right of the pivot block, the finished L and X entries into their owners' registers,
-//     and the next 8 columns of S and rows of W published.
-// Ownership is aligned with the 8-pivot blocks (two 4-wide thread blocks), so no entry is masked.
-// 8 steps, two barriers each.  A pivot at or below 1e-13 of the original diagonal entry counts as a
-// failure: the system is then left to the pivoted LU, which reports exact singularity as the
+// j = 8J .. 8J+7:
+//   A (wave 0): from the 8 columns j.. of S and the 8 rows j.. of W (one lane per row, resp. per
+//     column), L[r][j..j+7] (the pivots one after another, the pivot rows broadcast by readlane) and
+//     X[j..j+7][c] = L_jj^-1 W[j..j+7][c], into LDS;
+//   B (the owners): the rank-8 update S -= L L^T, W -= L X of their blocks below and right of the
+//     pivot block, and the finished L and X entries into place.
+// Lookahead: wave 0 does not wait for B of step J to start A of step J + 1.  The owners publish the
+// columns and rows of step J + 2 as they stand after their update of step J (two steps ahead), and
+// wave 0 brings those of step J + 1 up to date itself with the rank-8 update of step J, so A(J + 1)
+// runs while the owners are still in B(J): one barrier per step.  (Wave 0 owns blocks of the trailing
+// part only in steps 0 and 1; it does those updates after A.)  The panels, W rows, L and X go through
+// double buffers by step parity.  Ownership is aligned with the 8-pivot blocks (two 4-wide thread
+// blocks), so no entry is masked.  A pivot at or below 1e-13 of the original diagonal entry counts as
+// a failure: the system is then left to the pivoted LU, which reports exact singularity as the
 // reference's LAPACK call does.
 struct PotrfLds {
-    double colS[CB][PB + 1];    // S[r][j + q]
-    double xr[PB][CB];          // W[j + q][c]
-    double lbT[PB][CB];         // L[r][j + q], transposed
-    double xb[PB][CB];          // X[j + q][c]
+    union {
+        struct {
+            double colS[2][CB][PB + 1];   // S[r][j + q] of a step, as published
+            double xr[2][PB][CB];         // W[j + q][c] of a step, as published
+            double lbT[2][PB][CB];        // L[r][j + q], transposed
+            double xb[2][PB][CB];         // X[j + q][c]
+        } loop;
+        double Xs[CB][CB + 1];            // the inverse, for the outputs
+    } u;
     double od[CB];              // original diagonal entries for the pivot test
-    double Xs[CB][CB + 1];
     double zw[CB], zb[CB];
 };
 
 __device__ __forceinline__ void potrf_block(double *__restrict__ A, int ld, int n, int k0, const double *__restrict__ orig,
                                             int *__restrict__ info, double *__restrict__ Dinv, double *__restrict__ rw,
                                             double *__restrict__ rb, PotrfLds &sm) {
-    auto &colS = sm.colS;
-    auto &xr = sm.xr;
-    auto &lbT = sm.lbT;
-    auto &xb = sm.xb;
+    auto &colS = sm.u.loop.colS;
+    auto &xr = sm.u.loop.xr;
+    auto &lbT = sm.u.loop.lbT;
+    auto &xb = sm.u.loop.xb;
     auto &od = sm.od;
-    auto &Xs = sm.Xs;
+    auto &Xs = sm.u.Xs;
     auto &zw = sm.zw;
     auto &zb = sm.zb;
     const int nb = min(CB, n - k0);
@@ -308,83 +315,106 @@ __device__ __forceinline__ void potrf_block(double *__restrict__ A, int ld, int 
         od[t] = t < nb ? orig[(int64_t)(k0 + t) * ld + k0 + t] : 1.0;
     }
     // W is lower triangular: row entries right of the owners' blocks stay zero
-    for (int e = t; e < PB * CB; e += 256) xr[e / CB][e % CB] = 0.0;
+    for (int e = t; e < 2 * PB * CB; e += 256) (&xr[0][0][0])[e] = 0.0;
     __syncthreads();
-    // columns 0..7 of S and rows 0..7 of W
-    if (lower && tc < 2) {
+    // columns and W rows of steps 0 and 1 (S and W as loaded)
+    if (lower && tc < 4) {
 #pragma unroll
         for (int u = 0; u < 4; ++u)
 #pragma unroll
-            for (int v = 0; v < 4; ++v) colS[4 * tr + u][4 * tc + v] = a[u][v];
+            for (int v = 0; v < 4; ++v) colS[tc >> 1][4 * tr + u][4 * (tc & 1) + v] = a[u][v];
     }
-    if (lower && tr < 2) {
+    if (lower && tr < 4) {
 #pragma unroll
         for (int u = 0; u < 4; ++u)
 #pragma unroll
-            for (int v = 0; v < 4; ++v) xr[4 * tr + u][4 * tc + v] = x[u][v];
+            for (int v = 0; v < 4; ++v) xr[tr >> 1][4 * (tr & 1) + u][4 * tc + v] = x[u][v];
     }
+    __syncthreads();
     int bad = 0;
-    for (int J = 0; J < CB / PB; ++J) {
-        const int j = J * PB;
-        __syncthreads();
-        // ---- phase A: wave 0
-        if (wave == 0) {
-            // right-looking over the 8 pivots: only one multiply-add per pivot lies on the chain to
-            // the next pivot; the subtraction order per entry is the same as a left-looking sum
-            double c[PB], l[PB], Ljj[PB][PB], inv[PB];
+    // wave 0's panel of the next step: c[q] = S[lane][j + q], w[q] = W[j + q][lane]
+    double c[PB], w[PB];
+    if (wave == 0) {
 #pragma unroll
-            for (int q = 0; q < PB; ++q) c[q] = colS[lane][q];
+        for (int q = 0; q < PB; ++q) {
+            c[q] = colS[0][lane][q];
+            w[q] = xr[0][q][lane];
+        }
+    }
+    // ---- phase A of step J from c, w (wave 0): L and X of the step into buffer J & 1
+    auto phase_a = [&](int J) {
+        const int j = J * PB, bf = J & 1;
+        double l[PB], Ljj[PB][PB], inv[PB];
 #pragma unroll
-            for (int q = 0; q < PB; ++q) {
-                const double d = readlane_f64(c[q], j + q);
-                const bool live = j + q < nb;
-                const bool ok = d > 1e-13 * fabs(od[j + q]);
-                if (live && !ok && !bad) bad = k0 + j + q + 1;
-                const double dd = live && ok ? d : 1.0;   // dead or failed pivots stay 1
-                const double ip = rsqrt_newton(dd);
-                const double piv = dd * ip;
-                inv[q] = ip;
-                l[q] = lane > j + q ? c[q] * ip : (lane == j + q ? piv : 0.0);
+        for (int q = 0; q < PB; ++q) {
+            const double d = readlane_f64(c[q], j + q);
+            const bool live = j + q < nb;
+            const bool ok = d > 1e-13 * fabs(od[j + q]);
+            if (live && !ok && !bad) bad = k0 + j + q + 1;
+            const double dd = live && ok ? d : 1.0;   // dead or failed pivots stay 1
+            const double ip = rsqrt_newton(dd);
+            const double piv = dd * ip;
+            inv[q] = ip;
+            l[q] = lane > j + q ? c[q] * ip : (lane == j + q ? piv : 0.0);
 #pragma unroll
-                for (int p = q + 1; p < PB; ++p) {
-                    Ljj[p][q] = readlane_f64(l[q], j + p);
-                    c[p] = fma(-l[q], Ljj[p][q], c[p]);
-                }
-            }
-#pragma unroll
-            for (int q = 0; q < PB; ++q) lbT[q][lane] = l[q];
-            double x[PB];
-#pragma unroll
-            for (int q = 0; q < PB; ++q) x[q] = xr[q][lane];
-#pragma unroll
-            for (int q = 0; q < PB; ++q) {
-                const double xv = x[q] * inv[q];
-                xb[q][lane] = xv;
-#pragma unroll
-                for (int p = q + 1; p < PB; ++p) x[p] = fma(-Ljj[p][q], xv, x[p]);
+            for (int p = q + 1; p < PB; ++p) {
+                Ljj[p][q] = readlane_f64(l[q], j + p);
+                c[p] = fma(-l[q], Ljj[p][q], c[p]);
             }
         }
-        __syncthreads();
-        // ---- phase B: all threads
+#pragma unroll
+        for (int q = 0; q < PB; ++q) lbT[bf][q][lane] = l[q];
+#pragma unroll
+        for (int q = 0; q < PB; ++q) {
+            const double xv = w[q] * inv[q];
+            xb[bf][q][lane] = xv;
+#pragma unroll
+            for (int p = q + 1; p < PB; ++p) w[p] = fma(-Ljj[p][q], xv, w[p]);
+        }
+    };
+    if (wave == 0) phase_a(0);
+    for (int J = 0; J < CB / PB; ++J) {
+        const int j = J * PB, bf = J & 1;
+        __syncthreads();   // L, X of step J and the step-(J+1) publications are in LDS
+        // ---- wave 0: step J + 1's panel brought up to date with step J's rank-8 update, then its A
+        if (wave == 0 && J + 1 < CB / PB) {
+            const int jn = j + PB, bn = (J + 1) & 1;
+            double lr[PB];
+#pragma unroll
+            for (int p = 0; p < PB; ++p) lr[p] = lbT[bf][p][lane];
+#pragma unroll
+            for (int q = 0; q < PB; ++q) {
+                double cv = colS[bn][lane][q], wv = xr[bn][q][lane];
+#pragma unroll
+                for (int p = 0; p < PB; ++p) {
+                    cv = fma(-lr[p], lbT[bf][p][jn + q], cv);
+                    wv = fma(-lbT[bf][p][jn + q], xb[bf][p][lane], wv);
+                }
+                c[q] = cv;
+                w[q] = wv;
+            }
+            phase_a(J + 1);
+        }
+        // ---- phase B of step J (owners)
         if (lower && tr >= 2 * J) {
             if (tr >= 2 * J + 2) {
                 // rows below the pivot block: W -= L X, S -= L L^T right of it, L columns into place
                 const bool upd = tc >= 2 * J + 2, put = tc >= 2 * J && tc < 2 * J + 2;
 #pragma unroll
                 for (int q = 0; q < PB; ++q) {
-                    const double2 l01 = *reinterpret_cast<const double2 *>(&lbT[q][4 * tr]);
-                    const double2 l23 = *reinterpret_cast<const double2 *>(&lbT[q][4 * tr + 2]);
+                    const double2 l01 = *reinterpret_cast<const double2 *>(&lbT[bf][q][4 * tr]);
+                    const double2 l23 = *reinterpret_cast<const double2 *>(&lbT[bf][q][4 * tr + 2]);
                     const double lr[4] = {l01.x, l01.y, l23.x, l23.y};
-                    const double2 x01 = *reinterpret_cast<const double2 *>(&xb[q][4 * tc]);
-                    const double2 x23 = *reinterpret_cast<const double2 *>(&xb[q][4 * tc + 2]);
+                    const double2 x01 = *reinterpret_cast<const double2 *>(&xb[bf][q][4 * tc]);
+                    const double2 x23 = *reinterpret_cast<const double2 *>(&xb[bf][q][4 * tc + 2]);
                     const double xj[4] = {x01.x, x01.y, x23.x, x23.y};
 #pragma unroll
                     for (int u = 0; u < 4; ++u)
 #pragma unroll
                         for (int v = 0; v < 4; ++v) x[u][v] = fma(-lr[u], xj[v], x[u][v]);
                     if (upd) {
-                        const double2 c01 = *reinterpret_cast<const double2 *>(&lbT[q][4 * tc]);
-                        const double2 c23 = *reinterpret_cast<const double2 *>(&lbT[q][4 * tc + 2]);
+                        const double2 c01 = *reinterpret_cast<const double2 *>(&lbT[bf][q][4 * tc]);
+                        const double2 c23 = *reinterpret_cast<const double2 *>(&lbT[bf][q][4 * tc + 2]);
                         const double lc[4] = {c01.x, c01.y, c23.x, c23.y};
 #pragma unroll
                         for (int u = 0; u < 4; ++u)
@@ -396,7 +426,7 @@ __device__ __forceinline__ void potrf_block(double *__restrict__ A, int ld, int 
 #pragma unroll
                     for (int u = 0; u < 4; ++u)
 #pragma unroll
-                        for (int v = 0; v < 4; ++v) a[u][v] = lbT[4 * tc + v - j][4 * tr + u];
+                        for (int v = 0; v < 4; ++v) a[u][v] = lbT[bf][4 * tc + v - j][4 * tr + u];
                 }
             } else {
                 // rows of the pivot block: their X rows and the pivot block of L
@@ -404,29 +434,30 @@ __device__ __forceinline__ void potrf_block(double *__restrict__ A, int ld, int 
                 for (int u = 0; u < 4; ++u)
 #pragma unroll
                     for (int v = 0; v < 4; ++v) {
-                        const int r = 4 * tr + u, c = 4 * tc + v;
-                        x[u][v] = xb[r - j][c];
-                        if (tc >= 2 * J && c <= r) a[u][v] = lbT[c - j][r];
+                        const int r = 4 * tr + u, cc = 4 * tc + v;
+                        x[u][v] = xb[bf][r - j][cc];
+                        if (tc >= 2 * J && cc <= r) a[u][v] = lbT[bf][cc - j][r];
                     }
             }
-            // publish columns j+8.. of S and rows j+8.. of W for the next step
-            if (J + 1 < CB / PB) {
-                const int nj = j + PB;
-                if (tc >= 2 * J + 2 && tc < 2 * J + 4 && tr >= 2 * J + 2) {
+            // publish the columns of S and the rows of W of step J + 2, as they stand now
+            if (J + 2 < CB / PB) {
+                const int j2 = j + 2 * PB;
+                if (tc >= 2 * J + 4 && tc < 2 * J + 6 && tr >= 2 * J + 4) {
 #pragma unroll
                     for (int u = 0; u < 4; ++u)
 #pragma unroll
-                        for (int v = 0; v < 4; ++v) colS[4 * tr + u][4 * tc + v - nj] = a[u][v];
+                        for (int v = 0; v < 4; ++v) colS[bf][4 * tr + u][4 * tc + v - j2] = a[u][v];
                 }
-                if (tr >= 2 * J + 2 && tr < 2 * J + 4) {
+                if (tr >= 2 * J + 4 && tr < 2 * J + 6) {
 #pragma unroll
                     for (int u = 0; u < 4; ++u)
 #pragma unroll
-                        for (int v = 0; v < 4; ++v) xr[4 * tr + u - nj][4 * tc + v] = x[u][v];
+                        for (int v = 0; v < 4; ++v) xr[bf][4 * tr + u - j2][4 * tc + v] = x[u][v];
                 }
             }
         }
     }
+    __syncthreads();   // every owner is past its last update; the LDS is reused for the inverse
     // ---- outputs: factor, inverse, z_k = X r_k
 #pragma unroll
     for (int u = 0; u < 4; ++u)
