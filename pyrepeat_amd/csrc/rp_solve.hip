@@ -350,7 +350,7 @@ __device__ __forceinline__ void potrf_block(double *__restrict__ A, int ld, int 
             const double d = readlane_f64(c[q], j + q);
             const bool live = j + q < nb;
             const bool ok = d > 1e-13 * fabs(od[j + q]);
-            if (live && !ok && !bad) bad = k0 + j + q + 1;
+            bad = (live && !ok && !bad) ? k0 + j + q + 1 : bad;   // a select: no branch in the chain
             const double dd = live && ok ? d : 1.0;   // dead or failed pivots stay 1
             const double ip = rsqrt_newton(dd);
             const double piv = dd * ip;
