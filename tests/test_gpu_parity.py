@@ -238,6 +238,30 @@ def test_backsub_wait_limit_falls_back_to_lu(gpu_ctx, tmp_path):
     assert np.max(np.abs(out["0"][0] - out[None][0])) <= 1e-8
 
 
+def test_solve_large_n_matches_dense_solve(oracle, gpu_ctx):
+    """N' = 1600 takes the Cholesky path for large systems: while the trailing triangle has more
+    64 x 64 tiles than the GPU has CUs, the panel is its own launch and the update reads it; the last
+    steps fuse them; the backward substitution runs 25 chained blocks.  The device's normal equations,
+    centred and solved densely by the oracle (LU), must give the device solve's charges."""
+    from pyrepeat_amd import params
+    s = synth.tiny(seed=21, n_atoms=1600, n=76, L=90.0)
+    print("large-N solve: 1600 atoms, 76x77x78 grid", flush=True)
+    f = _abi.Fit(gpu_ctx, s.n_grid, s.axis, s.atomic_numbers, s.atom_pos, s.esp_cube, _abi.RP_ORDER_CUBE, 1,
+                 _abi.RP_PREC_FP64)
+    try:
+        f.accumulate()
+        ata, atb, sp, sv = f.normal_equations()
+        M = int(f.info.n_keep)
+        assert M > 4 * s.n_atoms
+        A, b = oracle.centre(ata, atb, sp, sv, M)
+        sol, _ = oracle.solve_kkt(A, b, np.ones(s.n_atoms), 0.0)
+        q, _ = f.solve(params.constraint_arrays(s.n_atoms, s.atomic_numbers, 0.0, None, None, None))
+        assert not f.last_solve_used_lu
+        assert np.max(np.abs(q - sol[:s.n_atoms])) <= 1e-6
+    finally:
+        f.close()
+
+
 def test_deterministic_bitwise(gpu_ctx):
     s = synth.crystal200()
     qs = []
