@@ -97,7 +97,7 @@ def _constraints(system, tmpdir):
 
 def run_gpu(ctx, system, con, precision, steps, warmup):
     from pyrepeat_amd import _abi
-    prec = _abi.RP_PREC_FP64 if precision == "fp64" else _abi.RP_PREC_OZAKI
+    prec = {"fp64": _abi.RP_PREC_FP64, "split": _abi.RP_PREC_OZAKI, "refined": _abi.RP_PREC_OZAKI_REFINED}[precision]
     fit = _abi.Fit(ctx, system.n_grid, system.axis, system.atomic_numbers, system.atom_pos, system.esp_cube,
                    _abi.RP_ORDER_CUBE, 1, prec, 1.0, 1000.0, 20.0)
     prepared = fit._constraints(con)
@@ -125,8 +125,8 @@ def run_gpu(ctx, system, con, precision, steps, warmup):
     M = int(info.n_keep)
     N = int(fit.n_atoms)
     np_ = -(-(N + 2) // 128) * 128
-    slices = 4
-    if prec == _abi.RP_PREC_OZAKI:     # chunk plan of the split path: <= 65536 rows, ~2 GiB of chunk buffers
+    slices = 3 if prec == _abi.RP_PREC_OZAKI_REFINED else 4
+    if prec != _abi.RP_PREC_FP64:     # chunk plan of the split path: <= 65536 rows, ~2 GiB of chunk buffers
         rows = max(512, min(65536, -(-M // 512) * 512))
         while rows > 512 and rows * (8 + slices) * np_ > 1 << 31:
             rows //= 2
@@ -135,7 +135,8 @@ def run_gpu(ctx, system, con, precision, steps, warmup):
         rows = min(65536, -(-M // 128) * 128)
     out = {"q": q, "lat": np.array(lat), "total": total, "kernels": stats, "breakdown_fits": breakdown_fits,
            "gram": gram, "M": M, "rows": rows, "slices": slices,
-           "N": int(fit.n_atoms), "K": int(info.n_k), "timings": fit.timings()}
+           "N": int(fit.n_atoms), "K": int(info.n_k), "timings": fit.timings(),
+           "refine": fit.refine_stats() if prec == _abi.RP_PREC_OZAKI_REFINED else None}
     fit.close()
     return out
 
@@ -279,6 +280,8 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="mof1000")
+    ap.add_argument("--chosen", choices=["split", "refined"], default="split",
+                    help="the chosen path: split (4 slices) or refined (3 slices + fp64 refinement)")
     ap.add_argument("--baseline-steps", type=int, default=None)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
@@ -308,8 +311,8 @@ def main():
     with tempfile.TemporaryDirectory() as td:
         con = _constraints(system, td)
     ctx = _abi.Context(0, 0)
-    _log("chosen (split) path ...")
-    split = run_gpu(ctx, system, con, "split", args.steps, args.warmup)
+    _log(f"chosen ({args.chosen}) path ...")
+    split = run_gpu(ctx, system, con, args.chosen, args.steps, args.warmup)
     _log(f"  {args.steps / split['total']:.2f} fits/s, p50 {1e3 * np.median(split['lat']):.2f} ms")
     bsteps = args.baseline_steps or args.steps
     _log("plain fp64 path ...")
@@ -323,7 +326,7 @@ def main():
     if os.path.exists(golden):
         g = json.load(open(golden))
         ref = np.array(g["charges"])
-        err_oracle = {"split_max_dq_e": float(np.max(np.abs(split["q"] - ref))),
+        err_oracle = {"chosen_max_dq_e": float(np.max(np.abs(split["q"] - ref))),
                       "fp64_max_dq_e": float(np.max(np.abs(f64["q"] - ref)))}
 
     fits_s = args.steps / split["total"]
@@ -347,7 +350,7 @@ def main():
             "launch_ms": round(ms / launches, 4), "rows_per_launch": rows,
             "note": "achieved = algorithmic fp64-equivalent flops of the Gram per launch (F_G share of the "
                     "launch's rows) / mean launch time (HIP events on the fit's stream); peak = dense int8 MFMA, "
-                    "the unit the kernel runs on; each fp product costs P^2 = 16 int8 slice products, which "
+                    f"the unit the kernel runs on; each fp product costs P^2 = {P * P} int8 slice products, which "
                     "executed_int8_tops counts", "slices": P}
     cpu = None
     if not args.no_cpu:
@@ -365,7 +368,9 @@ def main():
         "vs_baseline": None, "dtype": "i8 slices -> i32 exact, f64 accumulate/solve",
         "data": "synthetic (seeded exact Ewald ESP + 1e-5 noise)",
         "config": {"workload": args.workload, "n_atoms": N, "grid": [int(v) for v in system.n_grid],
-                   "kept_points": M, "k_vectors": K, "precision": "split (int8 Ozaki)"},
+                   "kept_points": M, "k_vectors": K,
+                   "precision": "split (int8 Ozaki, 4 slices)" if args.chosen == "split" else
+                   "refined (int8 Ozaki, 3 slices, fp64 residual refinement)"},
         "p50_ms": round(1e3 * float(np.median(split["lat"])), 3),
         "p99_ms": round(1e3 * float(np.percentile(split["lat"], 99)), 3),
         "fp64_fits_per_s": round(fits_s64, 4),
@@ -373,6 +378,7 @@ def main():
         "speedup_vs_fp64_gpu": round(fits_s / fits_s64, 3),
         "max_dq_vs_fp64_e": dq_vs_fp64, "within_1e-5": dq_vs_fp64 <= 1e-5,
         "error_vs_oracle": err_oracle,
+        "refine_steps_last_correction_e": split["refine"],
         "kernels_chosen": kms, "kernels_fp64": kms64,
         "roofline": roof,
         "cpu_baseline": cpu,

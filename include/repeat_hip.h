@@ -48,6 +48,14 @@ extern "C" {
                             real space in fp64 (erfc as one polynomial), Gram from
                             int8 fixed-point slices on MFMA with exact int32 accumulation,
                             fp64 across chunks and in the KKT solve */
+#define RP_PREC_OZAKI_REFINED 2  /* the split form with 3 slices per element, then iterative
+                            refinement of the solve: a step forms the residual of the fp64
+                            normal equations, Phi^T (V - Phi q) less the centring term, from
+                            the fp64 basis kept in HBM (M x (N + 2) doubles), and corrects q
+                            and the multiplier with the Cholesky factor of the split system.
+                            The result converges to the fp64 system's solution.  Steps stop
+                            when a correction is at most RP_REFINE_STOP_E (max |dq|, e). */
+#define RP_REFINE_STOP_E 1e-4
 
 typedef struct rp_ctx rp_ctx;
 typedef struct rp_fit rp_fit;
@@ -67,10 +75,13 @@ typedef struct rp_system {
 /* Command-line options of repeat.py that reach the fit path (repeat.py:2655-2660, 2680). */
 typedef struct rp_options {
     int32_t fit_type;            /* --fit-type: 1 Ewald (default), 0 bare 1/r */
-    int32_t precision;           /* RP_PREC_FP64 (--fp64) | RP_PREC_OZAKI */
+    int32_t precision;           /* RP_PREC_FP64 (--fp64) | RP_PREC_OZAKI | RP_PREC_OZAKI_REFINED */
     double vdw_factor;           /* --vdw-factor, default 1.0 */
     double vdw_max;              /* --vdw-max, default 1000.0 */
     double r_cutoff;             /* --cutoff, default 20.0 */
+    int32_t refine_steps;        /* RP_PREC_OZAKI_REFINED: at most this many refinement steps
+                                    (<= 0: 2); no counterpart in the reference */
+    int32_t reserved;            /* 0 */
 } rp_options;
 
 /* Sizes of a prepared fit. */
@@ -117,7 +128,8 @@ int rp_ctx_hbm_in_use(const rp_ctx *ctx, uint64_t *bytes);
 /* Kernel timing with HIP events on the context's stream (off by default; enabling resets the
  * counters).  kind: 0 basis (k-space + real space), 1 Gram, 2 filter mask, 3 KKT solve,
  * 4 per-fit tables (grid phases, factorized reciprocal tables, line table), 5 Phi slicing,
- * 6 reduction of the Gram's row-split partials.  total_ms sums the launches of that kind
+ * 6 reduction of the Gram's row-split partials, 7 refinement of the solve (residual passes over
+ * the kept basis and the triangular solves of the correction).  total_ms sums the launches of that kind
  * since the last reset. */
 int rp_ctx_profile(rp_ctx *ctx, int enable);
 /* The same for the kinds whose bit is set (bit k = kind k) only.  Every timed launch adds two event
@@ -165,6 +177,8 @@ int rp_fit_normal_equations(const rp_fit *fit, double *ata, double *atb, double 
  * falls back to lstsq).  *used_fallback (nullable) is 1 when the system went
  * through the LU with partial pivoting (a non-positive Cholesky pivot, or a
  * backward-substitution wait that gave up), 0 when the Cholesky solved it.
+ * Under RP_PREC_OZAKI_REFINED the Cholesky solution is refined against the fp64
+ * basis kept by rp_fit_accumulate (an LU fallback is not refined).
  */
 int rp_fit_solve(rp_fit *fit, const rp_constraints *con, double *q_out, double *solution_out,
                  int32_t *used_fallback);
@@ -189,6 +203,13 @@ int rp_fit_phi(rp_fit *fit, int64_t m, const double *grid_xyz, double *phi_out);
 /* Device time (ms) of the phases of the last accumulate / solve on this fit:
  * t[0] filter, t[1] accumulate, t[2] solve. */
 int rp_fit_timings(const rp_fit *fit, double *t3);
+
+/*
+ * Iterative refinement of the last rp_fit_solve (RP_PREC_OZAKI_REFINED; no counterpart in the
+ * reference, whose solve is fp64 throughout): the steps that ran and max |dq| (e) of the last
+ * correction applied.  0 and 0.0 for the other precisions and after an LU fallback.
+ */
+int rp_fit_refine_stats(const rp_fit *fit, int32_t *steps, double *last_correction);
 
 /*
  * Reload a fit with another system and options (ChargeCalculator.__init__ on a new

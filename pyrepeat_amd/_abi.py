@@ -26,7 +26,7 @@ RP_EAGAIN = -8
 
 RP_F64, RP_F32 = 0, 1
 RP_ORDER_CUBE, RP_ORDER_FLAT = 0, 1
-RP_PREC_FP64, RP_PREC_OZAKI = 0, 1
+RP_PREC_FP64, RP_PREC_OZAKI, RP_PREC_OZAKI_REFINED = 0, 1, 2
 
 EXPORTED = (
     "rp_ctx_create", "rp_ctx_destroy", "rp_last_error", "rp_ctx_hbm_in_use",
@@ -35,9 +35,9 @@ EXPORTED = (
     "rp_fit_phi", "rp_fit_timings", "rp_ctx_profile", "rp_ctx_profile_kinds", "rp_ctx_kernel_stats", "rp_fit_refit",
     "rp_fit_reset", "rp_server_create", "rp_server_destroy", "rp_server_error", "rp_server_now",
     "rp_request_bytes", "rp_submit", "rp_poll", "rp_server_stats_get",
-    "rp_cube_read_header", "rp_cube_read", "rp_parse_values", "rp_fit_qeq",
+    "rp_cube_read_header", "rp_cube_read", "rp_parse_values", "rp_fit_qeq", "rp_fit_refine_stats",
 )
-KERNEL_KINDS = {"kspace": 0, "gram": 1, "filter": 2, "solve": 3, "phase": 4, "slice": 5, "reduce": 6}
+KERNEL_KINDS = {"kspace": 0, "gram": 1, "filter": 2, "solve": 3, "phase": 4, "slice": 5, "reduce": 6, "refine": 7}
 
 
 class RepeatHipError(RuntimeError):
@@ -58,7 +58,8 @@ class RpSystem(ctypes.Structure):
 
 class RpOptions(ctypes.Structure):
     _fields_ = [("fit_type", ctypes.c_int32), ("precision", ctypes.c_int32), ("vdw_factor", ctypes.c_double),
-                ("vdw_max", ctypes.c_double), ("r_cutoff", ctypes.c_double)]
+                ("vdw_max", ctypes.c_double), ("r_cutoff", ctypes.c_double), ("refine_steps", ctypes.c_int32),
+                ("reserved", ctypes.c_int32)]
 
 
 class RpFitInfo(ctypes.Structure):
@@ -130,6 +131,7 @@ def load(path: str | None = None):
     L.rp_fit_phi_matvec.argtypes = [vp, dp, dp]
     L.rp_fit_phi.argtypes = [vp, ctypes.c_int64, dp, dp]
     L.rp_fit_timings.argtypes = [vp, dp]
+    L.rp_fit_refine_stats.argtypes = [vp, ctypes.POINTER(ctypes.c_int32), dp]
     L.rp_ctx_profile.argtypes = [vp, ctypes.c_int]
     L.rp_ctx_profile_kinds.argtypes = [vp, ctypes.c_uint32]
     L.rp_ctx_kernel_stats.argtypes = [vp, ctypes.c_int, dp, i64p]
@@ -279,19 +281,20 @@ def make_system(n_grid, axis, atomic_numbers, atom_xyz, esp, esp_order=RP_ORDER_
     return sysd, (Z, xyz, esp)
 
 
-def make_options(fit_type=1, precision=RP_PREC_FP64, vdw_factor=1.0, vdw_max=1000.0, r_cutoff=20.0):
-    return RpOptions(int(fit_type), int(precision), float(vdw_factor), float(vdw_max), float(r_cutoff))
+def make_options(fit_type=1, precision=RP_PREC_FP64, vdw_factor=1.0, vdw_max=1000.0, r_cutoff=20.0, refine_steps=0):
+    return RpOptions(int(fit_type), int(precision), float(vdw_factor), float(vdw_max), float(r_cutoff),
+                     int(refine_steps), 0)
 
 
 class Fit:
     """Device-resident fit (filter done at construction)."""
 
     def __init__(self, ctx: Context, n_grid, axis, atomic_numbers, atom_xyz, esp, esp_order=RP_ORDER_CUBE,
-                 fit_type=1, precision=RP_PREC_FP64, vdw_factor=1.0, vdw_max=1000.0, r_cutoff=20.0):
+                 fit_type=1, precision=RP_PREC_FP64, vdw_factor=1.0, vdw_max=1000.0, r_cutoff=20.0, refine_steps=0):
         L = load()
         self.ctx = ctx
         sysd, self._arrays = make_system(n_grid, axis, atomic_numbers, atom_xyz, esp, esp_order)
-        opt = make_options(fit_type, precision, vdw_factor, vdw_max, r_cutoff)
+        opt = make_options(fit_type, precision, vdw_factor, vdw_max, r_cutoff, refine_steps)
         h = ctypes.c_void_p()
         check(L.rp_fit_create(ctx.handle, ctypes.byref(sysd), ctypes.byref(opt), ctypes.byref(h)), ctx.handle)
         self.handle = h
@@ -300,10 +303,10 @@ class Fit:
         self.info = self._info()
 
     def reset(self, n_grid, axis, atomic_numbers, atom_xyz, esp, esp_order=RP_ORDER_CUBE, fit_type=1,
-              precision=RP_PREC_FP64, vdw_factor=1.0, vdw_max=1000.0, r_cutoff=20.0):
+              precision=RP_PREC_FP64, vdw_factor=1.0, vdw_max=1000.0, r_cutoff=20.0, refine_steps=0):
         """Reload this fit with another system, reusing its device buffers."""
         sysd, arrays = make_system(n_grid, axis, atomic_numbers, atom_xyz, esp, esp_order)
-        opt = make_options(fit_type, precision, vdw_factor, vdw_max, r_cutoff)
+        opt = make_options(fit_type, precision, vdw_factor, vdw_max, r_cutoff, refine_steps)
         check(load().rp_fit_reset(self.handle, ctypes.byref(sysd), ctypes.byref(opt)), self.ctx.handle)
         self._arrays = arrays
         self.n_atoms = int(sysd.n_atoms)
@@ -405,6 +408,13 @@ class Fit:
         J = np.empty((self.n_atoms, self.n_atoms)) if want_j else None
         check(load().rp_fit_qeq(self.handle, ptr(chi), ptr(js), float(q_tot), ptr(q), ptr(J)), self.ctx.handle)
         return (q, J) if want_j else q
+
+    def refine_stats(self):
+        """Refinement of the last solve (split form with refinement): steps run and max |dq| (e) of
+        the last correction."""
+        steps, last = ctypes.c_int32(0), ctypes.c_double(0.0)
+        check(load().rp_fit_refine_stats(self.handle, ctypes.byref(steps), ctypes.byref(last)), self.ctx.handle)
+        return int(steps.value), float(last.value)
 
     def timings(self):
         t = np.zeros(3)

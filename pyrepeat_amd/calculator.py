@@ -17,6 +17,9 @@ Same constructor arguments, method names and results as ``repeat.py``'s
 lattice axes in fp64, the real-space sum in fp64 (erfc as one polynomial on
 the argument range the cutoff allows), the Gram of the basis from int8 fixed-point slices on MFMA with exact int32
 accumulation, fp64 across chunks and in the solve.
+``precision="refined"`` is the split form with 3 slices followed by iterative
+refinement of the solve against the fp64 basis (kept in HBM): its charges
+converge to those of the fp64 normal equations.
 Every computation runs in ``librepeat_hip.so``; nothing falls back to the CPU.
 """
 from __future__ import annotations
@@ -30,7 +33,7 @@ from .cube import read_data, v_pot_view
 from .params import constraint_arrays, read_qeq_params, read_resp_params, read_symmetry
 from .tables import ATOM_SYMBOLS, DEFAULT_QEQ_PARAMS
 
-PRECISIONS = {"fp64": _abi.RP_PREC_FP64, "split": _abi.RP_PREC_OZAKI}
+PRECISIONS = {"fp64": _abi.RP_PREC_FP64, "split": _abi.RP_PREC_OZAKI, "refined": _abi.RP_PREC_OZAKI_REFINED}
 
 
 class ChargeCalculator:

@@ -207,6 +207,7 @@ int run_accumulate(rp_fit *fit) {
     if (!fit->ev_acc0) RP_HIP(ctx, hipEventCreate(&fit->ev_acc0));
     if (!fit->ev_acc1) RP_HIP(ctx, hipEventCreate(&fit->ev_acc1));
     fit->accumulated = false;
+    fit->phi_kept = false;
     fit->acc_timing_pending = false;
     RP_HIP(ctx, hipEventRecord(fit->ev_acc0, st));
     k_zero<<<ceil_div(g2, 256), 256, 0, st>>>(fit->d_gram.as<double>(), g2);
@@ -253,7 +254,8 @@ int load_system(rp_fit *fit, const rp_system *sys, const rp_options *opt) {
         return ctx->fail(RP_EINVAL, "rp_fit_create: system needs atoms, positions and ESP");
     if (sys->esp_dtype != RP_F64 && sys->esp_dtype != RP_F32) return ctx->fail(RP_EINVAL, "bad esp_dtype");
     if (sys->esp_order != RP_ORDER_CUBE && sys->esp_order != RP_ORDER_FLAT) return ctx->fail(RP_EINVAL, "bad esp_order");
-    if (opt->precision != RP_PREC_FP64 && opt->precision != RP_PREC_OZAKI) return ctx->fail(RP_EINVAL, "bad precision");
+    if (opt->precision != RP_PREC_FP64 && opt->precision != RP_PREC_OZAKI && opt->precision != RP_PREC_OZAKI_REFINED)
+        return ctx->fail(RP_EINVAL, "bad precision");
     fit->opt = *opt;
     fit->n_atoms = sys->n_atoms;
     fit->Z.assign(sys->atomic_number, sys->atomic_number + sys->n_atoms);
@@ -264,6 +266,9 @@ int load_system(rp_fit *fit, const rp_system *sys, const rp_options *opt) {
     fit->shifts_pruned.clear();
     fit->accumulated = false;
     fit->lu_fallback = false;
+    fit->phi_kept = false;
+    fit->refine_steps_done = 0;
+    fit->refine_last_dq = 0.0;
     fit->na = 0;
     if (fit->dev) fit->dev->ready = false;
     RP_CHECK(build_setup(sys, opt, fit->setup, ctx->err));
@@ -507,6 +512,13 @@ int rp_fit_phi(rp_fit *fit, int64_t m, const double *grid_xyz, double *phi_out) 
     if (!fit || m < 0 || (m > 0 && (!grid_xyz || !phi_out))) return RP_EINVAL;
     RP_HIP(fit->ctx, hipSetDevice(fit->ctx->device));
     return run_phi_points(fit, m, grid_xyz, phi_out);
+}
+
+int rp_fit_refine_stats(const rp_fit *fit, int32_t *steps, double *last_correction) {
+    if (!fit || !steps || !last_correction) return RP_EINVAL;
+    *steps = fit->refine_steps_done;
+    *last_correction = fit->refine_last_dq;
+    return RP_OK;
 }
 
 int rp_fit_timings(const rp_fit *fit, double *t3) {

@@ -140,9 +140,9 @@ int sep_prepare(rp_fit *fit, FitDevice &dv, int64_t rows, SepPlan &plan);
 // [NP][rows] fp64, with the column maxima of |Phi| (bit patterns) into colmax.
 void sep_phi_chunk(rp_fit *fit, FitDevice &dv, const SepPlan &plan, int chunk, int64_t r0, int64_t valid,
                    int64_t rows, double *phi, unsigned long long *colmax, hipStream_t st);
-// Column-major Phi chunk -> int8 slices [4][NP][rows] with per-column power-of-two scales.
+// Column-major Phi chunk -> int8 slices [P][NP][rows] (P = 3 or 4) with per-column power-of-two scales.
 void sep_slice_chunk(const double *phi, int NP, int64_t rows, int64_t valid, const unsigned long long *colmax,
-                     int8_t *slices, double *scale_out, hipStream_t st);
+                     int8_t *slices, double *scale_out, int P, hipStream_t st);
 __global__ void k_symmetrize(double *gram, int NP, int tile);
 
 }  // namespace rp

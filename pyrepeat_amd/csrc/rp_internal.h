@@ -20,7 +20,7 @@ void set_global_error(const std::string &msg);
 struct Ctx;
 
 // Kernel classes timed with HIP events when profiling is on (rp_ctx_profile).
-enum { KT_KSPACE = 0, KT_GRAM = 1, KT_FILTER = 2, KT_SOLVE = 3, KT_PHASE = 4, KT_SLICE = 5, KT_REDUCE = 6, KT_N = 7 };
+enum { KT_KSPACE = 0, KT_GRAM = 1, KT_FILTER = 2, KT_SOLVE = 3, KT_PHASE = 4, KT_SLICE = 5, KT_REDUCE = 6, KT_REFINE = 7, KT_N = 8 };
 
 // Device allocation charged against the owning context's HBM budget.
 struct DBuf {
@@ -159,7 +159,8 @@ inline ChunkPlan plan_chunks(int64_t M, int N, int K, int precision) {
             c.splits *= 2;
     } else {
         c.KP = 0;  // the factorized reciprocal sum needs no k-space operand
-        c.slices = 4;
+        // refined: 3 slices (the refinement removes their error); otherwise 4
+        c.slices = precision == RP_PREC_OZAKI_REFINED ? 3 : 4;
         int64_t rows = std::min<int64_t>(65536, round_up(M, 512));
         rows = std::max<int64_t>(512, rows);
         // bound the chunk buffers (Phi fp64, Phi slices) to about 2 GiB
@@ -276,7 +277,13 @@ struct rp_fit {
     }
     size_t h_stage_bytes = 0;
     int split_tiles_np = 0;         // NP the split Gram's tile list in ws[9] was built for
-    uint32_t bs_base = 0, bs_epoch = 0;   // backward substitution: tickets taken so far, solve epoch
+    uint32_t bs_base = 0, bs_epoch = 0;   // triangular solves: tickets taken so far, launch epoch
+    // refinement (RP_PREC_OZAKI_REFINED): the accumulate keeps every chunk of the fp64 basis in ws[7]
+    // ([chunk][NP][phi_rows] doubles); phi_kept says it matches the fit's current system
+    bool phi_kept = false;
+    int64_t phi_rows = 0;
+    int32_t refine_steps_done = 0;
+    double refine_last_dq = 0.0;
     rp::FitDevice *dev = nullptr;   // persistent device copy of the Ewald setup
     ~rp_fit();
 };

@@ -526,9 +526,12 @@ void sep_phi_chunk(rp_fit *fit, FitDevice &dv, const SepPlan &plan, int chunk, i
 }
 
 void sep_slice_chunk(const double *phi, int NP, int64_t rows, int64_t valid, const unsigned long long *colmax,
-                     int8_t *slices, double *scale_out, hipStream_t st) {
+                     int8_t *slices, double *scale_out, int P, hipStream_t st) {
     dim3 g((unsigned)ceil_div(rows, 256 * 16), (unsigned)NP);
-    k_slice_cols<4><<<g, 256, 0, st>>>(phi, rows, valid, colmax, slices, scale_out);
+    if (P == 3)
+        k_slice_cols<3><<<g, 256, 0, st>>>(phi, rows, valid, colmax, slices, scale_out);
+    else
+        k_slice_cols<4><<<g, 256, 0, st>>>(phi, rows, valid, colmax, slices, scale_out);
 }
 
 }  // namespace rp

@@ -71,8 +71,9 @@ __device__ __forceinline__ void gram_mainloop_dma(const GramOperands &op, int64_
                                                   ivec16 (&acc)[2 * P - 1], int8_t *lds) {
     constexpr int WAVES = NTHR / 64;
     constexpr int CHUNKS = P * (TM + TN) / 16;
-    constexpr int PER_WAVE = CHUNKS / WAVES;
-    static_assert(CHUNKS % WAVES == 0, "load chunks per wave");
+    // load instructions per wave and stage: PER_WAVE_HI for waves below CHUNKS % WAVES, else PER_WAVE_LO
+    constexpr int PER_WAVE_LO = CHUNKS / WAVES;
+    constexpr int PER_WAVE_HI = (CHUNKS + WAVES - 1) / WAVES;
     constexpr size_t STAGE = gram_stage_bytes<P>();
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
     const int wm = wave >> 2, wn = wave & 3;
@@ -87,8 +88,9 @@ __device__ __forceinline__ void gram_mainloop_dma(const GramOperands &op, int64_
         int8_t *buf = lds + (size_t)(kt % NSTAGE) * STAGE;
         const int kbase = k0 + kt * TK;
 #pragma unroll
-        for (int u = 0; u < PER_WAVE; ++u) {
+        for (int u = 0; u < PER_WAVE_HI; ++u) {
             const int ii = wave + WAVES * u;   // chunk of this load instruction (wave-uniform)
+            if (ii >= CHUNKS) break;
             int p, row;
             int64_t col0;
             int8_t *dst;
@@ -117,8 +119,12 @@ __device__ __forceinline__ void gram_mainloop_dma(const GramOperands &op, int64_
     for (int kt = 0; kt < nk; ++kt) {
         // this wave's loads of stage kt are done when at most the next stage's are outstanding;
         // the barrier makes every wave's visible and frees the buffer the next issue overwrites
-        if (kt + 1 < nk) wait_vmcnt<PER_WAVE>();
-        else wait_vmcnt<0>();
+        if (kt + 1 < nk) {
+            if (wave < CHUNKS % WAVES) wait_vmcnt<PER_WAVE_HI>();
+            else wait_vmcnt<PER_WAVE_LO>();
+        } else {
+            wait_vmcnt<0>();
+        }
         lds_barrier();
         if (kt + 2 < nk) issue(kt + 2);
         const int8_t *As = lds + (size_t)(kt % NSTAGE) * STAGE;
@@ -222,24 +228,29 @@ __global__ void k_zero_u64(unsigned long long *p, int n) {
 
 }  // namespace
 
-int accumulate_split(rp_fit *fit, FitDevice &dv) {
+namespace {
+
+// P slices per element; keep_phi: every chunk of the fp64 basis stays in ws[7] ([chunk][NP][rows]) for
+// the refinement of the solve, otherwise one chunk buffer (ws[2]) is reused.
+template <int P>
+int accumulate_split_p(rp_fit *fit, FitDevice &dv, bool keep_phi) {
     Ctx *ctx = fit->ctx;
     hipStream_t st = ctx->stream;
     const int K = dv.K, NP = dv.NP, N = fit->n_atoms;
     const int64_t M = fit->n_keep;
-    const ChunkPlan cp = plan_chunks(M, N, K, RP_PREC_OZAKI);
+    const ChunkPlan cp = plan_chunks(M, N, K, fit->opt.precision);
     const int64_t rows = cp.rows;
-    constexpr int P = 4;
     if (cp.NP != NP || cp.slices != P || TM != 64 || TN != 128 || rows % 512 != 0)
         return ctx->fail(RP_EINVAL, "split path: chunk plan does not match the kernel tiling");
-    DBuf &d_phi = fit->ws[2], &d_part = fit->ws[3], &d_slices = fit->ws[4], &d_colmax = fit->ws[5],
+    DBuf &d_phi = fit->ws[keep_phi ? 7 : 2], &d_part = fit->ws[3], &d_slices = fit->ws[4], &d_colmax = fit->ws[5],
          &d_scale = fit->ws[6], &d_tiles = fit->ws[9];
+    const int64_t n_chunks = ceil_div(M, rows);
     SepPlan plan;
     {
         KTimer kt(ctx, KT_PHASE);
         RP_CHECK(sep_prepare(fit, dv, rows, plan));
     }
-    RP_CHECK(d_phi.grow(ctx, sizeof(double) * NP * rows));
+    RP_CHECK(d_phi.grow(ctx, sizeof(double) * NP * rows * (keep_phi ? n_chunks : 1)));
     RP_CHECK(d_slices.grow(ctx, (size_t)P * NP * rows));
     RP_CHECK(d_colmax.grow(ctx, sizeof(unsigned long long) * NP));
     RP_CHECK(d_scale.grow(ctx, sizeof(double) * NP));
@@ -277,15 +288,16 @@ int accumulate_split(rp_fit *fit, FitDevice &dv) {
     int chunk = 0;
     for (int64_t r0 = 0; r0 < M; r0 += rows, ++chunk) {
         const int64_t valid = std::min<int64_t>(rows, M - r0);
+        double *phi = d_phi.as<double>() + (keep_phi ? (int64_t)chunk * NP * rows : 0);
         k_zero_u64<<<ceil_div(NP, 256), 256, 0, st>>>(d_colmax.as<unsigned long long>(), NP);
         {
             KTimer kt(ctx, KT_KSPACE);
-            sep_phi_chunk(fit, dv, plan, chunk, r0, valid, rows, d_phi.as<double>(), d_colmax.as<unsigned long long>(), st);
+            sep_phi_chunk(fit, dv, plan, chunk, r0, valid, rows, phi, d_colmax.as<unsigned long long>(), st);
         }
         {
             KTimer kts(ctx, KT_SLICE);
-            sep_slice_chunk(d_phi.as<double>(), NP, rows, valid, d_colmax.as<unsigned long long>(),
-                            d_slices.as<int8_t>(), d_scale.as<double>(), st);
+            sep_slice_chunk(phi, NP, rows, valid, d_colmax.as<unsigned long long>(), d_slices.as<int8_t>(),
+                            d_scale.as<double>(), P, st);
         }
         KTimer ktg(ctx, KT_GRAM);
         ge.accumulate = chunk > 0;
@@ -297,7 +309,16 @@ int accumulate_split(rp_fit *fit, FitDevice &dv) {
                                               fit->d_gram.as<double>(), NP);
     }
     RP_HIP(ctx, hipGetLastError());
+    fit->phi_kept = keep_phi;
+    fit->phi_rows = rows;
     return RP_OK;
+}
+
+}  // namespace
+
+int accumulate_split(rp_fit *fit, FitDevice &dv) {
+    if (fit->opt.precision == RP_PREC_OZAKI_REFINED) return accumulate_split_p<3>(fit, dv, true);
+    return accumulate_split_p<4>(fit, dv, false);
 }
 
 }  // namespace rp

@@ -120,7 +120,7 @@ uint64_t fit_bytes_bound(const Setup &s, int n_atoms, int esp_dtype, int precisi
     at(23, 8 * M);
     uint64_t sep = 0;
     if (precision != RP_PREC_FP64) {
-        const ChunkPlan c = plan_chunks(M, n_atoms, (int)K, RP_PREC_OZAKI);
+        const ChunkPlan c = plan_chunks(M, n_atoms, (int)K, precision);
         at(2, 8 * (uint64_t)c.NP * c.rows);
         at(3, 8 * split_cap(c.rows, c.n_tiles) * c.n_tiles * 64 * 128);
         at(4, (uint64_t)c.slices * c.NP * c.rows);
@@ -132,13 +132,23 @@ uint64_t fit_bytes_bound(const Setup &s, int n_atoms, int esp_dtype, int precisi
         const uint64_t nb = (uint64_t)ceil_div(M, 256), n_chunks = (uint64_t)ceil_div(M, c.rows);
         sep += 4 * 8 * 13 * 13 + (K > 0 ? 16 * 8 * 13 * (uint64_t)s.nz * N : 0);
         sep += 4 * nb + 8 * (nb + 1) + 4 * (M + 1) + 4 * M + 8 * n_chunks + 16 * 7;
+        if (precision == RP_PREC_OZAKI_REFINED) {
+            // every basis chunk kept for the refinement (fewer than M + rows rows in all), the
+            // residual e over the kept points and its block sums
+            at(7, 8 * (uint64_t)c.NP * (M + (uint64_t)c.rows));
+            at(8, 8 * (M + 2 + (uint64_t)ceil_div(M, 512)));
+        }
     }
     // bordered solve
     const uint64_t nr = n_reduced, n = nr + 1, ld = (uint64_t)round_up((int64_t)n + 1, 4);
     const uint64_t nsrc = std::max(n_src, 1);
     at(12, 8 * n * ld);
-    // packed constraint arrays: ints (red_ptr, red_src, expand_src), then doubles
-    at(13, (uint64_t)round_up((int64_t)(round_up((int64_t)(4 * ((nr + 1) + nsrc + N)), 8) + 8 * (nsrc + 3 * nr)), 16));
+    // packed constraint arrays: ints (red_ptr, red_src, expand_src, inv_ptr, inv_set), then doubles
+    // (red_mult, weights, diag_add, rhs_add, inv_mult)
+    at(13, (uint64_t)round_up((int64_t)(round_up((int64_t)(4 * ((nr + 1) + 2 * nsrc + 2 * N + 1)), 8) +
+                                        8 * (2 * nsrc + 3 * nr)), 16));
+    // refinement vectors and state
+    at(10, 8 * (uint64_t)round_up((int64_t)(2 * N + 3 * nr + 2), 2) + 16);
     at(19, 4 * n);
     at(20, 8 * n * ld);
     for (int i = 21; i <= 23; ++i) at(i, 8 * nr);

@@ -37,18 +37,29 @@ namespace {
 
 constexpr int NB = 32;
 
+// Refinement state of a solve (device): go = the next step runs, steps run so far, max |dq| of the last
+// correction.
+struct RefineState {
+    int go;
+    int steps;
+    double last_dq;
+};
+
 __global__ __launch_bounds__(256) void k_assemble(const double *__restrict__ gram, int NP, int N,
                                                   const int *__restrict__ red_ptr, const int *__restrict__ red_src,
                                                   const double *__restrict__ red_mult, const double *__restrict__ w,
                                                   const double *__restrict__ diag_add, const double *__restrict__ rhs_add,
                                                   double q_tot, int nr, double *__restrict__ K, double *__restrict__ K2,
                                                   int ld, double *__restrict__ wc, double *__restrict__ bc,
-                                                  int *__restrict__ cinfo) {
+                                                  int *__restrict__ cinfo, RefineState *__restrict__ rs) {
     const int n = nr + 1;
     const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (e == 0) {
         cinfo[0] = 0;
         cinfo[1] = 0;
+        rs->go = 1;
+        rs->steps = 0;
+        rs->last_dq = 0.0;
     }
     if (e >= (int64_t)n * (n + 1)) return;
     const int I = (int)(e / (n + 1)), J = (int)(e % (n + 1));
@@ -833,7 +844,9 @@ __global__ __launch_bounds__(256) void k_backsub(const double *__restrict__ A, i
                                                  const double *__restrict__ zb, double q_tot, double *__restrict__ x,
                                                  unsigned *__restrict__ sync, unsigned base, unsigned epoch,
                                                  unsigned spin_limit, int *__restrict__ status,
-                                                 const int *__restrict__ cinfo) {
+                                                 const int *__restrict__ cinfo,
+                                                 const double *__restrict__ q_tot_dev = nullptr,
+                                                 const int *__restrict__ go = nullptr) {
     __shared__ double Dk[CB][CB + 1];
     __shared__ double uk[CB], yk[CB];
     __shared__ double red[2][4];
@@ -847,6 +860,8 @@ __global__ __launch_bounds__(256) void k_backsub(const double *__restrict__ A, i
     }
     __syncthreads();
     if (*cinfo != 0) return;   // a pivot failed: the host reruns the system through the LU
+    if (go && (*go == 0 || *status != 0)) return;   // a refinement step that is not needed
+    if (q_tot_dev) q_tot = *q_tot_dev;
     const int nblk = (n + CB - 1) / CB;
     if (sh_ticket >= (unsigned)nblk) {   // tickets out of step with the host's count: give up, LU answers
         if (t == 0) atomicExch(status, 1);
@@ -929,6 +944,249 @@ __global__ void k_fill_u32(unsigned *p, int n, unsigned v) {
     if (i < n) p[i] = v;
 }
 
+// ---------------------------------------------------------------- iterative refinement (split form)
+//
+// The split Gram is the Gram of a perturbed basis (int8 slices of Phi).  A refinement step forms the
+// residual of the fp64 normal equations at the current solution x (reduced charges, multiplier x[nr])
+// from the fp64 basis kept by the accumulate:
+//   u = P x (expanded with the merge multiplicities),  e = V - Phi u over the kept points,
+//   r = Phi_c^T (V - Phi_c u) = Phi^T e - phibar sum(e)      (Phi_c the centred basis, phibar its mean),
+//   r1 = P^T r + rhs_add - diag_add x - w lambda,  r2 = q_tot - w . x,
+// then solves the bordered system of the split Gram for the correction with the same factor: a forward
+// substitution z_r = L^-1 r1 and k_backsub with (z_w, z_r, r2), which forms the multiplier's correction
+// (z_w . z_r - r2) / (z_w . z_w) and the charges' L^-T (z_r - dlambda z_w).
+//
+// State (RefineState, device): go = the step runs; a step runs while the previous correction exceeded
+// the stop threshold, the pivots were positive and no wait gave up.  Every kernel of a skipped step
+// returns at once.
+__device__ __forceinline__ bool refine_active(const int *cinfo, const RefineState *rs) {
+    return cinfo[0] == 0 && cinfo[1] == 0 && rs->go != 0;
+}
+
+// u[b] = sum over the merge sets holding atom b of multiplicity * x[set]
+__global__ __launch_bounds__(256) void k_ref_expand(const double *__restrict__ x, const int *__restrict__ inv_ptr,
+                                                    const int *__restrict__ inv_set, const double *__restrict__ inv_mult,
+                                                    int N, double *__restrict__ u, const int *__restrict__ cinfo,
+                                                    const RefineState *__restrict__ rs) {
+    if (!refine_active(cinfo, rs)) return;
+    const int b = blockIdx.x * 256 + threadIdx.x;
+    if (b >= N) return;
+    double s = 0.0;
+    for (int p = inv_ptr[b]; p < inv_ptr[b + 1]; ++p) s = fma(inv_mult[p], x[inv_set[p]], s);
+    u[b] = s;
+}
+
+// Block sum of one value per thread (256 threads), returned to every thread.
+__device__ __forceinline__ double block_sum_256(double v, double *red) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    __syncthreads();
+    if (lane == 0) red[wave] = v;
+    __syncthreads();
+    return (red[0] + red[1]) + (red[2] + red[3]);
+}
+
+// e = V - Phi u over the kept points, two rows per thread (the basis chunk c holds global rows
+// [c rows, (c + 1) rows), column-major [NP][rows]); per-block sums of e into esum.
+__global__ __launch_bounds__(256) void k_ref_rows(const double *__restrict__ phi, int64_t rows, int64_t plane, int N,
+                                                  int64_t M, const double *__restrict__ v, const double *__restrict__ u,
+                                                  double *__restrict__ e, double *__restrict__ esum,
+                                                  const int *__restrict__ cinfo, const RefineState *__restrict__ rs) {
+    __shared__ double red[4];
+    if (!refine_active(cinfo, rs)) return;
+    const int64_t R = 2 * ((int64_t)blockIdx.x * 256 + threadIdx.x);
+    double e0 = 0.0, e1 = 0.0;
+    if (R < M) {
+        const int64_t c = R / rows, r = R - c * rows;   // rows is even: R and R + 1 share the chunk
+        const double *col = phi + c * plane + r;
+        double a0[4] = {0.0, 0.0, 0.0, 0.0}, a1[4] = {0.0, 0.0, 0.0, 0.0};
+        int j = 0;
+        for (; j + 4 <= N; j += 4) {
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+                const double2 p = *reinterpret_cast<const double2 *>(col + (int64_t)(j + t) * rows);
+                a0[t] = fma(p.x, u[j + t], a0[t]);
+                a1[t] = fma(p.y, u[j + t], a1[t]);
+            }
+        }
+        for (; j < N; ++j) {
+            const double2 p = *reinterpret_cast<const double2 *>(col + (int64_t)j * rows);
+            a0[0] = fma(p.x, u[j], a0[0]);
+            a1[0] = fma(p.y, u[j], a1[0]);
+        }
+        e0 = v[R] - ((a0[0] + a0[1]) + (a0[2] + a0[3]));
+        e[R] = e0;
+        if (R + 1 < M) {   // the second row of the last pair may lie past the kept points
+            e1 = v[R + 1] - ((a1[0] + a1[1]) + (a1[2] + a1[3]));
+            e[R + 1] = e1;
+        }
+    }
+    const double sum = block_sum_256(e0 + e1, red);
+    if (threadIdx.x == 0) esum[blockIdx.x] = sum;
+}
+
+// One work-group per atom column j: g_j = sum_R Phi[j][R] e_R, s_j = sum_R Phi[j][R] over the kept
+// points, and r_j = g_j - s_j E / M with E = sum(e) (the block sums of k_ref_rows, in order).
+__global__ __launch_bounds__(256) void k_ref_cols(const double *__restrict__ phi, int64_t rows, int64_t plane,
+                                                  int64_t M, const double *__restrict__ e,
+                                                  const double *__restrict__ esum, int n_esum, double *__restrict__ r,
+                                                  const int *__restrict__ cinfo, const RefineState *__restrict__ rs) {
+    __shared__ double red[4];
+    if (!refine_active(cinfo, rs)) return;
+    const int j = blockIdx.x, t = threadIdx.x;
+    double g0 = 0.0, g1 = 0.0, s0 = 0.0, s1 = 0.0;
+    for (int64_t c0 = 0, c = 0; c0 < M; c0 += rows, ++c) {
+        const int64_t valid = min(rows, M - c0);
+        const double *col = phi + c * plane + (int64_t)j * rows;
+        const double *ec = e + c0;
+        int64_t q = 2 * t;
+        for (; q + 1 < valid; q += 512) {
+            const double2 p = *reinterpret_cast<const double2 *>(col + q);
+            g0 = fma(p.x, ec[q], g0);
+            g1 = fma(p.y, ec[q + 1], g1);
+            s0 += p.x;
+            s1 += p.y;
+        }
+        if (q < valid) {   // an odd row count leaves one row
+            g0 = fma(col[q], ec[q], g0);
+            s0 += col[q];
+        }
+    }
+    double E = 0.0;
+    for (int i = t; i < n_esum; i += 256) E += esum[i];
+    const double G = block_sum_256(g0 + g1, red);
+    const double S = block_sum_256(s0 + s1, red);
+    const double Et = block_sum_256(E, red);
+    if (t == 0) r[j] = G - S * (Et / (double)M);
+}
+
+// r1 = P^T r + rhs_add - diag_add x - w lambda (into r1), r2 = q_tot - w . x; one work-group.
+__global__ __launch_bounds__(256) void k_ref_rhs(const double *__restrict__ r, const int *__restrict__ red_ptr,
+                                                 const int *__restrict__ red_src, const double *__restrict__ red_mult,
+                                                 const double *__restrict__ w, const double *__restrict__ diag_add,
+                                                 const double *__restrict__ rhs_add, double q_tot, int nr,
+                                                 const double *__restrict__ x, double *__restrict__ r1,
+                                                 double *__restrict__ r2, const int *__restrict__ cinfo,
+                                                 const RefineState *__restrict__ rs) {
+    __shared__ double red[4];
+    if (!refine_active(cinfo, rs)) return;
+    const double lambda = x[nr];
+    double pw = 0.0;
+    for (int I = threadIdx.x; I < nr; I += 256) {
+        double s = 0.0;
+        for (int p = red_ptr[I]; p < red_ptr[I + 1]; ++p) s = fma(red_mult[p], r[red_src[p]], s);
+        if (rhs_add) s += rhs_add[I];
+        if (diag_add) s -= diag_add[I] * x[I];
+        r1[I] = s - w[I] * lambda;
+        pw = fma(w[I], x[I], pw);
+    }
+    const double wx = block_sum_256(pw, red);
+    if (threadIdx.x == 0) *r2 = q_tot - wx;
+}
+
+// z = L^-1 r1 in one launch: the order of k_backsub reversed.  Ticket i owns diagonal block i and waits
+// for the blocks before it, oldest first; L_{kb,m} (m < kb) is read from the transposed panels in the
+// upper triangle (row m0 + j, column j0 + i holds L[j0 + i][m0 + j]), loaded before each wait.
+// Work-group kb publishes z_kb = D_kb u_kb (D = the diagonal block's inverse) with a release fence and
+// its flag; a wait that sees no progress for spin_limit polls sets *status and the work-group leaves.
+__global__ __launch_bounds__(256) void k_fwdsub(const double *__restrict__ A, int ld, int n,
+                                                const double *__restrict__ Dinv, const double *__restrict__ r1,
+                                                double *__restrict__ z, unsigned *__restrict__ sync, unsigned base,
+                                                unsigned epoch, unsigned spin_limit, int *__restrict__ status,
+                                                const int *__restrict__ cinfo, const RefineState *__restrict__ rs) {
+    __shared__ double Dk[CB][CB + 1];
+    __shared__ double uk[CB], zm[CB];
+    __shared__ double part[4][CB];
+    __shared__ unsigned sh_ticket;
+    __shared__ int sh_abort;
+    const int t = threadIdx.x;
+    unsigned *counter = sync, *flags = sync + 1;
+    if (t == 0) {
+        sh_ticket = atomicAdd(counter, 1u) - base;
+        sh_abort = 0;
+    }
+    __syncthreads();
+    if (!refine_active(cinfo, rs)) return;
+    const int nblk = (n + CB - 1) / CB;
+    if (sh_ticket >= (unsigned)nblk) {
+        if (t == 0) atomicExch(status, 1);
+        return;
+    }
+    const int kb = (int)sh_ticket;
+    const int j0 = kb * CB, nb = min(CB, n - j0);
+    for (int q = t; q < CB * CB; q += 256) Dk[q / CB][q % CB] = Dinv[(int64_t)kb * CB * CB + q];
+    if (t < CB) uk[t] = t < nb ? r1[j0 + t] : 0.0;
+    // thread (i = t % 64: row of block kb, g = t / 64: 16 columns of block m)
+    const int i = t & 63, g = t >> 6;
+    const bool irow = i < nb;
+    for (int m = 0; m < kb; ++m) {
+        const int m0 = m * CB;   // blocks before kb are full
+        double lv[16];
+#pragma unroll
+        for (int q = 0; q < 16; ++q) lv[q] = irow ? A[(int64_t)(m0 + g * 16 + q) * ld + j0 + i] : 0.0;
+        if (t == 0) {
+            for (unsigned spins = 0;; ++spins) {
+                if (spins >= spin_limit) {
+                    sh_abort = 1;
+                    atomicExch(status, 1);
+                    break;
+                }
+                if (__hip_atomic_load(flags + m, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == epoch) {
+                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(2);
+            }
+        }
+        __syncthreads();
+        if (sh_abort) return;
+        if (t < CB) zm[t] = __hip_atomic_load(z + m0 + t, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __syncthreads();
+        double s = 0.0;
+#pragma unroll
+        for (int q = 0; q < 16; ++q) s = fma(lv[q], zm[g * 16 + q], s);
+        part[g][i] = s;
+        __syncthreads();
+        if (t < CB) uk[t] -= (part[0][t] + part[1][t]) + (part[2][t] + part[3][t]);
+        __syncthreads();
+    }
+    __syncthreads();
+    if (t < nb) {
+        double y = 0.0;
+        for (int q = 0; q <= t; ++q) y = fma(Dk[t][q], uk[q], y);
+        __hip_atomic_store(z + j0 + t, y, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    __threadfence();
+    __syncthreads();
+    if (t == 0) __hip_atomic_store(flags + kb, epoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// x += dx (charges and multiplier); the correction's max |dq| decides whether the next step runs.
+__global__ __launch_bounds__(256) void k_ref_update(double *__restrict__ x, const double *__restrict__ dx, int nr,
+                                                    double stop, const int *__restrict__ cinfo,
+                                                    RefineState *__restrict__ rs) {
+    __shared__ double red[4];
+    if (!refine_active(cinfo, rs)) return;
+    double mx = 0.0;
+    for (int I = threadIdx.x; I <= nr; I += 256) {
+        const double d = dx[I];
+        x[I] += d;
+        if (I < nr) mx = fmax(mx, fabs(d));
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) mx = fmax(mx, __shfl_xor(mx, o));
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0) red[wave] = mx;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const double m = fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
+        rs->steps += 1;
+        rs->last_dq = m;
+        rs->go = m > stop ? 1 : 0;
+    }
+}
+
 // Back substitution U x = y (y = column n of the factored augmented matrix); x -> out.
 __global__ __launch_bounds__(1024) void k_backsub(const double *__restrict__ A, int ld, int n, double *__restrict__ out) {
     extern __shared__ double y[];
@@ -955,13 +1213,19 @@ __global__ __launch_bounds__(256) void k_fetch_host(const uint4 *__restrict__ sr
     for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n16; i += (int64_t)gridDim.x * 256) dst[i] = src[i];
 }
 
-// The results into mapped pinned host memory: out[0] the pivot flag (-1: a backward-substitution wait
-// gave up, which the host also answers with the LU), out[1..N] the expanded charges,
-// out[1+N..1+N+n) the reduced solution and the multiplier; one synchronisation then returns them.
+// The results into mapped pinned host memory: out[0] the pivot flag (-1: a triangular-solve wait gave
+// up, which the host also answers with the LU), out[1..N] the expanded charges, out[1+N..1+N+n) the
+// reduced solution and the multiplier, then the refinement steps run and the last correction; one
+// synchronisation then returns them.
 __global__ __launch_bounds__(256) void k_publish(const double *__restrict__ x, const int *__restrict__ src, int N, int n,
-                                                 const int *__restrict__ cinfo, double *__restrict__ out) {
+                                                 const int *__restrict__ cinfo, const RefineState *__restrict__ rs,
+                                                 double *__restrict__ out) {
     const int i = blockIdx.x * 256 + threadIdx.x;
-    if (i == 0) out[0] = cinfo[0] != 0 ? (double)cinfo[0] : (cinfo[1] != 0 ? -1.0 : 0.0);
+    if (i == 0) {
+        out[0] = cinfo[0] != 0 ? (double)cinfo[0] : (cinfo[1] != 0 ? -1.0 : 0.0);
+        out[1 + N + n] = (double)rs->steps;
+        out[2 + N + n] = rs->last_dq;
+    }
     if (i < N) out[1 + i] = src[i] >= 0 ? x[src[i]] : 0.0;
     if (i < n) out[1 + N + i] = x[i];
 }
@@ -1010,13 +1274,14 @@ int run_solve(rp_fit *fit, const rp_constraints *con, double *q_out, double *sol
     const int n = nr + 1;
     const int ncols = n + 1;
     const int ld = (int)round_up(ncols, 4);
-    // the constraint arrays in one pinned block, one upload: ints (red_ptr, red_src, expand_src), then
-    // doubles (red_mult, weights, diag_add, rhs_add)
+    // the constraint arrays in one pinned block, one upload: ints (red_ptr, red_src, expand_src, and the
+    // merge sets by atom for the refinement: inv_ptr, inv_set), then doubles (red_mult, weights,
+    // diag_add, rhs_add, inv_mult)
     // The same mapped pinned block then receives the results (k_publish).
-    const size_t n_int = (size_t)(nr + 1) + nsrc + N;
+    const size_t n_int = (size_t)(nr + 1) + nsrc + N + (size_t)(N + 1) + nsrc;
     const size_t off_d = round_up((int64_t)(4 * n_int), 8);
-    const size_t stage = round_up((int64_t)(off_d + 8 * ((size_t)nsrc + 3 * (size_t)nr)), 16);
-    const size_t out_bytes = sizeof(double) * (1 + (size_t)N + n);
+    const size_t stage = round_up((int64_t)(off_d + 8 * (2 * (size_t)nsrc + 3 * (size_t)nr)), 16);
+    const size_t out_bytes = sizeof(double) * (3 + (size_t)N + n);
     if (fit->h_stage_bytes < stage + out_bytes) {
         if (fit->h_stage) (void)hipHostFree(fit->h_stage);
         fit->h_stage = nullptr;
@@ -1036,6 +1301,19 @@ int run_solve(rp_fit *fit, const rp_constraints *con, double *q_out, double *sol
         std::copy(con->weights, con->weights + nr, hd + nsrc);
         if (con->diag_add) std::copy(con->diag_add, con->diag_add + nr, hd + nsrc + nr);
         if (con->rhs_add) std::copy(con->rhs_add, con->rhs_add + nr, hd + nsrc + 2 * nr);
+        // the merge sets by atom (entries p of red_src in set order), for u = P x in the refinement
+        int *inv_ptr = hi + nr + 1 + nsrc + N, *inv_set = inv_ptr + N + 1;
+        double *inv_mult = hd + nsrc + 3 * nr;
+        std::fill(inv_ptr, inv_ptr + N + 1, 0);
+        for (int p = 0; p < nsrc; ++p) ++inv_ptr[con->red_src[p] + 1];
+        for (int a = 0; a < N; ++a) inv_ptr[a + 1] += inv_ptr[a];
+        std::vector<int> fill(inv_ptr, inv_ptr + N);
+        for (int I = 0; I < nr; ++I)
+            for (int p = con->red_ptr[I]; p < con->red_ptr[I + 1]; ++p) {
+                const int slot = fill[con->red_src[p]]++;
+                inv_set[slot] = I;
+                inv_mult[slot] = con->red_mult[p];
+            }
     }
     DBuf &d_K = fit->ws[12], &d_con = fit->ws[13], &d_piv = fit->ws[19];
     DBuf &d_info = fit->ws[24], &d_x = fit->ws[25], &d_q = fit->ws[27];
@@ -1050,14 +1328,11 @@ int run_solve(rp_fit *fit, const rp_constraints *con, double *q_out, double *sol
     double *h_out = reinterpret_cast<double *>(static_cast<char *>(fit->h_stage) + stage);
     double *h_out_dev = reinterpret_cast<double *>(static_cast<char *>(fit->h_stage_dev) + stage);
     const int *dc_ptr = d_con.as<int>(), *dc_src = dc_ptr + nr + 1, *dc_exp = dc_src + nsrc;
+    const int *dc_inv_ptr = dc_exp + N, *dc_inv_set = dc_inv_ptr + N + 1;
     const double *dc_d = reinterpret_cast<const double *>(static_cast<const char *>(d_con.p) + off_d);
     const double *dc_mult = dc_d, *dc_w = dc_d + nsrc, *dc_diag = dc_w + nr, *dc_rhs = dc_diag + nr;
+    const double *dc_inv_mult = dc_rhs + nr;
 
-    hipEvent_t e0, e1;
-    RP_HIP(ctx, hipEventCreate(&e0));
-    RP_HIP(ctx, hipEventCreate(&e1));
-    RP_HIP(ctx, hipEventRecord(e0, st));
-    KTimer *kts = new KTimer(ctx, KT_SOLVE);
     const int NPg = (int)round_up(N + 2, 128);
     DBuf &d_K2 = fit->ws[20], &d_wc = fit->ws[23];
     DBuf &d_cinfo = fit->ws[28], &d_bc = fit->ws[29], &d_dinv = fit->ws[30];
@@ -1066,12 +1341,27 @@ int run_solve(rp_fit *fit, const rp_constraints *con, double *q_out, double *sol
     RP_CHECK(d_wc.grow(ctx, sizeof(double) * nr));
     RP_CHECK(d_bc.grow(ctx, sizeof(double) * nr));
     RP_CHECK(d_dinv.grow(ctx, sizeof(double) * CB * CB * (size_t)ceil_div(nr, CB)));
+    // refinement vectors: u (N), r (N), r1 (nr), z (nr), dx (nr + 1), r2, then the state (16 bytes);
+    // e over the kept points and its block sums
+    const bool refine = fit->opt.precision == RP_PREC_OZAKI_REFINED && fit->phi_kept;
+    DBuf &d_rv = fit->ws[10], &d_e = fit->ws[8];
+    const size_t rv_doubles = 2 * (size_t)N + 3 * (size_t)nr + 2;
+    RP_CHECK(d_rv.grow(ctx, sizeof(double) * round_up((int64_t)rv_doubles, 2) + sizeof(RefineState)));
+    RefineState *d_rs = reinterpret_cast<RefineState *>(d_rv.as<double>() + round_up((int64_t)rv_doubles, 2));
+    const int n_esum = (int)ceil_div(fit->n_keep, 512);
+    if (refine) RP_CHECK(d_e.grow(ctx, sizeof(double) * ((size_t)fit->n_keep + 2 + (size_t)n_esum)));
+
+    hipEvent_t e0, e1;
+    RP_HIP(ctx, hipEventCreate(&e0));
+    RP_HIP(ctx, hipEventCreate(&e1));
+    RP_HIP(ctx, hipEventRecord(e0, st));
+    KTimer *kts = new KTimer(ctx, KT_SOLVE);
     // the bordered system, its copy for the LU fallback and the w and b columns of the Cholesky
     k_assemble<<<ceil_div((int64_t)n * ncols, 256), 256, 0, st>>>(fit->d_gram.as<double>(), NPg, N, dc_ptr, dc_src, dc_mult,
                                                                   dc_w, con->diag_add ? dc_diag : nullptr,
                                                                   con->rhs_add ? dc_rhs : nullptr, con->q_tot, nr,
                                                                   d_K.as<double>(), d_K2.as<double>(), ld, d_wc.as<double>(),
-                                                                  d_bc.as<double>(), d_cinfo.as<int>());
+                                                                  d_bc.as<double>(), d_cinfo.as<int>(), d_rs);
     // ---- Cholesky of A (leading nr x nr block) + Schur step
     // block 0 on its own; every later diagonal block is factored by the preceding trailing update
     k_potrf_inv<<<1, 256, 0, st>>>(d_K.as<double>(), ld, nr, 0, d_K2.as<double>(), d_cinfo.as<int>(), d_dinv.as<double>(),
@@ -1115,19 +1405,48 @@ int run_solve(rp_fit *fit, const rp_constraints *con, double *q_out, double *sol
         fit->bs_base = 0;
         fit->bs_epoch = 0;
     }
-    ++fit->bs_epoch;
     static const unsigned spin_limit = [] {
         const char *v = std::getenv("RP_BACKSUB_SPIN");
         return v ? (unsigned)std::strtoul(v, nullptr, 10) : BACKSUB_SPIN;
     }();
+    // every ticketed launch takes nblk tickets and a fresh epoch for its flags
     k_backsub<<<nblk, 256, 0, st>>>(d_K.as<double>(), ld, nr, d_dinv.as<double>(), d_wc.as<double>(),
                                     d_bc.as<double>(), con->q_tot, d_x.as<double>(), d_sync.as<unsigned>(),
-                                    fit->bs_base, fit->bs_epoch, spin_limit, d_cinfo.as<int>() + 1,
+                                    fit->bs_base, ++fit->bs_epoch, spin_limit, d_cinfo.as<int>() + 1,
                                     d_cinfo.as<int>());
     fit->bs_base += (unsigned)nblk;
-    k_publish<<<ceil_div(N + 1, 256), 256, 0, st>>>(d_x.as<double>(), dc_exp, N, n, d_cinfo.as<int>(), h_out_dev);
-    delete kts;   // records the end event before the synchronisation below
+    delete kts;
     kts = nullptr;
+    if (refine) {
+        // fp64 residual against the kept basis, correction through the same factor; a step whose
+        // predecessor's correction was at most RP_REFINE_STOP_E returns at once on the device
+        KTimer ktr(ctx, KT_REFINE);
+        const int64_t M = fit->n_keep, rows = fit->phi_rows, plane = (int64_t)NPg * rows;
+        double *e = d_e.as<double>(), *esum = e + M + 2;
+        double *u = d_rv.as<double>(), *r = u + N, *r1 = r + N, *z = r1 + nr, *dx = z + nr, *r2 = dx + nr + 1;
+        const double *phi = fit->ws[7].as<double>();
+        const int steps = fit->opt.refine_steps > 0 ? fit->opt.refine_steps : 2;
+        for (int it = 0; it < steps; ++it) {
+            k_ref_expand<<<ceil_div(N, 256), 256, 0, st>>>(d_x.as<double>(), dc_inv_ptr, dc_inv_set, dc_inv_mult, N, u,
+                                                           d_cinfo.as<int>(), d_rs);
+            k_ref_rows<<<n_esum, 256, 0, st>>>(phi, rows, plane, N, M, fit->d_v.as<double>(), u, e, esum,
+                                               d_cinfo.as<int>(), d_rs);
+            k_ref_cols<<<N, 256, 0, st>>>(phi, rows, plane, M, e, esum, n_esum, r, d_cinfo.as<int>(), d_rs);
+            k_ref_rhs<<<1, 256, 0, st>>>(r, dc_ptr, dc_src, dc_mult, dc_w, con->diag_add ? dc_diag : nullptr,
+                                         con->rhs_add ? dc_rhs : nullptr, con->q_tot, nr, d_x.as<double>(), r1, r2,
+                                         d_cinfo.as<int>(), d_rs);
+            k_fwdsub<<<nblk, 256, 0, st>>>(d_K.as<double>(), ld, nr, d_dinv.as<double>(), r1, z, d_sync.as<unsigned>(),
+                                           fit->bs_base, ++fit->bs_epoch, spin_limit, d_cinfo.as<int>() + 1,
+                                           d_cinfo.as<int>(), d_rs);
+            fit->bs_base += (unsigned)nblk;
+            k_backsub<<<nblk, 256, 0, st>>>(d_K.as<double>(), ld, nr, d_dinv.as<double>(), d_wc.as<double>(), z, 0.0,
+                                            dx, d_sync.as<unsigned>(), fit->bs_base, ++fit->bs_epoch, spin_limit,
+                                            d_cinfo.as<int>() + 1, d_cinfo.as<int>(), r2, &d_rs->go);
+            fit->bs_base += (unsigned)nblk;
+            k_ref_update<<<1, 256, 0, st>>>(d_x.as<double>(), dx, nr, RP_REFINE_STOP_E, d_cinfo.as<int>(), d_rs);
+        }
+    }
+    k_publish<<<ceil_div(N + 1, 256), 256, 0, st>>>(d_x.as<double>(), dc_exp, N, n, d_cinfo.as<int>(), d_rs, h_out_dev);
     RP_HIP(ctx, hipGetLastError());
     RP_HIP(ctx, hipEventRecord(e1, st));
     RP_HIP(ctx, hipStreamSynchronize(st));
@@ -1137,6 +1456,8 @@ int run_solve(rp_fit *fit, const rp_constraints *con, double *q_out, double *sol
         if (solution_out) std::copy(h_out + 1 + N, h_out + 1 + N + n, solution_out);
     }
     fit->lu_fallback = cinfo != 0;
+    fit->refine_steps_done = refine && cinfo == 0 ? (int32_t)h_out[1 + N + n] : 0;
+    fit->refine_last_dq = refine && cinfo == 0 ? h_out[2 + N + n] : 0.0;
     if (cinfo < 0) {
         // a backward-substitution wait gave up: start the tickets and flags afresh for the next solve
         const int words = (int)(d_sync.bytes / sizeof(unsigned));

@@ -1,0 +1,122 @@
+"""Split form with iterative refinement (RP_PREC_OZAKI_REFINED) on the GPU, through the C ABI.
+
+The Gram comes from 3 int8 slices per basis element; each refinement step forms the residual of
+the fp64 normal equations from the fp64 basis kept in HBM and corrects the solve with the split
+system's Cholesky factor.  The charges converge to the fp64 system's solution, so the bars are
+the fp64 path's: within 1e-6 e of the oracle's golden charges, within 1e-9 e of the oracle on
+small systems (symmetry with merge multiplicities, RESP, non-zero total charge), and within
+rounding (1e-8 e at condition numbers up to ~2e6) of the device's own fp64 path.
+"""
+import numpy as np
+import pytest
+
+from pyrepeat_amd import _abi, params, synth
+from pyrepeat_amd.calculator import ChargeCalculator
+from pyrepeat_amd.cube import cube_from_system
+
+from test_gpu_parity import _calc, _golden
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("name", ["water", "crystal200"])
+def test_refined_matches_golden_and_fp64(gpu_ctx, tmp_path, name):
+    g = _golden(name)
+    s = getattr(synth, name)()
+    print(f"fit refined {name}", flush=True)
+    calc = _calc(gpu_ctx, s, "refined", tmp_path)
+    q = calc.fit_charges()
+    steps, last = calc._fit.refine_stats()
+    dq = np.max(np.abs(q - np.array(g["charges"])))
+    print(f"  max|dq| vs golden = {dq:.3e}, steps {steps}, last correction {last:.2e}", flush=True)
+    assert not calc._fit.last_solve_used_lu
+    assert 1 <= steps <= 2
+    assert dq <= 1e-6
+    assert abs(q.sum() - s.q_tot) < 1e-9
+    calc.close()
+    c64 = _calc(gpu_ctx, s, "fp64", tmp_path)
+    q64 = c64.fit_charges()
+    c64.close()
+    assert np.max(np.abs(q - q64)) <= 1e-8
+
+
+@pytest.mark.parametrize("symm_text,q_tot,resp", [
+    ("1,3\n2-2\n4,6,8\n", 1.0, None),
+    ("1,3\n1,3\n5,7\n", 0.0, "H 0.1 0.0\nC 0.05 -0.1\n"),   # atom 3 twice in set 1: multiplicity 2
+])
+def test_refined_symmetry_resp_matches_oracle(oracle, gpu_ctx, tmp_path, symm_text, q_tot, resp):
+    s = synth.tiny(seed=12, n_atoms=8, n=16, L=11.0, triclinic=True)
+    p = tmp_path / "symm.txt"
+    p.write_text(symm_text)
+    rp = None
+    if resp:
+        rp = tmp_path / "resp.txt"
+        rp.write_text(resp)
+    print(f"refined symmetry case {symm_text!r} q_tot={q_tot} resp={resp is not None}", flush=True)
+    r = oracle.fit(s.n_grid, s.axis, s.atomic_numbers, s.atom_pos, s.esp_flat(), q_tot=q_tot,
+                   symm=oracle.read_symmetry(str(p)), resp=oracle.read_resp_params(str(rp)) if rp else None)
+    calc = ChargeCalculator(cube=cube_from_system(s), q_tot=q_tot, symm_file=str(p),
+                            resp_file=str(rp) if rp else None, precision="refined", context=gpu_ctx)
+    q = calc.fit_charges()
+    assert calc._fit.refine_stats()[0] >= 1
+    np.testing.assert_allclose(q, r.charges, atol=1e-9)
+    calc.close()
+
+
+def test_refined_deterministic_and_basis_kept(gpu_ctx, tmp_path):
+    """Two fits are bitwise equal; the --stats matvec (its own chunk buffer) between two solves leaves
+    the kept basis intact, so the second solve repeats the first bitwise; refit does the same."""
+    s = synth.crystal200()
+    print("refined determinism: crystal200", flush=True)
+    calc = _calc(gpu_ctx, s, "refined", tmp_path)
+    q1 = calc.fit_charges()
+    calc._fit.phi_matvec(q1)
+    q2, _ = calc._fit.solve(calc.constraints())
+    q3 = calc._fit.refit(calc.constraints())
+    calc.close()
+    calc = _calc(gpu_ctx, s, "refined", tmp_path)
+    q4 = calc.fit_charges()
+    calc.close()
+    for q in (q2, q3, q4):
+        np.testing.assert_array_equal(q, q1)
+
+
+def test_refined_step_limit(gpu_ctx):
+    """refine_steps bounds the steps; a step runs only while the last correction exceeded 1e-4 e."""
+    s = synth.water()
+    con = params.constraint_arrays(s.n_atoms, s.atomic_numbers, 0.0, None, None, None)
+    out = {}
+    for steps in (1, 3):
+        print(f"refined step limit {steps}", flush=True)
+        f = _abi.Fit(gpu_ctx, s.n_grid, s.axis, s.atomic_numbers, s.atom_pos, s.esp_cube, _abi.RP_ORDER_CUBE, 1,
+                     _abi.RP_PREC_OZAKI_REFINED, refine_steps=steps)
+        f.accumulate()
+        q, _ = f.solve(con)
+        out[steps] = (q, *f.refine_stats())
+        f.close()
+    assert out[1][1] == 1
+    n3, last3 = out[3][1], out[3][2]
+    assert 1 <= n3 <= 3
+    assert n3 == 3 or last3 <= 1e-4
+    # steps after the first only shrink a correction that was already at most 1e-4 e
+    assert np.max(np.abs(out[1][0] - out[3][0])) <= 1e-4
+
+
+def test_refined_large_n_matches_fp64(gpu_ctx):
+    """N = 1600: 25 diagonal blocks in the ticketed forward and backward substitutions of each step,
+    the separate panel launches of the large-N Cholesky.  The refined charges equal the device's fp64
+    path (same fp64 normal equations) to rounding."""
+    s = synth.tiny(seed=21, n_atoms=1600, n=76, L=90.0)
+    con = params.constraint_arrays(s.n_atoms, s.atomic_numbers, 0.0, None, None, None)
+    print("refined large-N: 1600 atoms, 76x77x78 grid", flush=True)
+    qs = {}
+    for prec in (_abi.RP_PREC_OZAKI_REFINED, _abi.RP_PREC_FP64):
+        f = _abi.Fit(gpu_ctx, s.n_grid, s.axis, s.atomic_numbers, s.atom_pos, s.esp_cube, _abi.RP_ORDER_CUBE, 1,
+                     prec)
+        f.accumulate()
+        qs[prec], _ = f.solve(con)
+        assert not f.last_solve_used_lu
+        if prec == _abi.RP_PREC_OZAKI_REFINED:
+            print(f"  steps, last correction = {f.refine_stats()}", flush=True)
+        f.close()
+    assert np.max(np.abs(qs[_abi.RP_PREC_OZAKI_REFINED] - qs[_abi.RP_PREC_FP64])) <= 1e-7

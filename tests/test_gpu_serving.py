@@ -18,6 +18,7 @@ pytestmark = pytest.mark.gpu
 
 SPLIT = _abi.make_options(1, _abi.RP_PREC_OZAKI, 1.0, 1000.0, 20.0)
 F64 = _abi.make_options(1, _abi.RP_PREC_FP64, 1.0, 1000.0, 20.0)
+REFINED = _abi.make_options(1, _abi.RP_PREC_OZAKI_REFINED, 1.0, 1000.0, 20.0)
 
 
 def _con(s):
@@ -53,7 +54,7 @@ def stream12():
     return [(s, _con(s)) for s in reqs]
 
 
-@pytest.mark.parametrize("precision", [_abi.RP_PREC_OZAKI, _abi.RP_PREC_FP64])
+@pytest.mark.parametrize("precision", [_abi.RP_PREC_OZAKI, _abi.RP_PREC_FP64, _abi.RP_PREC_OZAKI_REFINED])
 def test_reset_matches_fresh_fit_without_growth(stream12, precision):
     big = max(stream12, key=lambda sc: int(np.prod(sc[0].n_grid)) * sc[0].n_atoms)
     small = min(stream12, key=lambda sc: int(np.prod(sc[0].n_grid)) * sc[0].n_atoms)
@@ -82,7 +83,8 @@ def test_request_bound_covers_device_memory(stream12):
     cases = [(stream12[0][0], stream12[0][1]), (synth.water(), None), (synth.crystal200(), None)]
     for s, con in cases:
         con = con or _con(s)
-        for opts, prec in ((SPLIT, _abi.RP_PREC_OZAKI), (F64, _abi.RP_PREC_FP64)):
+        for opts, prec in ((SPLIT, _abi.RP_PREC_OZAKI), (F64, _abi.RP_PREC_FP64),
+                           (REFINED, _abi.RP_PREC_OZAKI_REFINED)):
             print(f"bound case {s.name} precision={prec}", flush=True)
             bound = _abi.Server.request_bytes(_args(s), con, opts)
             ctx = _abi.Context(0, 0)
@@ -94,7 +96,10 @@ def test_request_bound_covers_device_memory(stream12):
             f.close()
             ctx.close()
             assert used <= bound, (s.name, prec, used, bound)
-            assert bound < 4 * used + (64 << 20), (s.name, prec, used, bound)   # and not absurdly loose
+            # and not absurdly loose; the refined form keeps the basis of every kept point, which the
+            # bound (fixed before the filter) takes for the whole grid
+            kept = 8 * (-(-(s.n_atoms + 2) // 128) * 128) * int(np.prod(s.n_grid)) if prec == _abi.RP_PREC_OZAKI_REFINED else 0
+            assert bound < 4 * used + kept + (64 << 20), (s.name, prec, used, bound)
 
 
 def test_server_mix_equals_alone(stream12):
