@@ -103,9 +103,9 @@ def run_gpu(ctx, system, con, precision, steps, warmup):
     prepared = fit._constraints(con)
     for _ in range(warmup):
         q = fit.refit(con, prepared)
-    # inside the timed region only the roofline kernel (the Gram) is timed: every timed launch adds
-    # two event records to the stream, each a few microseconds of idle GPU
-    ctx.profile(True, kinds=("gram",))
+    # inside the timed region only the roofline kernel (the basis, the most time per fit) is timed:
+    # every timed launch adds two event records to the stream, each a few microseconds of idle GPU
+    ctx.profile(True, kinds=("kspace",))
     lat = []
     t_start = time.perf_counter()
     for i in range(steps):
@@ -113,7 +113,7 @@ def run_gpu(ctx, system, con, precision, steps, warmup):
         q = fit.refit(con, prepared)
         lat.append(time.perf_counter() - t0)
     total = time.perf_counter() - t_start
-    gram = ctx.kernel_stats()["gram"]
+    dominant = ctx.kernel_stats()["kspace"]
     # per-kernel breakdown: every kind timed, on further fits after the timed region
     breakdown_fits = max(1, min(steps, 5))
     ctx.profile(True)
@@ -134,7 +134,7 @@ def run_gpu(ctx, system, con, precision, steps, warmup):
     else:
         rows = min(65536, -(-M // 128) * 128)
     out = {"q": q, "lat": np.array(lat), "total": total, "kernels": stats, "breakdown_fits": breakdown_fits,
-           "gram": gram, "M": M, "rows": rows, "slices": slices,
+           "dominant": dominant, "M": M, "rows": rows, "slices": slices,
            "N": int(fit.n_atoms), "K": int(info.n_k), "timings": fit.timings(),
            "refine": fit.refine_stats() if prec == _abi.RP_PREC_OZAKI_REFINED else None}
     fit.close()
@@ -221,7 +221,8 @@ def serve_main(args):
     import tempfile
     with tempfile.TemporaryDirectory() as td:
         reqs = [(s, _constraints(s, td)) for s in systems]
-    split = _abi.make_options(1, _abi.RP_PREC_OZAKI, 1.0, 1000.0, 20.0)
+    chosen_prec = _abi.RP_PREC_OZAKI_REFINED if args.chosen == "refined" else _abi.RP_PREC_OZAKI
+    split = _abi.make_options(1, chosen_prec, 1.0, 1000.0, 20.0)
     f64 = _abi.make_options(1, _abi.RP_PREC_FP64, 1.0, 1000.0, 20.0)
     srv = _abi.Server(0, args.slots, int(args.budget_gb * 1e9))
     # warm-up: every slot sees the largest request once, so the timed stream allocates nothing
@@ -256,19 +257,21 @@ def serve_main(args):
         "metric": "charge fits/s + p50/p99 latency, serving stream (Poisson 20 req/s, 20-800 atoms)",
         "value": round(R / span, 4), "unit": "fits/s", "n_gpus": 1, "steps": R, "warmup": args.slots,
         "ms_per_step": round(1e3 * span / R, 3), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "i8 slices -> i32 exact, f64 accumulate/solve",
+        "dtype": "f64 basis/solve/refinement, i8 slices -> i32 exact Gram" if args.chosen == "refined" else
+                 "f64 basis/solve, i8 slices -> i32 exact Gram",
         "data": "synthetic stream (seeded Ewald ESP + 1e-5 noise, fp32 cubes)",
         "config": {"workload": f"stream{R}", "requests": R, "rate_per_s": 20.0, "slots": args.slots,
                    "hbm_budget_gb": args.budget_gb, "atoms_min": int(n_atoms.min()), "atoms_max": int(n_atoms.max()),
                    "grid_points_total": int(sum(int(np.prod(s.n_grid)) for s in systems)),
-                   "precision": "split (int8 Ozaki)"},
+                   "precision": "split (int8 Ozaki, 4 slices)" if args.chosen == "split" else
+                   "refined (int8 Ozaki, 3 slices, fp64 residual refinement)"},
         "p50_ms": round(1e3 * float(np.median(lat)), 3), "p99_ms": round(1e3 * float(np.percentile(lat, 99)), 3),
         "service_p50_ms": round(1e3 * float(np.median(service)), 3),
         "queue_wait_p99_ms": round(1e3 * float(np.percentile(queue_wait, 99)), 3),
         "capacity_fits_per_s": round(R / span_sat, 4),
         "max_running": int(st_sat["max_running"]), "peak_reserved_gb": round(st_sat["peak_reserved"] / 1e9, 3),
         "open_loop_max_running": int(st_open["max_running"]),
-        "mix_vs_alone_max_dq_e": dq_alone, "split_vs_fp64_max_dq_e": dq_f64, "sample_requests": sample,
+        "mix_vs_alone_max_dq_e": dq_alone, "chosen_vs_fp64_max_dq_e": dq_f64, "sample_requests": sample,
     }
     print(json.dumps(line))
     return 0
@@ -280,8 +283,10 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="mof1000")
-    ap.add_argument("--chosen", choices=["split", "refined"], default="split",
-                    help="the chosen path: split (4 slices) or refined (3 slices + fp64 refinement)")
+    ap.add_argument("--chosen", choices=["split", "refined"], default=None,
+                    help="the chosen path: split (4 slices) or refined (3 slices + fp64 refinement); "
+                         "default refined for a single fit, split for the serving stream (small requests, "
+                         "where the refinement's passes cost more than the 3-slice Gram saves)")
     ap.add_argument("--baseline-steps", type=int, default=None)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
@@ -296,6 +301,8 @@ def main():
         print(json.dumps({"error": "this benchmark runs on one GPU: the fit path is measured on one MI355X; "
                                    "multi-GPU sharding is described, not run"}))
         return 2
+    if args.chosen is None:
+        args.chosen = "split" if args.serve else "refined"
     if args.calibrate_copy_mib:
         calibrate_copy(args.calibrate_copy_mib)
     if args.serve:
@@ -332,26 +339,40 @@ def main():
     fits_s = args.steps / split["total"]
     fits_s64 = bsteps / f64["total"]
     M, N, K = split["M"], split["N"], split["K"]
-    # dominant kernel of the chosen path (the int8 Gram) and its roofline
-    ms, launches = split["gram"]
+    # dominant kernel of the chosen path (the basis, k_sep_phi: the most time per fit) and its roofline
+    ms, launches = split["dominant"]
     rows = split["rows"]
-    # F_G = M N (N + 1) + 3 M N per fit (Phi^T Phi, Phi^T V, sums); one launch covers `rows` points
-    alg_per_launch = (M * N * (N + 1) + 3.0 * M * N) / (launches / args.steps)
+    launches_per_fit = launches / args.steps
     t_launch = ms / launches * 1e-3
-    achieved = alg_per_launch / t_launch / 1e12
+    # F_k = 4 M N K per fit (the k-space contraction of the reference's formulation, counted once); one
+    # launch covers `rows` points
+    fk_per_launch = 4.0 * M * N * K / launches_per_fit
+    achieved = fk_per_launch / t_launch / 1e12
+    roof = {"bound": "mfma", "kernel": "k_sep_phi", "achieved": round(achieved, 3), "peak": PEAK_FP64_TFLOPS,
+            "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP64_TFLOPS, 4),
+            "traffic": traffic_from_profile("k_sep_phi", args.workload),
+            "launch_ms": round(ms / launches, 4), "rows_per_launch": rows,
+            "note": "achieved = algorithmic k-space flops F_k = 4 M N K (the reference's cos/sin GEMM "
+                    "formulation) of the launch's rows / mean launch time (HIP events on the fit's stream); "
+                    "peak = fp64, the arithmetic of the kernel; frac > 1 because the factorized sum does about "
+                    "1/25 of those flops (plus the real-space sum) on the fp64 vector units"}
+    # the int8 Gram (MFMA), from the per-kernel breakdown fits after the timed region
+    gms, glaunch = split["kernels"]["gram"]
+    g_launch = gms / glaunch * 1e-3
+    alg_per_launch = (M * N * (N + 1) + 3.0 * M * N) / (glaunch / split["breakdown_fits"])
     np_ = ((N + 2 + 127) // 128) * 128
     tiles = sum(1 for bi in range(np_ // 64) for bj in range(np_ // 128) if bi * 64 <= bj * 128 + 127)
     P = split["slices"]
-    executed = 2.0 * rows * tiles * 64 * 128 * P * P / t_launch / 1e12    # int8 ops issued, P^2 slice products
-    roof = {"bound": "mfma", "kernel": "k_oz_gram", "achieved": round(achieved, 3), "peak": PEAK_INT8_TOPS,
-            "unit": "TFLOP/s", "frac": round(achieved / PEAK_INT8_TOPS, 5),
-            "traffic": traffic_from_profile("k_oz_gram", args.workload),
-            "executed_int8_tops": round(executed, 1), "executed_frac": round(executed / PEAK_INT8_TOPS, 4),
-            "launch_ms": round(ms / launches, 4), "rows_per_launch": rows,
-            "note": "achieved = algorithmic fp64-equivalent flops of the Gram per launch (F_G share of the "
-                    "launch's rows) / mean launch time (HIP events on the fit's stream); peak = dense int8 MFMA, "
-                    f"the unit the kernel runs on; each fp product costs P^2 = {P * P} int8 slice products, which "
-                    "executed_int8_tops counts", "slices": P}
+    executed = 2.0 * rows * tiles * 64 * 128 * P * P / g_launch / 1e12    # int8 ops issued, P^2 slice products
+    roof_gram = {"bound": "mfma", "kernel": "k_oz_gram", "achieved": round(alg_per_launch / g_launch / 1e12, 3),
+                 "peak": PEAK_INT8_TOPS, "unit": "TFLOP/s",
+                 "frac": round(alg_per_launch / g_launch / 1e12 / PEAK_INT8_TOPS, 5),
+                 "traffic": traffic_from_profile("k_oz_gram", args.workload),
+                 "executed_int8_tops": round(executed, 1), "executed_frac": round(executed / PEAK_INT8_TOPS, 4),
+                 "launch_ms": round(gms / glaunch, 4), "slices": P,
+                 "note": "achieved = fp64-equivalent Gram flops F_G = M N (N + 1) + 3 M N of the launch's rows / "
+                         f"mean launch time (events, breakdown fits); peak = dense int8 MFMA; each fp product costs "
+                         f"P^2 = {P * P} int8 slice products, which executed_int8_tops counts"}
     cpu = None
     if not args.no_cpu:
         _log("cpu baseline (oracle sample) ...")
@@ -365,7 +386,9 @@ def main():
         "metric": "charge fits/s + p50/p99 latency, chosen vs plain FP64 GPU at <=1e-5 e",
         "value": round(fits_s, 4), "unit": "fits/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": round(1e3 * split["total"] / args.steps, 3), "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "i8 slices -> i32 exact, f64 accumulate/solve",
+        "vs_baseline": None,
+        "dtype": "f64 basis/solve/refinement, i8 slices -> i32 exact Gram" if args.chosen == "refined" else
+                 "f64 basis/solve, i8 slices -> i32 exact Gram",
         "data": "synthetic (seeded exact Ewald ESP + 1e-5 noise)",
         "config": {"workload": args.workload, "n_atoms": N, "grid": [int(v) for v in system.n_grid],
                    "kept_points": M, "k_vectors": K,
@@ -381,6 +404,7 @@ def main():
         "refine_steps_last_correction_e": split["refine"],
         "kernels_chosen": kms, "kernels_fp64": kms64,
         "roofline": roof,
+        "roofline_gram": roof_gram,
         "cpu_baseline": cpu,
         "speedup_vs_cpu": round(fits_s / cpu["value"], 1) if cpu else None,
     }

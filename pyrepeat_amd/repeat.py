@@ -5,8 +5,10 @@
         [--stats] [--output F] [--fp64] [--precision fp64|split|refined] [--charge repeat|qeq]
 
 Every fit runs on the GPU through ``librepeat_hip.so``.  As with the reference's
-``--gpu``, the default precision is the split (mixed) form and ``--fp64``
-selects the plain fp64 path.  ``--gpu``, ``--cores``, ``--block-k`` and
+``--gpu``, the default precision is a split (mixed) form: ``refined`` (int8-sliced
+Gram, then fp64 iterative refinement of the solve, so the charges are those of
+the fp64 normal equations); ``--fp64`` selects the plain fp64 path and
+``--precision split`` the split form without refinement.  ``--gpu``, ``--cores``, ``--block-k`` and
 ``--no-jit`` are accepted for compatibility and do not change the result.
 The charge table, ``Total charge`` and ``RMS error`` lines and the
 ``--output`` rows use the reference's formats (``repeat.py:2775-2789``).
@@ -53,7 +55,7 @@ def main(argv=None):
         print(f"Error: Cube file '{args.cube_file}' not found!")
         return 1
     from .calculator import ChargeCalculator
-    precision = args.precision or ("fp64" if args.fp64 else "split")
+    precision = args.precision or ("fp64" if args.fp64 else "refined")
     device = 0
     if args.gpu not in (None, "", "all"):
         device = int(str(args.gpu).split(",")[0])
