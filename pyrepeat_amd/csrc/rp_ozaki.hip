@@ -11,12 +11,14 @@
 //
 // Use (fit_charges steps 1-3, repeat.py:2195-2262): the Gram [Phi|V|1]^T [Phi|V|1] of each chunk of
 // kept points, from Phi sliced per column (scale from the column maximum over the chunk), upper
-// tiles, 4 slices of each operand (16 products, 7 exact groups).  Phi itself comes from the
-// factorized reciprocal sum and the fp64 real-space sum (rp_kspace.hip).
+// tiles, 4 slices of each operand (16 products, 7 exact groups) in the split form, 3 (9 products,
+// 5 groups) in the refined form, whose solve is then refined against the fp64 Phi kept in HBM
+// (rp_solve.hip).  Phi itself comes from the factorized reciprocal sum and the fp64 real-space sum
+// (rp_kspace.hip).
 //
 // MFMA: v_mfma_i32_32x32x32_i8, one 32x32 tile per wave, 8 waves per block (2 x 4)
 // -> 64x128 block tile, k-step 32, slices loaded straight into LDS, three stage buffers.
-// Seven exact int32 group accumulators per wave tile.
+// 2P - 1 exact int32 group accumulators per wave tile.
 #include <algorithm>
 #include <cmath>
 
