@@ -54,8 +54,10 @@ extern "C" {
                             the fp64 basis kept in HBM (M x (N + 2) doubles), and corrects q
                             and the multiplier with the Cholesky factor of the split system.
                             The result converges to the fp64 system's solution.  Steps stop
-                            when a correction is at most RP_REFINE_STOP_E (max |dq|, e). */
-#define RP_REFINE_STOP_E 1e-4
+                            when a correction is at most RP_REFINE_STOP_E (max |dq|, e): a step
+                            shrinks the error by rho <= 5.3e-3 on the systems measured (cond(A)
+                            up to 4.9e10), so the error left is at most ~2e-6 e. */
+#define RP_REFINE_STOP_E 3e-4
 
 typedef struct rp_ctx rp_ctx;
 typedef struct rp_fit rp_fit;
