@@ -120,3 +120,28 @@ def test_refined_large_n_matches_fp64(gpu_ctx):
             print(f"  steps, last correction = {f.refine_stats()}", flush=True)
         f.close()
     assert np.max(np.abs(qs[_abi.RP_PREC_OZAKI_REFINED] - qs[_abi.RP_PREC_FP64])) <= 1e-7
+
+
+def test_refined_fit_type0_matches_oracle(oracle, gpu_ctx):
+    """Bare 1/r basis (no reciprocal part): the kept basis and the refinement are the same code."""
+    s = synth.tiny(seed=13, n_atoms=4, n=14, L=10.0)
+    print("refined fit-type 0", flush=True)
+    r = oracle.fit(s.n_grid, s.axis, s.atomic_numbers, s.atom_pos, s.esp_flat(), fit_flag=0)
+    calc = ChargeCalculator(cube=cube_from_system(s), fit_flag=0, precision="refined", context=gpu_ctx)
+    np.testing.assert_allclose(calc.fit_charges(), r.charges, atol=1e-9)
+    calc.close()
+
+
+def test_refined_singular_kkt_reports(gpu_ctx):
+    """Two atoms at one place give identical basis columns, so identical slices and an exactly singular
+    split Gram: the Cholesky stops, every refinement kernel stands down, and the LU reports the zero pivot."""
+    s = synth.tiny(seed=14, n_atoms=3, n=12, L=9.0)
+    pos = s.atom_pos.copy()
+    pos[1] = pos[0]
+    print("refined singular KKT", flush=True)
+    calc = ChargeCalculator(cube={"n_atoms": 3, "n_grid": s.n_grid, "atom_index": s.atomic_numbers, "atom_pos": pos,
+                                  "axis_vector": s.axis, "esp_cube": s.esp_cube}, precision="refined", context=gpu_ctx)
+    with pytest.raises(_abi.SingularKKTError):
+        calc.fit_charges()
+    assert calc._fit.refine_stats() == (0, 0.0)
+    calc.close()
