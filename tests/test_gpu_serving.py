@@ -98,7 +98,9 @@ def test_request_bound_covers_device_memory(stream12):
             assert used <= bound, (s.name, prec, used, bound)
             # and not absurdly loose; the refined form keeps the basis of every kept point, which the
             # bound (fixed before the filter) takes for the whole grid
-            kept = 8 * (-(-(s.n_atoms + 2) // 128) * 128) * int(np.prod(s.n_grid)) if prec == _abi.RP_PREC_OZAKI_REFINED else 0
+            kept = 0
+            if prec == _abi.RP_PREC_OZAKI_REFINED:
+                kept = 8 * (-(-(s.n_atoms + 2) // 128) * 128) * int(np.prod(s.n_grid))
             assert bound < 4 * used + kept + (64 << 20), (s.name, prec, used, bound)
 
 
