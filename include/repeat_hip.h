@@ -192,7 +192,8 @@ int rp_fit_solve(rp_fit *fit, const rp_constraints *con, double *q_out, double *
  */
 int rp_fit_refit(rp_fit *fit, const rp_constraints *con, double *q_out);
 
-/* compute_stats (repeat.py:2326-2441): v_coul[m] = (Phi q)[m] for every kept point, streamed. */
+/* compute_stats (repeat.py:2326-2441): v_coul[m] = (Phi q)[m] for every kept point, streamed (from the
+ * basis that RP_PREC_OZAKI_REFINED keeps after rp_fit_accumulate, otherwise recomputed in fp64). */
 int rp_fit_phi_matvec(rp_fit *fit, const double *q, double *v_coul);
 
 /*

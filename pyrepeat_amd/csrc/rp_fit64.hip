@@ -256,6 +256,33 @@ __global__ __launch_bounds__(256) void k_rowdot(const double *__restrict__ Phi, 
     if (lane == 0) out[r] = s;
 }
 
+// v = Phi q over the kept points, from the basis the refined form keeps ([chunk][NP][rows], column-major
+// per chunk, chunk c holding rows [c rows, (c + 1) rows)); two rows per thread.
+__global__ __launch_bounds__(256) void k_kept_matvec(const double *__restrict__ phi, int64_t rows, int64_t plane, int N,
+                                                     int64_t M, const double *__restrict__ q, double *__restrict__ out) {
+    const int64_t R = 2 * ((int64_t)blockIdx.x * 256 + threadIdx.x);
+    if (R >= M) return;
+    const int64_t c = R / rows, r = R - c * rows;   // rows is even: R and R + 1 share the chunk
+    const double *col = phi + c * plane + r;
+    double a0[4] = {0.0, 0.0, 0.0, 0.0}, a1[4] = {0.0, 0.0, 0.0, 0.0};
+    int j = 0;
+    for (; j + 4 <= N; j += 4) {
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            const double2 p = *reinterpret_cast<const double2 *>(col + (int64_t)(j + t) * rows);
+            a0[t] = fma(p.x, q[j + t], a0[t]);
+            a1[t] = fma(p.y, q[j + t], a1[t]);
+        }
+    }
+    for (; j < N; ++j) {
+        const double2 p = *reinterpret_cast<const double2 *>(col + (int64_t)j * rows);
+        a0[0] = fma(p.x, q[j], a0[0]);
+        a1[0] = fma(p.y, q[j], a1[0]);
+    }
+    out[R] = (a0[0] + a0[1]) + (a0[2] + a0[3]);
+    if (R + 1 < M) out[R + 1] = (a1[0] + a1[1]) + (a1[2] + a1[3]);
+}
+
 }  // namespace
 
 // Lower triangle from the upper one (every element with i <= j has been accumulated).
@@ -326,6 +353,21 @@ int accumulate_f64(rp_fit *fit, FitDevice &dv) {
 int run_phi_matvec(rp_fit *fit, const double *q, double *v_coul) {
     Ctx *ctx = fit->ctx;
     hipStream_t st = ctx->stream;
+    if (fit->phi_kept) {
+        // the refined form keeps the fit's fp64 basis of every kept point: one streaming pass over it
+        const int N = fit->n_atoms;
+        const int64_t M = fit->n_keep, rows = fit->phi_rows, NP = round_up(N + 2, 128);
+        DBuf &d_q = fit->ws[22], &d_out = fit->ws[23];
+        RP_CHECK(d_q.grow(ctx, sizeof(double) * N));
+        RP_CHECK(d_out.grow(ctx, sizeof(double) * M));
+        RP_HIP(ctx, hipMemcpyAsync(d_q.p, q, sizeof(double) * N, hipMemcpyHostToDevice, st));
+        k_kept_matvec<<<ceil_div(M, 512), 256, 0, st>>>(fit->ws[7].as<double>(), rows, NP * rows, N, M,
+                                                         d_q.as<double>(), d_out.as<double>());
+        RP_HIP(ctx, hipGetLastError());
+        RP_HIP(ctx, hipMemcpyAsync(v_coul, d_out.p, sizeof(double) * M, hipMemcpyDeviceToHost, st));
+        RP_HIP(ctx, hipStreamSynchronize(st));
+        return RP_OK;
+    }
     // the matvec always evaluates the basis in fp64 (KP padded for the fp64 GEMM)
     FitDevice *dvp = nullptr;
     RP_CHECK(device_setup(fit, RP_PREC_FP64, dvp));

@@ -145,3 +145,28 @@ def test_refined_singular_kkt_reports(gpu_ctx):
         calc.fit_charges()
     assert calc._fit.refine_stats() == (0, 0.0)
     calc.close()
+
+
+def test_refined_stats_from_kept_basis(oracle, gpu_ctx, tmp_path):
+    """compute_stats of a refined fit streams the kept basis instead of recomputing it: V_coul and the
+    RMS match the oracle on a small cell, and V_coul matches the fp64 recomputation on crystal200."""
+    s = synth.tiny(seed=6, n_atoms=4, n=14, L=10.0, triclinic=True)
+    print("refined stats: tiny6", flush=True)
+    r = oracle.fit(s.n_grid, s.axis, s.atomic_numbers, s.atom_pos, s.esp_flat(), stats=True)
+    calc = ChargeCalculator(cube=cube_from_system(s), precision="refined", context=gpu_ctx)
+    q = calc.fit_charges()
+    np.testing.assert_allclose(q, r.charges, atol=1e-9)
+    st = calc.compute_stats(q, need_v_coul=True)
+    np.testing.assert_allclose(st["V_coul"], r.v_coul, rtol=1e-10, atol=1e-12)
+    assert abs(st["rms_error"] - r.rms) < 1e-10
+    calc.close()
+    s = synth.crystal200()
+    print("refined stats: crystal200 against the fp64 recomputation", flush=True)
+    calc = _calc(gpu_ctx, s, "refined", tmp_path)
+    q = calc.fit_charges()
+    v_kept = calc._fit.phi_matvec(q)
+    calc.close()
+    c64 = _calc(gpu_ctx, s, "fp64", tmp_path)   # not accumulated: phi_matvec recomputes the basis
+    v64 = c64._fit.phi_matvec(q)
+    c64.close()
+    np.testing.assert_allclose(v_kept, v64, rtol=1e-11, atol=1e-13 * np.abs(v64).max())
