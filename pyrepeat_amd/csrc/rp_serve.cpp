@@ -93,7 +93,7 @@ int validate_request(const rp_request *r, std::string &err) {
 }  // namespace
 
 // Upper bound of the device bytes one fit of this request allocates: vdW filter, Ewald tables,
-// the chunk buffers of its precision (and the factorized reciprocal tables of the split form), the fp64 matvec of --stats, Gram and solve, with the kept
+// the chunk buffers of its precision (and the factorized reciprocal tables of the split form), the matvec of --stats, Gram and solve, with the kept
 // count bounded by the grid size.  The fit's grow-only workspaces are shared between phases,
 // so each is bounded by the largest size any phase asks of it.
 uint64_t fit_bytes_bound(const Setup &s, int n_atoms, int esp_dtype, int precision, int n_reduced, int n_src) {
@@ -111,17 +111,20 @@ uint64_t fit_bytes_bound(const Setup &s, int n_atoms, int esp_dtype, int precisi
     };
     const ChunkPlan c64 = plan_chunks(M, n_atoms, (int)K, RP_PREC_FP64);
     const uint64_t NP = c64.NP;
-    // fp64 basis chunks: the fp64 accumulate, and the matvec of either precision
-    at(0, 8 * (uint64_t)c64.rows * c64.KP);
-    at(1, 8 * (uint64_t)c64.KP * NP);
-    at(2, 8 * (uint64_t)c64.rows * NP);
+    // fp64 basis chunks: the fp64 accumulate, and the matvec of the fp64 and split forms (the refined
+    // form's matvec streams the basis it keeps)
+    if (precision != RP_PREC_OZAKI_REFINED) {
+        at(0, 8 * (uint64_t)c64.rows * c64.KP);
+        at(1, 8 * (uint64_t)c64.KP * NP);
+        at(2, 8 * (uint64_t)c64.rows * NP);
+    }
     if (precision == RP_PREC_FP64) at(3, 8 * split_cap(c64.rows, c64.n_tiles) * c64.n_tiles * 128 * 128);
     at(22, 8 * N);
     at(23, 8 * M);
     uint64_t sep = 0;
     if (precision != RP_PREC_FP64) {
         const ChunkPlan c = plan_chunks(M, n_atoms, (int)K, precision);
-        at(2, 8 * (uint64_t)c.NP * c.rows);
+        if (precision == RP_PREC_OZAKI) at(2, 8 * (uint64_t)c.NP * c.rows);   // the refined form keeps ws[7]
         at(3, 8 * split_cap(c.rows, c.n_tiles) * c.n_tiles * 64 * 128);
         at(4, (uint64_t)c.slices * c.NP * c.rows);
         at(5, 8 * (uint64_t)c.NP);
