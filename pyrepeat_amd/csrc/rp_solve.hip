@@ -24,6 +24,10 @@
 // row interchanges applied to the other columns, a triangular solve for the U
 // block row and a rank-NB update of the trailing matrix.  The rhs column rides
 // along, so it holds L^-1 P b at the end; a back substitution finishes.
+//
+// Refined split form: the Cholesky solution of the split Gram's system is then corrected by iterative
+// refinement against the fp64 basis that the accumulate kept (the refinement kernels below), so the
+// result solves the fp64 normal equations.
 #include <algorithm>
 #include <cmath>
 #include <cstdlib>
