@@ -170,3 +170,25 @@ def test_refined_stats_from_kept_basis(oracle, gpu_ctx, tmp_path):
     v64 = c64._fit.phi_matvec(q)
     c64.close()
     np.testing.assert_allclose(v_kept, v64, rtol=1e-11, atol=1e-13 * np.abs(v64).max())
+
+
+def test_refined_gram_close_to_fp64(gpu_ctx):
+    """The refinement converges for any Gram close enough to the fp64 one, so the charge bars alone would
+    not notice a faulty 3-slice Gram: its normal equations are checked directly against the fp64 path's
+    (3 slices hold each element to 2^-22 of its column maximum: relative ~2^-21 per Gram entry)."""
+    s = synth.tiny(seed=3, n_atoms=6, n=16, L=10.0, triclinic=True)
+    print("refined Gram vs fp64: tiny3", flush=True)
+    out = []
+    for prec in (_abi.RP_PREC_FP64, _abi.RP_PREC_OZAKI_REFINED):
+        f = _abi.Fit(gpu_ctx, s.n_grid, s.axis, s.atomic_numbers, s.atom_pos, s.esp_cube, _abi.RP_ORDER_CUBE, 1,
+                     prec)
+        f.accumulate()
+        out.append(f.normal_equations())
+        f.close()
+    (a64, b64, s64, v64), (a3, b3, s3, v3) = out
+    # the 4-slice bars of test_split_normal_equations_close_to_fp64 (rtol 2e-6, atol 1e-9 max) times 2^7
+    np.testing.assert_allclose(a3, a64, rtol=2.6e-4, atol=1.3e-7 * np.abs(a64).max())
+    np.testing.assert_allclose(b3, b64, rtol=2.6e-4, atol=1.3e-7 * np.abs(b64).max())
+    np.testing.assert_allclose(s3, s64, rtol=2.6e-4)
+    # and it differs from fp64 at all (the slices, not a silent fp64 copy)
+    assert np.max(np.abs(a3 - a64)) > 0.0
