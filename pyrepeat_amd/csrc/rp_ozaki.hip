@@ -29,7 +29,7 @@ namespace rp {
 
 namespace {
 
-constexpr int TM = 64, TN = 128, TK = 64;
+constexpr int TM = 64, TN = 128;
 constexpr int NTHR = 512;
 
 // Slices [P][NP][rows] of the chunk's Phi (column-major per slice: rows contiguous).
@@ -40,20 +40,29 @@ struct GramOperands {
 };
 
 // Main loop: the exact int32 group sums I_s = sum_{p+q=s} A_p^T B_q (s = 0 .. 2P-2) of the block
-// tile (columns m0.. of A, n0.. of B) over k (= grid rows) [k0, k1), a multiple of TK.  Eight waves
-// as 2 x 4, one 32 x 32 tile each.  The operands come by direct global-to-LDS loads
-// (global_load_lds_dwordx4): no register staging, NSTAGE stage buffers, loads issued two stages ahead.  A stage holds, per
-// slice, TM (A) and TN (B) rows of TK = 64 bytes, unpadded; 16-byte quarter q of row r is stored at
-// quarter q ^ ((r >> 2) & 3) so the fragment reads of 16 consecutive rows fall on distinct banks.
-// One load instruction fills 1 KB (16 rows): lane l writes base + 16 l, i.e. row 16 c + l / 4,
-// stored quarter l % 4, and so fetches logical quarter (l % 4) ^ ((row >> 2) & 3).
-constexpr int NSTAGE = 3;
+// tile (columns m0.. of A, n0.. of B) over k (= grid rows) [k0, k1), a multiple of the stage depth KD.
+// Eight waves as 2 x 4, one 32 x 32 tile each.  The operands come by direct global-to-LDS loads
+// (global_load_lds_dwordx4): no register staging, NS stage buffers, loads issued NS - 1 stages ahead.
+// A stage holds, per slice, TM (A) and TN (B) rows of KD bytes, unpadded, its 16-byte units swizzled
+// by row (gram_swz) so the fragment reads of 16 consecutive rows fall on distinct banks.  One load
+// instruction fills 1 KB (1024 / KD rows): lane l writes base + 16 l, i.e. row l / (KD / 16), stored
+// unit l % (KD / 16), and so fetches logical unit (l % (KD / 16)) ^ swz(row).
 typedef __attribute__((address_space(3))) void lds_void;
 
+// Stage depth (bytes of k per row) and stage buffers: 3 slices take 128-deep stages in two buffers
+// (72 KB each: half the barriers), 4 slices 64-deep stages in three (48 KB each).
 template <int P>
-constexpr size_t gram_stage_bytes() { return (size_t)P * (TM + TN) * TK; }
+constexpr int gram_kd() { return P == 3 ? 128 : 64; }
 template <int P>
-constexpr size_t gram_lds_bytes_dma() { return NSTAGE * gram_stage_bytes<P>(); }   // 144 KiB (P = 4)
+constexpr int gram_ns() { return P == 3 ? 2 : 3; }
+template <int P>
+constexpr size_t gram_stage_bytes() { return (size_t)P * (TM + TN) * gram_kd<P>(); }
+template <int P>
+constexpr size_t gram_lds_bytes_dma() { return gram_ns<P>() * gram_stage_bytes<P>(); }   // 144 KiB
+// 16-byte unit q of row r of a stage is stored at unit q ^ swz(r), so the fragment reads of 16
+// consecutive rows fall on distinct banks: (r >> 2) & 3 for 64-byte rows, (r >> 1) & 7 for 128-byte rows.
+template <int KD>
+__device__ __forceinline__ int gram_swz(int r) { return KD == 64 ? ((r >> 2) & 3) : ((r >> 1) & 7); }
 
 // s_waitcnt with only the vector-memory count (gfx9 encoding: vmcnt bits [3:0] and [15:14],
 // expcnt [6:4] and lgkmcnt [11:8] left at their maxima).
@@ -71,8 +80,11 @@ __device__ __forceinline__ void lds_barrier() { asm volatile("s_waitcnt lgkmcnt(
 template <int P>
 __device__ __forceinline__ void gram_mainloop_dma(const GramOperands &op, int64_t m0, int64_t n0, int k0, int k1,
                                                   ivec16 (&acc)[2 * P - 1], int8_t *lds) {
+    constexpr int KD = gram_kd<P>(), NS = gram_ns<P>();
+    constexpr int RPI = 1024 / KD;   // rows per load instruction (1 KB)
+    constexpr int LPR = KD / 16;     // lanes per row
     constexpr int WAVES = NTHR / 64;
-    constexpr int CHUNKS = P * (TM + TN) / 16;
+    constexpr int CHUNKS = P * (TM + TN) / RPI;
     // load instructions per wave and stage: PER_WAVE_HI for waves below CHUNKS % WAVES, else PER_WAVE_LO
     constexpr int PER_WAVE_LO = CHUNKS / WAVES;
     constexpr int PER_WAVE_HI = (CHUNKS + WAVES - 1) / WAVES;
@@ -83,12 +95,12 @@ __device__ __forceinline__ void gram_mainloop_dma(const GramOperands &op, int64_
     for (int g = 0; g < 2 * P - 1; ++g)
 #pragma unroll
         for (int r = 0; r < 16; ++r) acc[g][r] = 0;
-    const int nk = (k1 - k0) / TK;
+    const int nk = (k1 - k0) / KD;
     if (nk <= 0) return;
-    const int lrow = lane >> 2, qst = lane & 3;
+    const int lrow = lane / LPR, qst = lane % LPR;
     auto issue = [&](int kt) {
-        int8_t *buf = lds + (size_t)(kt % NSTAGE) * STAGE;
-        const int kbase = k0 + kt * TK;
+        int8_t *buf = lds + (size_t)(kt % NS) * STAGE;
+        const int kbase = k0 + kt * KD;
 #pragma unroll
         for (int u = 0; u < PER_WAVE_HI; ++u) {
             const int ii = wave + WAVES * u;   // chunk of this load instruction (wave-uniform)
@@ -96,56 +108,56 @@ __device__ __forceinline__ void gram_mainloop_dma(const GramOperands &op, int64_
             int p, row;
             int64_t col0;
             int8_t *dst;
-            if (ii < P * TM / 16) {
-                p = ii / (TM / 16);
-                const int rc = ii % (TM / 16);
-                row = 16 * rc + lrow;
+            if (ii < P * TM / RPI) {
+                p = ii / (TM / RPI);
+                const int rc = ii % (TM / RPI);
+                row = RPI * rc + lrow;
                 col0 = m0;
-                dst = buf + (size_t)(p * TM + 16 * rc) * TK;
+                dst = buf + (size_t)(p * TM + RPI * rc) * KD;
             } else {
-                const int jj = ii - P * TM / 16;
-                p = jj / (TN / 16);
-                const int rc = jj % (TN / 16);
-                row = 16 * rc + lrow;
+                const int jj = ii - P * TM / RPI;
+                p = jj / (TN / RPI);
+                const int rc = jj % (TN / RPI);
+                row = RPI * rc + lrow;
                 col0 = n0;
-                dst = buf + (size_t)P * TM * TK + (size_t)(p * TN + 16 * rc) * TK;
+                dst = buf + (size_t)P * TM * KD + (size_t)(p * TN + RPI * rc) * KD;
             }
-            const int qlog = qst ^ ((row >> 2) & 3);
+            const int qlog = qst ^ gram_swz<KD>(row);
             const int8_t *src = op.S + p * op.plane + (col0 + row) * op.ld + kbase + 16 * qlog;
             __builtin_amdgcn_global_load_lds(src, (lds_void *)dst, 16, 0, 0);
         }
     };
-    issue(0);
-    if (nk > 1) issue(1);
+    for (int q = 0; q < NS - 1 && q < nk; ++q) issue(q);
     const int ra = wm * 32 + (lane & 31), rb = wn * 32 + (lane & 31);
     for (int kt = 0; kt < nk; ++kt) {
-        // this wave's loads of stage kt are done when at most the next stage's are outstanding;
-        // the barrier makes every wave's visible and frees the buffer the next issue overwrites
-        if (kt + 1 < nk) {
+        // this wave's loads of stage kt are done when at most those of the stages issued after it (up
+        // to NS - 2) are outstanding; the barrier makes every wave's visible and frees the buffer the
+        // next issue overwrites (stage kt - 1's)
+        if (NS > 2 && kt + 1 < nk) {
             if (wave < CHUNKS % WAVES) wait_vmcnt<PER_WAVE_HI>();
             else wait_vmcnt<PER_WAVE_LO>();
         } else {
             wait_vmcnt<0>();
         }
         lds_barrier();
-        if (kt + 2 < nk) issue(kt + 2);
-        const int8_t *As = lds + (size_t)(kt % NSTAGE) * STAGE;
-        const int8_t *Bs = As + (size_t)P * TM * TK;
+        if (kt + NS - 1 < nk) issue(kt + NS - 1);
+        const int8_t *As = lds + (size_t)(kt % NS) * STAGE;
+        const int8_t *Bs = As + (size_t)P * TM * KD;
         // fragments of both 32-deep halves up front: the second half's reads overlap the first
         // half's MFMAs
-        ivec4 af[TK / 32][P], bf[TK / 32][P];
+        ivec4 af[KD / 32][P], bf[KD / 32][P];
 #pragma unroll
-        for (int h = 0; h < TK / 32; ++h) {
-            const int qa = (2 * h + (lane >> 5)) ^ ((ra >> 2) & 3);
-            const int qb = (2 * h + (lane >> 5)) ^ ((rb >> 2) & 3);
+        for (int h = 0; h < KD / 32; ++h) {
+            const int qa = (2 * h + (lane >> 5)) ^ gram_swz<KD>(ra);
+            const int qb = (2 * h + (lane >> 5)) ^ gram_swz<KD>(rb);
 #pragma unroll
             for (int p = 0; p < P; ++p) {
-                af[h][p] = *reinterpret_cast<const ivec4 *>(As + (size_t)(p * TM + ra) * TK + 16 * qa);
-                bf[h][p] = *reinterpret_cast<const ivec4 *>(Bs + (size_t)(p * TN + rb) * TK + 16 * qb);
+                af[h][p] = *reinterpret_cast<const ivec4 *>(As + (size_t)(p * TM + ra) * KD + 16 * qa);
+                bf[h][p] = *reinterpret_cast<const ivec4 *>(Bs + (size_t)(p * TN + rb) * KD + 16 * qb);
             }
         }
 #pragma unroll
-        for (int h = 0; h < TK / 32; ++h)
+        for (int h = 0; h < KD / 32; ++h)
 #pragma unroll
             for (int p = 0; p < P; ++p)
 #pragma unroll
@@ -263,7 +275,7 @@ int accumulate_split_p(rp_fit *fit, FitDevice &dv, bool keep_phi) {
             if (bi * TM <= bj * TN + TN - 1) tiles.push_back(make_int2(bi, bj));
     const int n_tiles = (int)tiles.size();
     const int splits = cp.splits;
-    if (n_tiles != cp.n_tiles || rows % (splits * TK) != 0)
+    if (n_tiles != cp.n_tiles || rows % (splits * gram_kd<P>()) != 0)
         return ctx->fail(RP_EINVAL, "split path: Gram tiling does not match the chunk plan");
     const int rps = (int)(rows / splits);
     // exactness of the int32 groups: a slice digit is at most 127 (first slice) or 64 (the rest), so
