@@ -215,6 +215,24 @@ int rp_fit_timings(const rp_fit *fit, double *t3);
 int rp_fit_refine_stats(const rp_fit *fit, int32_t *steps, double *last_correction);
 
 /*
+ * Streaming geometry of the basis for n_keep kept points, n_atoms atoms and n_k k-vectors (no
+ * counterpart in the reference, whose chunk size follows free RAM, repeat.py:2196-2207): the chunk
+ * rows, the padded widths, the Gram's row splits per chunk and the rows of one Gram stage.  Every
+ * accumulate allocates and launches from this plan; each row split holds whole stages.
+ */
+typedef struct rp_chunk_plan_info {
+    int64_t rows;                /* kept points per chunk */
+    int64_t n_chunks;
+    int32_t np;                  /* padded basis width: N + V column + ones column, rounded to 128 */
+    int32_t kp;                  /* padded [cos | sin] width of the fp64 k-space GEMM (0: factorized) */
+    int32_t splits;              /* Gram row splits per chunk */
+    int32_t n_tiles;             /* Gram tiles of the upper triangle */
+    int32_t slices;              /* int8 slices per element (0: fp64 Gram) */
+    int32_t stage_rows;          /* rows of one Gram stage (k depth) */
+} rp_chunk_plan_info;
+int rp_chunk_plan(int64_t n_keep, int32_t n_atoms, int32_t n_k, int32_t precision, rp_chunk_plan_info *out);
+
+/*
  * Reload a fit with another system and options (ChargeCalculator.__init__ on a new
  * cube, repeat.py:1736-1786), keeping its device buffers: they only grow, so a
  * fit object can serve a stream of systems without device allocation once it

@@ -36,6 +36,7 @@ EXPORTED = (
     "rp_fit_reset", "rp_server_create", "rp_server_destroy", "rp_server_error", "rp_server_now",
     "rp_request_bytes", "rp_submit", "rp_poll", "rp_server_stats_get",
     "rp_cube_read_header", "rp_cube_read", "rp_parse_values", "rp_fit_qeq", "rp_fit_refine_stats",
+    "rp_chunk_plan",
 )
 KERNEL_KINDS = {"kspace": 0, "gram": 1, "filter": 2, "solve": 3, "phase": 4, "slice": 5, "reduce": 6, "refine": 7}
 
@@ -100,6 +101,12 @@ class RpCubeHeader(ctypes.Structure):
                 ("axis", ctypes.c_double * 9), ("values_offset", ctypes.c_int64)]
 
 
+class RpChunkPlanInfo(ctypes.Structure):
+    _fields_ = [("rows", ctypes.c_int64), ("n_chunks", ctypes.c_int64), ("np", ctypes.c_int32), ("kp", ctypes.c_int32),
+                ("splits", ctypes.c_int32), ("n_tiles", ctypes.c_int32), ("slices", ctypes.c_int32),
+                ("stage_rows", ctypes.c_int32)]
+
+
 _lib = None
 
 
@@ -152,6 +159,8 @@ def load(path: str | None = None):
                                ctypes.c_int]
     L.rp_fit_qeq.argtypes = [vp, dp, dp, ctypes.c_double, dp, dp]
     L.rp_parse_values.argtypes = [ctypes.c_char_p, ctypes.c_int64, dp, ctypes.c_int64, ctypes.c_int, i64p]
+    L.rp_chunk_plan.argtypes = [ctypes.c_int64, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
+                                ctypes.POINTER(RpChunkPlanInfo)]
     for name in EXPORTED:
         if getattr(L, name).restype is None and name not in ("rp_last_error", "rp_server_error", "rp_server_now"):
             getattr(L, name).restype = ctypes.c_int
@@ -161,6 +170,13 @@ def load(path: str | None = None):
 
 def ptr(a, ctype=ctypes.c_double):
     return None if a is None else a.ctypes.data_as(ctypes.POINTER(ctype))
+
+
+def chunk_plan(n_keep: int, n_atoms: int, n_k: int, precision: int) -> dict:
+    """The streaming geometry the accumulate of this size allocates and launches from (host only)."""
+    info = RpChunkPlanInfo()
+    check(load().rp_chunk_plan(int(n_keep), int(n_atoms), int(n_k), int(precision), ctypes.byref(info)))
+    return {name: int(getattr(info, name)) for name, _ in RpChunkPlanInfo._fields_}
 
 
 def check(rc, ctx=None):

@@ -530,4 +530,19 @@ int rp_fit_timings(const rp_fit *fit, double *t3) {
     return RP_OK;
 }
 
+int rp_chunk_plan(int64_t n_keep, int32_t n_atoms, int32_t n_k, int32_t precision, rp_chunk_plan_info *out) {
+    if (!out || n_keep < 1 || n_atoms < 1 || n_k < 0 || precision < RP_PREC_FP64 || precision > RP_PREC_OZAKI_REFINED)
+        return RP_EINVAL;
+    const ChunkPlan c = plan_chunks(n_keep, n_atoms, n_k, precision);
+    out->rows = c.rows;
+    out->n_chunks = ceil_div(n_keep, c.rows);
+    out->np = c.NP;
+    out->kp = c.KP;
+    out->splits = c.splits;
+    out->n_tiles = c.n_tiles;
+    out->slices = precision == RP_PREC_FP64 ? 0 : c.slices;
+    out->stage_rows = precision == RP_PREC_FP64 ? f64_gram_stage_rows : gram_stage_rows(c.slices);
+    return RP_OK;
+}
+
 }  // extern "C"

@@ -24,6 +24,7 @@ namespace rp {
 namespace {
 
 constexpr int BM = 128, BN = 128, BK = 16;
+static_assert(BK == f64_gram_stage_rows, "row splits of the fp64 Gram are planned in BK-row stages");
 
 // A tile (BM x BK) from row-major src (ld), transposed into As[BK][BM]; B tile (BK x BN) row-major into Bs[BK][BN].
 struct TileLoader {

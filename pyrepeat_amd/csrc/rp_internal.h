@@ -144,6 +144,11 @@ struct ChunkPlan {
     int slices = 4;      // split form: int8 slices per Phi element
 };
 
+// Rows of one stage of the split Gram (k_oz_gram) and of the fp64 Gram (k_syrk_f64): a row split
+// must hold a whole number of them.
+constexpr int gram_stage_rows(int slices) { return slices == 3 ? 128 : 64; }
+constexpr int f64_gram_stage_rows = 16;
+
 inline ChunkPlan plan_chunks(int64_t M, int N, int K, int precision) {
     ChunkPlan c;
     c.NP = (int)round_up(N + 2, 128);
@@ -170,7 +175,9 @@ inline ChunkPlan plan_chunks(int64_t M, int N, int K, int precision) {
         for (int bi = 0; bi < c.NP / 64; ++bi)
             for (int bj = 0; bj < c.NP / 128; ++bj)
                 if (bi * 64 <= bj * 128 + 127) ++c.n_tiles;
-        while ((int64_t)c.n_tiles * c.splits < 1024 && c.rows / (c.splits * 2) >= 1024 && (c.rows / (c.splits * 2)) % 64 == 0)
+        // every row split must be whole stages of the Gram: 128 rows deep with 3 slices, 64 with 4
+        const int64_t kd = gram_stage_rows(c.slices);
+        while ((int64_t)c.n_tiles * c.splits < 1024 && c.rows / (c.splits * 2) >= 1024 && (c.rows / (c.splits * 2)) % kd == 0)
             c.splits *= 2;
     }
     return c;

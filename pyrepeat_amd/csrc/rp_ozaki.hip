@@ -52,7 +52,7 @@ typedef __attribute__((address_space(3))) void lds_void;
 // Stage depth (bytes of k per row) and stage buffers: 3 slices take 128-deep stages in two buffers
 // (72 KB each: half the barriers), 4 slices 64-deep stages in three (48 KB each).
 template <int P>
-constexpr int gram_kd() { return P == 3 ? 128 : 64; }
+constexpr int gram_kd() { return gram_stage_rows(P); }
 template <int P>
 constexpr int gram_ns() { return P == 3 ? 2 : 3; }
 template <int P>

@@ -63,6 +63,37 @@ def test_refined_symmetry_resp_matches_oracle(oracle, gpu_ctx, tmp_path, symm_te
     calc.close()
 
 
+@pytest.mark.parametrize("n_atoms,r", [(20, 0), (24, 0), (50, 1), (64, 0)])
+def test_refined_small_serving_cells(oracle, gpu_ctx, tmp_path, n_atoms, r):
+    """Serving-stream cells of 20-64 atoms keep 10-56k points; their 3-slice plans once cut a chunk into
+    row splits of 1088 rows, which the Gram's 128-row stages do not divide, and the fit failed.  The
+    refined charges match the oracle (total charge and symmetry groups of the request included) and
+    the device's fp64 path."""
+    s = synth.stream_request(r, n_atoms=n_atoms)
+    symm = None
+    if s.symm_groups:
+        symm = tmp_path / "symm.txt"
+        symm.write_text(s.symm_text())
+    print(f"refined serving cell: {n_atoms} atoms, request {r}, grid {list(s.n_grid)}, "
+          f"q_tot {s.q_tot}, {len(s.symm_groups)} groups", flush=True)
+    ref = oracle.fit(s.n_grid, s.axis, s.atomic_numbers, s.atom_pos, s.esp_flat(), q_tot=s.q_tot,
+                     symm=oracle.read_symmetry(str(symm)) if symm else None)
+    out = {}
+    for precision in ("refined", "fp64"):
+        calc = _calc(gpu_ctx, s, precision, tmp_path)
+        out[precision] = calc.fit_charges()
+        plan = _abi.chunk_plan(calc._fit.info.n_keep, s.n_atoms, calc._fit.info.n_k,
+                               _abi.RP_PREC_OZAKI_REFINED)
+        calc.close()
+    print(f"  plan {plan}", flush=True)
+    dq = np.max(np.abs(out["refined"] - ref.charges))
+    print(f"  max|dq| refined vs oracle {dq:.2e}, vs fp64 {np.max(np.abs(out['refined'] - out['fp64'])):.2e}",
+          flush=True)
+    assert dq <= 1e-6
+    assert np.max(np.abs(out["refined"] - out["fp64"])) <= 1e-8
+    assert abs(out["refined"].sum() - s.q_tot) < 1e-9
+
+
 def test_refined_deterministic_and_basis_kept(gpu_ctx, tmp_path):
     """Two fits are bitwise equal; the --stats matvec (its own chunk buffer) between two solves leaves
     the kept basis intact, so the second solve repeats the first bitwise; refit does the same."""

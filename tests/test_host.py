@@ -134,6 +134,32 @@ def test_request_bound_on_host_and_invalid_requests():
     assert e.value.code == _abi.RP_EINVAL
 
 
+@pytest.mark.parametrize("precision", [_abi.RP_PREC_FP64, _abi.RP_PREC_OZAKI, _abi.RP_PREC_OZAKI_REFINED])
+def test_chunk_plan_row_splits_hold_whole_gram_stages(precision):
+    """Every row split of every plan is whole stages of its Gram kernel (128 rows for 3 slices,
+    64 for 4, 16 for fp64) and short enough for exact int32 slice groups.  The kept counts of small
+    serving cells (10574, 17223, 31782, 47322 for 20-64 atoms) once gave 3-slice plans with
+    1088-row splits that the Gram rejected."""
+    counts = sorted(set(range(512, 70001, 512)) | set(range(1, 70001, 997)) |
+                    {10574, 17223, 31782, 47322, 8704, 97963, 57297, 252456, 943000})
+    for n_atoms in (4, 20, 50, 64, 100, 199, 200, 500, 1000, 1600, 5000):
+        for m in counts:
+            c = _abi.chunk_plan(m, n_atoms, 754, precision)
+            rps = c["rows"] // c["splits"]
+            assert c["rows"] % (c["splits"] * c["stage_rows"]) == 0, (n_atoms, m, c)
+            assert rps >= c["stage_rows"] and c["n_chunks"] * c["rows"] >= m
+            assert c["np"] % 128 == 0 and c["np"] >= n_atoms + 2
+            if precision != _abi.RP_PREC_FP64:
+                p = c["slices"]
+                assert p == (3 if precision == _abi.RP_PREC_OZAKI_REFINED else 4)
+                assert c["rows"] % 512 == 0
+                # a group's per-row sum is at most 2*127*64 + (P-2)*64*64; rps of them fit in int32
+                assert (2 * 127 * 64 + (p - 2) * 64 * 64) * rps <= 2**31 - 1
+    assert _abi.chunk_plan(10574, 20, 754, _abi.RP_PREC_OZAKI_REFINED)["rows"] % 128 == 0
+    with pytest.raises(_abi.RepeatHipError):
+        _abi.chunk_plan(0, 20, 754, precision)
+
+
 def test_stream_splitting_sums_the_same_potential():
     """The serving stream's Ewald splitting (eta 0.6, Rc 8, |k| <= 5) gives the potential of the
     default splitting (eta 0.3, Rc 20, |k| <= 2.5): Ewald sums do not depend on eta once converged."""
