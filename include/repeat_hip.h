@@ -203,6 +203,23 @@ int rp_fit_phi_matvec(rp_fit *fit, const double *q, double *v_coul);
  */
 int rp_fit_phi(rp_fit *fit, int64_t m, const double *grid_xyz, double *phi_out);
 
+/*
+ * rp_fit_phi without device allocation (no counterpart in the reference).  For callers that cannot
+ * allocate device memory: code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc.  The same basis rows, bitwise, as rp_fit_phi.  Everything on the device goes through a
+ * static 16 MiB array per GPU (held until the process ends; its address is looked up when a context is
+ * created on that GPU): the fit's k-set, pruned shifts and atoms, then the points in blocks of
+ * about 2M / (N + 3) rows.  Limits: N <= RP_PHI_NOALLOC_MAX_ATOMS, K <= RP_PHI_NOALLOC_MAX_K,
+ * pruned shifts <= RP_PHI_NOALLOC_MAX_SHIFTS; beyond them RP_EINVAL with the limits in the message.
+ * Every call on one GPU uses the same array, so calls that overlap on a GPU (two contexts or threads)
+ * race and must be serialised by the caller.
+ */
+#define RP_PHI_NOALLOC_SCRATCH_BYTES (16u << 20)
+#define RP_PHI_NOALLOC_MAX_ATOMS 4096
+#define RP_PHI_NOALLOC_MAX_K 16384
+#define RP_PHI_NOALLOC_MAX_SHIFTS 4096
+int rp_fit_phi_noalloc(rp_fit *fit, int64_t m, const double *grid_xyz, double *phi_out);
+
 /* Device time (ms) of the phases of the last accumulate / solve on this fit:
  * t[0] filter, t[1] accumulate, t[2] solve. */
 int rp_fit_timings(const rp_fit *fit, double *t3);

@@ -36,7 +36,7 @@ EXPORTED = (
     "rp_fit_reset", "rp_server_create", "rp_server_destroy", "rp_server_error", "rp_server_now",
     "rp_request_bytes", "rp_submit", "rp_poll", "rp_server_stats_get",
     "rp_cube_read_header", "rp_cube_read", "rp_parse_values", "rp_fit_qeq", "rp_fit_refine_stats",
-    "rp_chunk_plan",
+    "rp_chunk_plan", "rp_fit_phi_noalloc",
 )
 KERNEL_KINDS = {"kspace": 0, "gram": 1, "filter": 2, "solve": 3, "phase": 4, "slice": 5, "reduce": 6, "refine": 7}
 
@@ -137,6 +137,7 @@ def load(path: str | None = None):
     L.rp_fit_solve.argtypes = [vp, ctypes.POINTER(RpConstraints), dp, dp, ctypes.POINTER(ctypes.c_int32)]
     L.rp_fit_phi_matvec.argtypes = [vp, dp, dp]
     L.rp_fit_phi.argtypes = [vp, ctypes.c_int64, dp, dp]
+    L.rp_fit_phi_noalloc.argtypes = [vp, ctypes.c_int64, dp, dp]
     L.rp_fit_timings.argtypes = [vp, dp]
     L.rp_fit_refine_stats.argtypes = [vp, ctypes.POINTER(ctypes.c_int32), dp]
     L.rp_ctx_profile.argtypes = [vp, ctypes.c_int]
@@ -414,6 +415,13 @@ class Fit:
         g = np.ascontiguousarray(grid_xyz, dtype=np.float64).reshape(-1, 3)
         out = np.empty((len(g), self.n_atoms))
         check(load().rp_fit_phi(self.handle, len(g), ptr(g), ptr(out)), self.ctx.handle)
+        return out
+
+    def phi_noalloc(self, grid_xyz):
+        """``phi`` through the entry point that allocates no device memory (static per-GPU scratch)."""
+        g = np.ascontiguousarray(grid_xyz, dtype=np.float64).reshape(-1, 3)
+        out = np.empty((len(g), self.n_atoms))
+        check(load().rp_fit_phi_noalloc(self.handle, len(g), ptr(g), ptr(out)), self.ctx.handle)
         return out
 
     def qeq(self, chi, j_self, q_tot=0.0, want_j=False):

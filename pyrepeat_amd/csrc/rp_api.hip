@@ -86,6 +86,51 @@ __global__ __launch_bounds__(256) void k_zero(double *p, int64_t n) {
 
 }  // namespace
 
+}  // namespace rp
+
+// Working memory of rp_fit_phi_noalloc: one copy per GPU, held until the process ends, so that entry
+// point allocates nothing.  Host code never passes this name to a kernel or a copy (on the host it is
+// a stand-in, not a device address): it uses the address hipGetSymbolAddress returns for the device.
+static __device__ __align__(256) double scratch[RP_PHI_NOALLOC_SCRATCH_BYTES / sizeof(double)];
+
+namespace rp {
+
+namespace {
+std::mutex g_scratch_mu;
+std::vector<double *> g_scratch_ptr;   // per device: the address of `scratch` there (nullptr: not looked up)
+std::vector<std::string> g_scratch_err;
+}  // namespace
+
+// Looks up the device address of the no-alloc scratch on `device` (which must be current), once per device.
+void noalloc_scratch_init(int device) {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if ((int)g_scratch_ptr.size() <= device) {
+        g_scratch_ptr.resize(device + 1, nullptr);
+        g_scratch_err.resize(device + 1);
+    }
+    if (g_scratch_ptr[device]) return;
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+        g_scratch_err[device] = std::string("hipGetSymbolAddress(scratch) failed: ") + hipGetErrorString(e);
+        return;
+    }
+    g_scratch_ptr[device] = static_cast<double *>(p);
+    g_scratch_err[device].clear();
+}
+
+// The scratch address on `device`, or nullptr with the reason in err.
+double *noalloc_scratch(int device, std::string &err) {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (device < 0 || device >= (int)g_scratch_ptr.size() || !g_scratch_ptr[device]) {
+        err = device >= 0 && device < (int)g_scratch_err.size() && !g_scratch_err[device].empty()
+                  ? g_scratch_err[device]
+                  : std::string("no-alloc scratch not set up on this device");
+        return nullptr;
+    }
+    return g_scratch_ptr[device];
+}
+
 int prepare_device(rp_fit *fit, FitDevice &dv, int precision) {
     Ctx *ctx = fit->ctx;
     hipStream_t st = ctx->stream;
@@ -245,6 +290,61 @@ int run_phi_points(rp_fit *fit, int64_t m, const double *grid_xyz, double *phi_o
     return RP_OK;
 }
 
+// rp_fit_phi without device allocation: the k-set, the pruned shifts and the atoms are uploaded into the
+// static scratch, then the points go through it in blocks (points in, basis rows out), each block the
+// same k_phi_points launch as rp_fit_phi's, so the rows are bitwise the same.
+int run_phi_points_noalloc(rp_fit *fit, int64_t m, const double *grid_xyz, double *phi_out) {
+    Ctx *ctx = fit->ctx;
+    hipStream_t st = ctx->stream;
+    const Setup &s = fit->setup;
+    const int N = fit->n_atoms;
+    const int K = s.fit_type == 1 ? (int)s.kcoefs.size() : 0;
+    const int S = (int)(fit->shifts_pruned.size() / 3);
+    if (N > RP_PHI_NOALLOC_MAX_ATOMS || K > RP_PHI_NOALLOC_MAX_K || S > RP_PHI_NOALLOC_MAX_SHIFTS)
+        return ctx->fail(RP_EINVAL, "rp_fit_phi_noalloc: " + std::to_string(N) + " atoms, " + std::to_string(K) +
+                                        " k-vectors, " + std::to_string(S) + " shifts exceed the static scratch (at most " +
+                                        std::to_string(RP_PHI_NOALLOC_MAX_ATOMS) + ", " + std::to_string(RP_PHI_NOALLOC_MAX_K) +
+                                        ", " + std::to_string(RP_PHI_NOALLOC_MAX_SHIFTS) + ")");
+    std::string err;
+    double *base = noalloc_scratch(ctx->device, err);
+    if (!base) return ctx->fail(RP_ESTATE, "rp_fit_phi_noalloc: " + err);
+    // layout in doubles, every region on a 256-byte boundary: kvecs 3K | kcoef K | shifts 3S | atoms 3N |
+    // points 3B | basis rows B N
+    const int64_t cap = RP_PHI_NOALLOC_SCRATCH_BYTES / sizeof(double);
+    const int64_t o_kc = round_up(3 * (int64_t)K, 32), o_sh = o_kc + round_up(K, 32);
+    const int64_t o_at = o_sh + round_up(3 * (int64_t)S, 32), o_pt = o_at + round_up(3 * (int64_t)N, 32);
+    const int64_t B = (cap - o_pt - 32) / (3 + N);   // round_up(3B, 32) + B N <= cap - o_pt
+    const int64_t o_out = o_pt + round_up(3 * B, 32);
+    if (B < 1 || o_out + B * N > cap) return ctx->fail(RP_EINVAL, "rp_fit_phi_noalloc: static scratch too small");
+    if (K > 0) {
+        RP_HIP(ctx, hipMemcpyAsync(base, s.kvecs.data(), sizeof(double) * 3 * K, hipMemcpyHostToDevice, st));
+        RP_HIP(ctx, hipMemcpyAsync(base + o_kc, s.kcoefs.data(), sizeof(double) * K, hipMemcpyHostToDevice, st));
+    }
+    if (S > 0)
+        RP_HIP(ctx, hipMemcpyAsync(base + o_sh, fit->shifts_pruned.data(), sizeof(double) * 3 * S, hipMemcpyHostToDevice, st));
+    RP_HIP(ctx, hipMemcpyAsync(base + o_at, fit->atom_xyz.data(), sizeof(double) * 3 * N, hipMemcpyHostToDevice, st));
+    EpiArgs e{};
+    e.N = N;
+    e.fit_type = s.fit_type;
+    e.atoms = base + o_at;
+    e.shifts = base + o_sh;
+    e.S = S;
+    e.R2 = s.r_cutoff * s.r_cutoff;
+    e.sqrt_alpha = s.sqrt_alpha;
+    e.r_cut = s.r_cutoff;
+    for (int64_t p0 = 0; p0 < m; p0 += B) {
+        const int64_t nb = std::min<int64_t>(B, m - p0);
+        // stream order: this block's upload follows the previous block's download of the rows
+        RP_HIP(ctx, hipMemcpyAsync(base + o_pt, grid_xyz + 3 * p0, sizeof(double) * 3 * nb, hipMemcpyHostToDevice, st));
+        k_phi_points<<<ceil_div(nb * N, 256), 256, 0, st>>>(nb, base + o_pt, N, base + o_at, K, base, base + o_kc, e,
+                                                             base + o_out);
+        RP_HIP(ctx, hipGetLastError());
+        RP_HIP(ctx, hipMemcpyAsync(phi_out + p0 * N, base + o_out, sizeof(double) * nb * N, hipMemcpyDeviceToHost, st));
+    }
+    RP_HIP(ctx, hipStreamSynchronize(st));
+    return RP_OK;
+}
+
 // Validates a system, derives the host setup, uploads atoms and the cube, and runs the vdW filter.
 // Reuses the fit's device buffers (grow-only), so reloading a fit with a system of equal or smaller
 // size allocates nothing.
@@ -317,6 +417,7 @@ int rp_ctx_create(int device, uint64_t hbm_budget_bytes, rp_ctx **out) {
         set_global_error(std::string("hipSetDevice failed: ") + hipGetErrorString(e));
         return RP_EHIP;
     }
+    noalloc_scratch_init(device);   // the device address of rp_fit_phi_noalloc's static scratch
     rp_ctx *c = new rp_ctx();
     c->device = device;
     c->budget = hbm_budget_bytes;
@@ -512,6 +613,12 @@ int rp_fit_phi(rp_fit *fit, int64_t m, const double *grid_xyz, double *phi_out) 
     if (!fit || m < 0 || (m > 0 && (!grid_xyz || !phi_out))) return RP_EINVAL;
     RP_HIP(fit->ctx, hipSetDevice(fit->ctx->device));
     return run_phi_points(fit, m, grid_xyz, phi_out);
+}
+
+int rp_fit_phi_noalloc(rp_fit *fit, int64_t m, const double *grid_xyz, double *phi_out) {
+    if (!fit || m < 0 || (m > 0 && (!grid_xyz || !phi_out))) return RP_EINVAL;
+    RP_HIP(fit->ctx, hipSetDevice(fit->ctx->device));
+    return run_phi_points_noalloc(fit, m, grid_xyz, phi_out);
 }
 
 int rp_fit_refine_stats(const rp_fit *fit, int32_t *steps, double *last_correction) {
