@@ -16,6 +16,7 @@ import pytest
 from pyrepeat_amd import synth, tables
 
 REF = "/root/reference/repeat.py"
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
 # ------------------------------------------------------------------ tables
@@ -30,6 +31,63 @@ def test_vdw_table_matches_reference_text():
     m2 = re.search(r"ATOM_SYMBOLS = \[(.*?)\]", src, re.S)
     syms = [s.strip().strip('"') for s in m2.group(1).replace("\n", " ").split(",") if s.strip()]
     assert tuple(syms) == tables.ATOM_SYMBOLS
+
+
+def _cpp_vdw_table():
+    src = open(os.path.join(ROOT, "pyrepeat_amd", "csrc", "rp_setup.cpp")).read()
+    m = re.search(r"const double kVdw\[(\d+)\] = \{(.*?)\};", src, re.S)
+    vals = [float(x) for x in m.group(2).replace("\n", " ").split(",") if x.strip()]
+    return int(m.group(1)), np.array(vals)
+
+
+def test_device_vdw_table_matches_host_table():
+    """The library's own radius table (the filter's thresholds come from it) holds the same 107 values
+    as the host table, which the next test ties to the reference text."""
+    n, vals = _cpp_vdw_table()
+    assert n == 107 and len(vals) == 107
+    np.testing.assert_array_equal(vals, tables.VDW_RADII_BOHR)
+
+
+@pytest.mark.skipif(not os.path.exists(REF), reason="reference source not mounted")
+def test_device_vdw_table_matches_reference_text():
+    src = open(REF).read()
+    m = re.search(r"VDW_RADII = np.array\(\[(.*?)\]", src, re.S)
+    ref = np.array([float(x) for x in m.group(1).replace("\n", " ").split(",") if x.strip()])
+    np.testing.assert_array_equal(_cpp_vdw_table()[1], ref)
+
+
+@pytest.mark.skipif(not os.path.exists(REF), reason="reference source not mounted")
+def test_qeq_default_table_matches_reference_text():
+    """DEFAULT_QEQ_PARAMS (repeat.py:611-650), parsed from the reference's text with dict-literal
+    semantics (a repeated key keeps its last value), equals the table the QEq paths use."""
+    src = open(REF).read()
+    m = re.search(r"DEFAULT_QEQ_PARAMS = \{(.*?)\n\}", src, re.S)
+    ref = {}
+    for sym, chi, hard in re.findall(r'"([A-Z][a-z]?)":\s*\(\s*([-0-9.eE+]+)\s*,\s*([-0-9.eE+]+)\s*\)', m.group(1)):
+        ref[sym] = (float(chi), float(hard))
+    assert len(ref) == 65
+    assert ref == dict(tables.DEFAULT_QEQ_PARAMS)
+
+
+@pytest.mark.parametrize("trees,n,expected", [
+    # one tree: linked atom 2 is dropped from the independent list, then copies its base
+    ([{"base": 0, "linked": [2]}], 5, [10, 11, 10, 12, 13]),
+    # two trees: independents 0, 1, 2, 5 take g0..g3; linked atoms copy their bases
+    ([{"base": 1, "linked": [4, 6]}, {"base": 0, "linked": [3]}], 7, [10, 11, 12, 10, 11, 13, 11]),
+    # a linked atom that is the base of a later tree: the copies run in tree order
+    ([{"base": 0, "linked": [2]}, {"base": 2, "linked": [4]}], 5, [10, 11, 10, 12, 10]),
+    # the same trees in the other order: atom 4 copies atom 2 before atom 2 is set, so stays 0
+    ([{"base": 2, "linked": [4]}, {"base": 0, "linked": [2]}], 5, [10, 11, 10, 12, 0]),
+])
+def test_expand_sum_phi_known_answers(oracle, trees, n, expected):
+    """_expand_sum_phi (repeat.py:2491-2509) as fit_charges calls it (repeat.py:2317-2320): with the
+    unreduced sum_phi g = (10, 11, 12, ...), scattered as if it were reduced.  Expected vectors are
+    worked by hand from the reference's loops; the oracle and the product's stats path must both give
+    them."""
+    from pyrepeat_amd.calculator import _expand_sum_phi_quirk
+    g = np.arange(10.0, 10.0 + n)
+    np.testing.assert_array_equal(oracle.expand_sum_phi(g, trees, n), np.array(expected, dtype=float))
+    np.testing.assert_array_equal(_expand_sum_phi_quirk(g, {"trees": trees}, n), np.array(expected, dtype=float))
 
 
 def test_vdw_lookup_index_rules():
