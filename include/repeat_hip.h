@@ -52,12 +52,20 @@ extern "C" {
                             refinement of the solve: a step forms the residual of the fp64
                             normal equations, Phi^T (V - Phi q) less the centring term, from
                             the fp64 basis kept in HBM (M x (N + 2) doubles), and corrects q
-                            and the multiplier with the Cholesky factor of the split system.
-                            The result converges to the fp64 system's solution.  Steps stop
-                            when a correction is at most RP_REFINE_STOP_E (max |dq|, e): a step
-                            shrinks the error by rho <= 5.3e-3 on the systems measured (cond(A)
-                            up to 4.9e10), so the error left is at most ~2e-6 e. */
-#define RP_REFINE_STOP_E 3e-4
+                            and the multiplier with the factors of the split system.  The result
+                            converges to the fp64 system's solution.  After each step the error
+                            left is bounded by rho / (1 - rho) * max |dq|, with rho the ratio of
+                            the last two corrections (RP_REFINE_RHO_PRIOR after the first); steps
+                            run while that bound exceeds RP_REFINE_TARGET_E, at most
+                            rp_options.refine_steps (rp_fit_solve_report gives the bound). */
+#define RP_REFINE_TARGET_E 1e-6
+#define RP_REFINE_RHO_PRIOR 1e-2     /* twice the largest contraction measured (5.3e-3, cond(A) 4.9e10) */
+#define RP_REFINE_STEPS_DEFAULT 4
+
+/* How rp_fit_solve solved the bordered system (rp_fit_solve's *used_fallback, rp_solve_report.solver). */
+#define RP_SOLVER_CHOLESKY 0   /* Cholesky of A and the Schur complement of the Lagrange row */
+#define RP_SOLVER_LU 1         /* LU with partial pivoting (a non-positive Cholesky pivot, or a wait that gave up) */
+#define RP_SOLVER_LSTSQ 2      /* exactly singular: minimum-norm least squares (repeat.py:2306-2310) */
 
 typedef struct rp_ctx rp_ctx;
 typedef struct rp_fit rp_fit;
@@ -82,7 +90,7 @@ typedef struct rp_options {
     double vdw_max;              /* --vdw-max, default 1000.0 */
     double r_cutoff;             /* --cutoff, default 20.0 */
     int32_t refine_steps;        /* RP_PREC_OZAKI_REFINED: at most this many refinement steps
-                                    (<= 0: 2); no counterpart in the reference */
+                                    (<= 0: RP_REFINE_STEPS_DEFAULT); no counterpart in the reference */
     int32_t reserved;            /* 0 */
 } rp_options;
 
@@ -172,15 +180,17 @@ int rp_fit_normal_equations(const rp_fit *fit, double *ata, double *atb, double 
 
 /*
  * fit_charges steps 4-5 (repeat.py:2266-2313) on the device: centring, symmetry
- * merge, RESP/QEq diagonal terms, bordered [[A, w], [w^T, 0]] LU solve with
- * partial pivoting, expansion to all atoms.  q_out: N charges; solution_out
- * (N'+1, nullable): reduced charges and the Lagrange multiplier.
- * Returns RP_ESINGULAR if an LU pivot is exactly zero (the reference then
- * falls back to lstsq).  *used_fallback (nullable) is 1 when the system went
- * through the LU with partial pivoting (a non-positive Cholesky pivot, or a
- * backward-substitution wait that gave up), 0 when the Cholesky solved it.
- * Under RP_PREC_OZAKI_REFINED the Cholesky solution is refined against the fp64
- * basis kept by rp_fit_accumulate (an LU fallback is not refined).
+ * merge, RESP/QEq diagonal terms, bordered [[A, w], [w^T, 0]] solve, expansion to
+ * all atoms.  q_out: N charges; solution_out (N'+1, nullable): reduced charges and
+ * the Lagrange multiplier.  *used_fallback (nullable) is the RP_SOLVER_* that solved
+ * it: the Cholesky; LU with partial pivoting when a Cholesky pivot is not positive (or
+ * a triangular-solve wait gave up); when an LU pivot is exactly zero, the minimum-norm
+ * least-squares solution, as the reference's np.linalg.lstsq(rcond=None) branch
+ * (repeat.py:2306-2310, the same warning on stderr): a Jacobi eigendecomposition of
+ * the symmetric bordered matrix, eigenvalues at most 2^-52 (N'+1) max|l| dropped.
+ * Under RP_PREC_OZAKI_REFINED the solution of any of the three is refined against
+ * the fp64 basis kept by rp_fit_accumulate, each correction solved with the same
+ * factors.
  */
 int rp_fit_solve(rp_fit *fit, const rp_constraints *con, double *q_out, double *solution_out,
                  int32_t *used_fallback);
@@ -227,9 +237,20 @@ int rp_fit_timings(const rp_fit *fit, double *t3);
 /*
  * Iterative refinement of the last rp_fit_solve (RP_PREC_OZAKI_REFINED; no counterpart in the
  * reference, whose solve is fp64 throughout): the steps that ran and max |dq| (e) of the last
- * correction applied.  0 and 0.0 for the other precisions and after an LU fallback.
+ * correction applied.  0 and 0.0 for the other precisions.
  */
 int rp_fit_refine_stats(const rp_fit *fit, int32_t *steps, double *last_correction);
+
+/* How the last rp_fit_solve went (no counterpart in the reference). */
+typedef struct rp_solve_report {
+    int32_t solver;              /* RP_SOLVER_* */
+    int32_t refine_steps;        /* refinement steps run (RP_PREC_OZAKI_REFINED) */
+    double last_correction;      /* max |dq| of the last correction (e) */
+    double error_bound;          /* refined: estimated max |q - q_fp64| left (e); -1: no estimate (other forms) */
+    int32_t within_target;       /* refined: error_bound <= RP_REFINE_TARGET_E; other forms: 1 */
+    int32_t reserved;
+} rp_solve_report;
+int rp_fit_solve_report(const rp_fit *fit, rp_solve_report *out);
 
 /*
  * Streaming geometry of the basis for n_keep kept points, n_atoms atoms and n_k k-vectors (no

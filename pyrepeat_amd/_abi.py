@@ -36,8 +36,10 @@ EXPORTED = (
     "rp_fit_reset", "rp_server_create", "rp_server_destroy", "rp_server_error", "rp_server_now",
     "rp_request_bytes", "rp_submit", "rp_poll", "rp_server_stats_get",
     "rp_cube_read_header", "rp_cube_read", "rp_parse_values", "rp_fit_qeq", "rp_fit_refine_stats",
-    "rp_chunk_plan", "rp_fit_phi_noalloc",
+    "rp_chunk_plan", "rp_fit_phi_noalloc", "rp_fit_solve_report",
 )
+RP_SOLVER_CHOLESKY, RP_SOLVER_LU, RP_SOLVER_LSTSQ = 0, 1, 2
+RP_REFINE_TARGET_E = 1e-6
 KERNEL_KINDS = {"kspace": 0, "gram": 1, "filter": 2, "solve": 3, "phase": 4, "slice": 5, "reduce": 6, "refine": 7}
 
 
@@ -101,6 +103,11 @@ class RpCubeHeader(ctypes.Structure):
                 ("axis", ctypes.c_double * 9), ("values_offset", ctypes.c_int64)]
 
 
+class RpSolveReport(ctypes.Structure):
+    _fields_ = [("solver", ctypes.c_int32), ("refine_steps", ctypes.c_int32), ("last_correction", ctypes.c_double),
+                ("error_bound", ctypes.c_double), ("within_target", ctypes.c_int32), ("reserved", ctypes.c_int32)]
+
+
 class RpChunkPlanInfo(ctypes.Structure):
     _fields_ = [("rows", ctypes.c_int64), ("n_chunks", ctypes.c_int64), ("np", ctypes.c_int32), ("kp", ctypes.c_int32),
                 ("splits", ctypes.c_int32), ("n_tiles", ctypes.c_int32), ("slices", ctypes.c_int32),
@@ -138,6 +145,7 @@ def load(path: str | None = None):
     L.rp_fit_phi_matvec.argtypes = [vp, dp, dp]
     L.rp_fit_phi.argtypes = [vp, ctypes.c_int64, dp, dp]
     L.rp_fit_phi_noalloc.argtypes = [vp, ctypes.c_int64, dp, dp]
+    L.rp_fit_solve_report.argtypes = [vp, ctypes.POINTER(RpSolveReport)]
     L.rp_fit_timings.argtypes = [vp, dp]
     L.rp_fit_refine_stats.argtypes = [vp, ctypes.POINTER(ctypes.c_int32), dp]
     L.rp_ctx_profile.argtypes = [vp, ctypes.c_int]
@@ -394,8 +402,17 @@ class Fit:
         sol = np.empty(nr + 1)
         fb = ctypes.c_int32(0)
         check(load().rp_fit_solve(self.handle, ctypes.byref(c), ptr(q), ptr(sol), ctypes.byref(fb)), self.ctx.handle)
-        self.last_solve_used_lu = bool(fb.value)
+        self.last_solver = int(fb.value)
+        self.last_solve_used_lu = self.last_solver != RP_SOLVER_CHOLESKY
         return q, sol
+
+    def solve_report(self):
+        """How the last solve went: solver (RP_SOLVER_*), refinement steps, last correction, error bound."""
+        r = RpSolveReport()
+        check(load().rp_fit_solve_report(self.handle, ctypes.byref(r)), self.ctx.handle)
+        return {"solver": int(r.solver), "refine_steps": int(r.refine_steps),
+                "last_correction": float(r.last_correction), "error_bound": float(r.error_bound),
+                "within_target": bool(r.within_target)}
 
     def refit(self, con, c_prepared=None):
         """Whole fit (filter, accumulate, solve) from the device-resident cube."""

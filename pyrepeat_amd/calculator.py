@@ -24,6 +24,7 @@ Every computation runs in ``librepeat_hip.so``; nothing falls back to the CPU.
 """
 from __future__ import annotations
 
+import sys
 import time
 
 import numpy as np
@@ -89,9 +90,28 @@ class ChargeCalculator:
         return constraint_arrays(self.n_atoms, self.cube["atom_index"], self.q_tot, self.symm_data,
                                  self.resp_data, self.qeq_data)
 
+    # A refined fit whose error bound stays above this after its steps is redone in fp64.
+    REFINED_BOUND_LIMIT_E = 1e-5
+
     def fit_charges(self):
         self._fit.accumulate()
         q, sol = self._fit.solve(self.constraints())
+        self.solve_report = self._fit.solve_report()
+        bound = self.solve_report["error_bound"]
+        if self.precision == "refined" and not bound <= self.REFINED_BOUND_LIMIT_E:
+            # the refinement did not converge far enough within its steps: the fp64 path's solve
+            # (Gram in fp64) of the same system instead
+            print(f"Warning: refinement left an error bound of {bound:.2e} e after "
+                  f"{self.solve_report['refine_steps']} steps; solving with the fp64 path.", file=sys.stderr)
+            self._fit.close()
+            self._fit = _abi.Fit(self.ctx, self.cube["n_grid"], self.cube["axis_vector"], self.cube["atom_index"],
+                                 self.cube["atom_pos"], self.cube["esp_cube"],
+                                 self.cube.get("esp_order", _abi.RP_ORDER_CUBE), self.fit_flag, _abi.RP_PREC_FP64,
+                                 self.vdw_factor, self.vdw_max, self.R_cutoff)
+            self._grid = None
+            self._fit.accumulate()
+            q, sol = self._fit.solve(self.constraints())
+            self.solve_report = dict(self._fit.solve_report(), refined_bound_e=bound)
         _, _, sum_phi, sum_v = self._fit.normal_equations()
         # The reference scatters the *unreduced* sum_phi as if it were reduced (repeat.py:2317-2320, 2491-2509).
         self._sum_phi = _expand_sum_phi_quirk(sum_phi, self.symm_data, self.n_atoms) if self.symm_data else sum_phi

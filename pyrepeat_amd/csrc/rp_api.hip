@@ -369,6 +369,8 @@ int load_system(rp_fit *fit, const rp_system *sys, const rp_options *opt) {
     fit->phi_kept = false;
     fit->refine_steps_done = 0;
     fit->refine_last_dq = 0.0;
+    fit->refine_bound = -1.0;
+    fit->solver = RP_SOLVER_CHOLESKY;
     fit->na = 0;
     if (fit->dev) fit->dev->ready = false;
     RP_CHECK(build_setup(sys, opt, fit->setup, ctx->err));
@@ -625,6 +627,18 @@ int rp_fit_refine_stats(const rp_fit *fit, int32_t *steps, double *last_correcti
     if (!fit || !steps || !last_correction) return RP_EINVAL;
     *steps = fit->refine_steps_done;
     *last_correction = fit->refine_last_dq;
+    return RP_OK;
+}
+
+int rp_fit_solve_report(const rp_fit *fit, rp_solve_report *out) {
+    if (!fit || !out) return RP_EINVAL;
+    const bool refined = fit->opt.precision == RP_PREC_OZAKI_REFINED;
+    out->solver = fit->solver;
+    out->refine_steps = fit->refine_steps_done;
+    out->last_correction = fit->refine_last_dq;
+    out->error_bound = refined ? fit->refine_bound : -1.0;
+    out->within_target = refined ? (fit->refine_bound >= 0.0 && fit->refine_bound <= RP_REFINE_TARGET_E) : 1;
+    out->reserved = 0;
     return RP_OK;
 }
 

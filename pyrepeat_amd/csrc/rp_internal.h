@@ -291,6 +291,8 @@ struct rp_fit {
     int64_t phi_rows = 0;
     int32_t refine_steps_done = 0;
     double refine_last_dq = 0.0;
+    double refine_bound = -1.0;     // estimated error left by the refinement (e), -1: none
+    int32_t solver = RP_SOLVER_CHOLESKY;   // how the last solve went
     rp::FitDevice *dev = nullptr;   // persistent device copy of the Ewald setup
     ~rp_fit();
 };

@@ -146,6 +146,9 @@ uint64_t fit_bytes_bound(const Setup &s, int n_atoms, int esp_dtype, int precisi
     const uint64_t nr = n_reduced, n = nr + 1, ld = (uint64_t)round_up((int64_t)n + 1, 4);
     const uint64_t nsrc = std::max(n_src, 1);
     at(12, 8 * n * ld);
+    // least-squares fallback of an exactly singular system: eigenvectors, round pairs and vectors
+    at(11, 8 * n * n);
+    at(14, (uint64_t)round_up((int64_t)(48 * ((n + 1) / 2)), 256) + 8 * (2 * n + 4) + 256);
     // packed constraint arrays: ints (red_ptr, red_src, expand_src, inv_ptr, inv_set), then doubles
     // (red_mult, weights, diag_add, rhs_add, inv_mult)
     at(13, (uint64_t)round_up((int64_t)(round_up((int64_t)(4 * ((nr + 1) + 2 * nsrc + 2 * N + 1)), 8) +

@@ -47,7 +47,12 @@ struct RefineState {
     int go;
     int steps;
     double last_dq;
+    double bound;   // estimated max |q - q_converged| left after the last step (e); -1 before the first
 };
+
+// Contraction assumed for the bound after a single step, where no ratio of corrections exists yet:
+// twice the largest per-step contraction measured (5.3e-3, a 5000-atom cell with cond(A) = 4.9e10).
+constexpr double RHO_PRIOR = 1e-2;
 
 __global__ __launch_bounds__(256) void k_assemble(const double *__restrict__ gram, int NP, int N,
                                                   const int *__restrict__ red_ptr, const int *__restrict__ red_src,
@@ -64,6 +69,7 @@ __global__ __launch_bounds__(256) void k_assemble(const double *__restrict__ gra
         rs->go = 1;
         rs->steps = 0;
         rs->last_dq = 0.0;
+        rs->bound = -1.0;
     }
     if (e >= (int64_t)n * (n + 1)) return;
     const int I = (int)(e / (n + 1)), J = (int)(e % (n + 1));
@@ -1166,9 +1172,12 @@ __global__ __launch_bounds__(256) void k_fwdsub(const double *__restrict__ A, in
     if (t == 0) __hip_atomic_store(flags + kb, epoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// x += dx (charges and multiplier); the correction's max |dq| decides whether the next step runs.
+// x += dx (charges and multiplier).  The correction's max |dq| gives the error bound left: the steps
+// contract the error by rho per step, estimated as the ratio of the last two corrections (RHO_PRIOR
+// after the first), so the remaining error is at most rho / (1 - rho) |dq|.  The next step runs while
+// that bound exceeds the target.
 __global__ __launch_bounds__(256) void k_ref_update(double *__restrict__ x, const double *__restrict__ dx, int nr,
-                                                    double stop, const int *__restrict__ cinfo,
+                                                    double target, const int *__restrict__ cinfo,
                                                     RefineState *__restrict__ rs) {
     __shared__ double red[4];
     if (!refine_active(cinfo, rs)) return;
@@ -1185,10 +1194,239 @@ __global__ __launch_bounds__(256) void k_ref_update(double *__restrict__ x, cons
     __syncthreads();
     if (threadIdx.x == 0) {
         const double m = fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
+        const double prev = rs->last_dq;
         rs->steps += 1;
         rs->last_dq = m;
-        rs->go = m > stop ? 1 : 0;
+        const double rho = (rs->steps >= 2 && prev > 0.0) ? m / prev : RHO_PRIOR;
+        const double bound = m == 0.0 ? 0.0 : (rho < 1.0 ? rho / (1.0 - rho) * m : INFINITY);
+        rs->bound = bound;
+        rs->go = bound > target ? 1 : 0;
     }
+}
+
+// Restarts the refinement after a fallback solve: clears the pivot / wait flags and the step state.
+__global__ void k_ref_reset(int *__restrict__ cinfo, RefineState *__restrict__ rs) {
+    cinfo[0] = 0;
+    cinfo[1] = 0;
+    rs->go = 1;
+    rs->steps = 0;
+    rs->last_dq = 0.0;
+    rs->bound = -1.0;
+}
+
+// dx = F^-1 [r1; r2] with the LU factors F (unit lower L below the diagonal, U on and above it) and the
+// interchanges piv of lu_solve_augmented (LAPACK getrs order: swaps, L, U); one work-group.  Used for the
+// refinement's corrections after the Cholesky fell back to LU.
+__global__ __launch_bounds__(1024) void k_lu_getrs(const double *__restrict__ F, int ld, int n, const int *__restrict__ piv,
+                                                   const double *__restrict__ r1, const double *__restrict__ r2,
+                                                   double *__restrict__ dx, const int *__restrict__ cinfo,
+                                                   const RefineState *__restrict__ rs) {
+    extern __shared__ double y[];
+    if (!refine_active(cinfo, rs)) return;
+    const int t = threadIdx.x;
+    for (int i = t; i < n; i += 1024) y[i] = i < n - 1 ? r1[i] : *r2;
+    __syncthreads();
+    if (t == 0)
+        for (int j = 0; j < n; ++j) {
+            const int p = piv[j];
+            if (p != j) {
+                const double tmp = y[j];
+                y[j] = y[p];
+                y[p] = tmp;
+            }
+        }
+    __syncthreads();
+    for (int j = 0; j < n - 1; ++j) {
+        const double yj = y[j];
+        for (int r = j + 1 + t; r < n; r += 1024) y[r] -= F[(int64_t)r * ld + j] * yj;
+        __syncthreads();
+    }
+    for (int i = n - 1; i >= 0; --i) {
+        const double xi = y[i] / F[(int64_t)i * ld + i];
+        __syncthreads();
+        if (t == 0) y[i] = xi;
+        for (int r = t; r < i; r += 1024) y[r] -= F[(int64_t)r * ld + i] * xi;
+        __syncthreads();
+    }
+    for (int i = t; i < n; i += 1024) dx[i] = y[i];
+}
+
+// ---- minimum-norm least squares of the bordered system (numpy.linalg.lstsq, rcond=None): the
+// symmetric matrix S = V diag(l) V^T by parallel cyclic Jacobi, then x = sum over |l_i| > eps n max|l|
+// of v_i (v_i . y) / l_i.  For symmetric S the singular values are |l_i| and the right singular
+// vectors are the v_i, so this is the SVD solution lstsq computes.
+//
+// A sweep visits every pair (p, q) once in ne - 1 rounds of ne / 2 disjoint pairs (ne = n rounded up to
+// even; index n is a dummy when n is odd): round-robin pairing, slot 0 fixed and slot s >= 1 holding
+// index 1 + (s - 1 + r) mod (ne - 1); pair l joins slots l and ne - 1 - l.  The rotations of a round act on
+// disjoint planes, so S <- G^T S G for all of them at once: work-group l rewrites rows p_l and q_l, each
+// entry from the 2 x 2 block of its own rows and the column pair of that entry.  Rows of W are the
+// eigenvectors (W <- G^T W).
+struct JacPair {
+    double c, s, t, apq;
+    int p, q, rot, pad_;
+};
+
+__device__ __forceinline__ int jac_index(int slot, int r, int ne) { return slot == 0 ? 0 : 1 + (slot - 1 + r) % (ne - 1); }
+__device__ __forceinline__ int jac_slot(int idx, int r, int ne) {
+    return idx == 0 ? 0 : 1 + ((idx - 1 - r) % (ne - 1) + (ne - 1)) % (ne - 1);
+}
+
+// Rotation of each pair of round r: zero S[p][q] (Rutishauser's tangent), unless |S[p][q]| is at most
+// rel sqrt(|S[p][p] S[q][q]|) or the absolute floor; counts the rotations of the sweep.
+__global__ __launch_bounds__(256) void k_jac_angles(const double *__restrict__ S, int ld, int n, int ne, int r,
+                                                    const double *__restrict__ floor_abs, double rel,
+                                                    JacPair *__restrict__ pairs, int *__restrict__ n_rot) {
+    const int l = blockIdx.x * 256 + threadIdx.x;
+    if (l >= ne / 2) return;
+    JacPair P{1.0, 0.0, 0.0, 0.0, jac_index(l, r, ne), jac_index(ne - 1 - l, r, ne), 0, 0};
+    if (P.p < n && P.q < n) {
+        const double app = S[(int64_t)P.p * ld + P.p], aqq = S[(int64_t)P.q * ld + P.q];
+        const double apq = 0.5 * (S[(int64_t)P.p * ld + P.q] + S[(int64_t)P.q * ld + P.p]);
+        if (fabs(apq) > fmax(rel * sqrt(fabs(app) * fabs(aqq)), *floor_abs)) {
+            const double th = (aqq - app) / (2.0 * apq);
+            const double t = fabs(th) > 1e150 ? 0.5 / th : copysign(1.0, th) / (fabs(th) + sqrt(fma(th, th, 1.0)));
+            const double c = 1.0 / sqrt(fma(t, t, 1.0));
+            P.c = c;
+            P.s = t * c;
+            P.t = t;
+            P.apq = apq;
+            P.rot = 1;
+            atomicAdd(n_rot, 1);
+        }
+    }
+    pairs[l] = P;
+}
+
+// One round: work-group l rewrites rows p_l, q_l of S (both held in LDS first) and of W.
+__global__ __launch_bounds__(256) void k_jac_round(double *__restrict__ S, int ld, int n, int ne, int r,
+                                                   const JacPair *__restrict__ pairs, double *__restrict__ W) {
+    extern __shared__ double rows[];   // [2][n]: old rows p and q
+    const JacPair P = pairs[blockIdx.x];
+    const bool hp = P.p < n, hq = P.q < n;
+    double *sp = rows, *sq = rows + n;
+    const int t = threadIdx.x;
+    for (int j = t; j < n; j += 256) {
+        sp[j] = hp ? S[(int64_t)P.p * ld + j] : 0.0;
+        sq[j] = hq ? S[(int64_t)P.q * ld + j] : 0.0;
+    }
+    __syncthreads();
+    for (int j = t; j < n; j += 256) {
+        const int slot = jac_slot(j, r, ne), pslot = ne - 1 - slot;
+        const int pj = jac_index(pslot, r, ne);
+        double vp = sp[j], vq = sq[j];
+        if (pj < n) {
+            const JacPair Q = pairs[min(slot, pslot)];
+            // column j of S G: c S[., j] - s S[., partner] for the first of a pair, s S[., partner] + c S[., j]
+            // for the second
+            if (slot < pslot) {
+                vp = Q.c * sp[j] - Q.s * sp[pj];
+                vq = Q.c * sq[j] - Q.s * sq[pj];
+            } else {
+                vp = Q.s * sp[pj] + Q.c * sp[j];
+                vq = Q.s * sq[pj] + Q.c * sq[j];
+            }
+        }
+        double np_ = P.c * vp - P.s * vq, nq = P.s * vp + P.c * vq;
+        if (P.rot && (j == P.p || j == P.q)) {
+            // the 2 x 2 block of the pair itself, diagonalised exactly
+            if (j == P.p) {
+                np_ = sp[P.p] - P.t * P.apq;
+                nq = 0.0;
+            } else {
+                np_ = 0.0;
+                nq = sq[P.q] + P.t * P.apq;
+            }
+        }
+        if (hp) S[(int64_t)P.p * ld + j] = np_;
+        if (hq) S[(int64_t)P.q * ld + j] = nq;
+    }
+    if (P.rot)
+        for (int i = t; i < n; i += 256) {
+            const double wp = W[(int64_t)P.p * n + i], wq = W[(int64_t)P.q * n + i];
+            W[(int64_t)P.p * n + i] = P.c * wp - P.s * wq;
+            W[(int64_t)P.q * n + i] = P.s * wp + P.c * wq;
+        }
+}
+
+__global__ __launch_bounds__(256) void k_eye(double *__restrict__ W, int n) {
+    const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (e < (int64_t)n * n) W[e] = (e / n == e % n) ? 1.0 : 0.0;
+}
+
+__global__ void k_fill_i32(int *p, int v) { *p = v; }
+
+// scal[0] = the rotation floor (||S||_F times 2^-52 / 100); one work-group.
+__global__ __launch_bounds__(1024) void k_jac_floor(const double *__restrict__ S, int ld, int n, double *__restrict__ scal) {
+    __shared__ double red[16];
+    double acc = 0.0;
+    for (int64_t e = threadIdx.x; e < (int64_t)n * n; e += 1024) {
+        const double v = S[(e / n) * ld + e % n];
+        acc = fma(v, v, acc);
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double s = 0.0;
+        for (int w = 0; w < 16; ++w) s += red[w];
+        scal[0] = sqrt(s) * 2.220446049250313e-16 * 1e-2;
+    }
+}
+
+// scal[1] = the cut-off eps * n * max |l| of numpy.linalg.lstsq (rcond = None); one work-group.
+__global__ __launch_bounds__(1024) void k_jac_cut(const double *__restrict__ S, int ld, int n, double *__restrict__ scal) {
+    __shared__ double red[16];
+    double m = 0.0;
+    for (int i = threadIdx.x; i < n; i += 1024) m = fmax(m, fabs(S[(int64_t)i * ld + i]));
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o));
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double s = 0.0;
+        for (int w = 0; w < 16; ++w) s = fmax(s, red[w]);
+        scal[1] = 2.220446049250313e-16 * (double)n * s;
+    }
+}
+
+// y = [a[0 .. n-1) ; *b] (a strided by stride), into one contiguous vector.
+__global__ __launch_bounds__(256) void k_gather_rhs(const double *__restrict__ a, int64_t stride, const double *__restrict__ b,
+                                                    int n, double *__restrict__ y, const int *__restrict__ cinfo,
+                                                    const RefineState *__restrict__ rs) {
+    if (rs && !refine_active(cinfo, rs)) return;
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i < n) y[i] = i < n - 1 ? a[(int64_t)i * stride] : *b;
+}
+
+// coef_i = (w_i . y) / l_i for |l_i| above the cut-off, else 0; one work-group per i.
+__global__ __launch_bounds__(256) void k_pinv_coef(const double *__restrict__ W, const double *__restrict__ S, int ld, int n,
+                                                   const double *__restrict__ y, const double *__restrict__ scal,
+                                                   double *__restrict__ coef, const int *__restrict__ cinfo,
+                                                   const RefineState *__restrict__ rs) {
+    __shared__ double red[4];
+    if (rs && !refine_active(cinfo, rs)) return;
+    const int i = blockIdx.x;
+    double acc = 0.0;
+    for (int j = threadIdx.x; j < n; j += 256) acc = fma(W[(int64_t)i * n + j], y[j], acc);
+    const double d = block_sum_256(acc, red);
+    if (threadIdx.x == 0) {
+        const double l = S[(int64_t)i * ld + i];
+        coef[i] = fabs(l) > scal[1] ? d / l : 0.0;
+    }
+}
+
+// x_j = sum_i coef_i w_i[j]
+__global__ __launch_bounds__(256) void k_pinv_combine(const double *__restrict__ W, int n, const double *__restrict__ coef,
+                                                      double *__restrict__ x, const int *__restrict__ cinfo,
+                                                      const RefineState *__restrict__ rs) {
+    if (rs && !refine_active(cinfo, rs)) return;
+    const int j = blockIdx.x * 256 + threadIdx.x;
+    if (j >= n) return;
+    double acc = 0.0;
+    for (int i = 0; i < n; ++i) acc = fma(coef[i], W[(int64_t)i * n + j], acc);
+    x[j] = acc;
 }
 
 // Back substitution U x = y (y = column n of the factored augmented matrix); x -> out.
@@ -1204,11 +1442,6 @@ __global__ __launch_bounds__(1024) void k_backsub(const double *__restrict__ A, 
         __syncthreads();
     }
     for (int r = threadIdx.x; r < n; r += 1024) out[r] = y[r];
-}
-
-__global__ void k_expand(const double *__restrict__ x, const int *__restrict__ src, int N, double *__restrict__ q) {
-    const int a = blockIdx.x * 256 + threadIdx.x;
-    if (a < N) q[a] = src[a] >= 0 ? x[src[a]] : 0.0;
 }
 
 // Copy of the constraint block from mapped pinned host memory (a few KB; the kernel's loads cross
@@ -1229,6 +1462,7 @@ __global__ __launch_bounds__(256) void k_publish(const double *__restrict__ x, c
         out[0] = cinfo[0] != 0 ? (double)cinfo[0] : (cinfo[1] != 0 ? -1.0 : 0.0);
         out[1 + N + n] = (double)rs->steps;
         out[2 + N + n] = rs->last_dq;
+        out[3 + N + n] = rs->bound;
     }
     if (i < N) out[1 + i] = src[i] >= 0 ? x[src[i]] : 0.0;
     if (i < n) out[1 + N + i] = x[i];
@@ -1263,6 +1497,42 @@ int lu_solve_augmented(Ctx *ctx, double *K2, int ld, int n, int *piv, int *info,
     return RP_OK;
 }
 
+namespace {
+
+// Eigendecomposition of the symmetric n x n block of S (in place: S -> diag(l), rows of W = eigenvectors)
+// by parallel cyclic Jacobi; scal: [0] rotation floor, [1] lstsq cut-off (set here).  Sweeps until one
+// makes no rotation (at most max_sweeps); one synchronisation per sweep reads the count.
+int jacobi_eigen(Ctx *ctx, double *S, int ld, int n, double *W, JacPair *pairs, int *d_rot, int *h_rot,
+                 double *scal, hipStream_t st, int max_sweeps = 40) {
+    const int ne = n + (n & 1), m = ne / 2;
+    const size_t lds = 2 * sizeof(double) * (size_t)n;
+    if (lds > 160 * 1024) return ctx->fail(RP_EINVAL, "least-squares fallback: more than 10240 unknowns");
+    if (lds > 64 * 1024)
+        RP_HIP(ctx, hipFuncSetAttribute(reinterpret_cast<const void *>(&k_jac_round),
+                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    k_eye<<<ceil_div((int64_t)n * n, 256), 256, 0, st>>>(W, n);
+    k_jac_floor<<<1, 1024, 0, st>>>(S, ld, n, scal);
+    for (int sweep = 0; sweep < max_sweeps; ++sweep) {
+        k_fill_i32<<<1, 1, 0, st>>>(d_rot, 0);
+        for (int r = 0; r < ne - 1; ++r) {
+            k_jac_angles<<<ceil_div(m, 256), 256, 0, st>>>(S, ld, n, ne, r, scal, 1e-15, pairs, d_rot);
+            k_jac_round<<<m, 256, lds, st>>>(S, ld, n, ne, r, pairs, W);
+        }
+        RP_HIP(ctx, hipGetLastError());
+        RP_HIP(ctx, hipMemcpyAsync(h_rot, d_rot, sizeof(int), hipMemcpyDeviceToHost, st));
+        RP_HIP(ctx, hipStreamSynchronize(st));
+        if (*h_rot == 0) {
+            k_jac_cut<<<1, 1024, 0, st>>>(S, ld, n, scal);
+            RP_HIP(ctx, hipGetLastError());
+            return RP_OK;
+        }
+    }
+    return ctx->fail(RP_ESINGULAR, "system matrix is exactly singular and the least-squares fallback (Jacobi "
+                                   "eigendecomposition) did not converge in " + std::to_string(max_sweeps) + " sweeps");
+}
+
+}  // namespace
+
 int run_solve(rp_fit *fit, const rp_constraints *con, double *q_out, double *solution_out, int32_t *used_fallback) {
     Ctx *ctx = fit->ctx;
     hipStream_t st = ctx->stream;
@@ -1285,7 +1555,7 @@ int run_solve(rp_fit *fit, const rp_constraints *con, double *q_out, double *sol
     const size_t n_int = (size_t)(nr + 1) + nsrc + N + (size_t)(N + 1) + nsrc;
     const size_t off_d = round_up((int64_t)(4 * n_int), 8);
     const size_t stage = round_up((int64_t)(off_d + 8 * (2 * (size_t)nsrc + 3 * (size_t)nr)), 16);
-    const size_t out_bytes = sizeof(double) * (3 + (size_t)N + n);
+    const size_t out_bytes = sizeof(double) * (4 + (size_t)N + n);
     if (fit->h_stage_bytes < stage + out_bytes) {
         if (fit->h_stage) (void)hipHostFree(fit->h_stage);
         fit->h_stage = nullptr;
@@ -1423,13 +1693,13 @@ int run_solve(rp_fit *fit, const rp_constraints *con, double *q_out, double *sol
     kts = nullptr;
     if (refine) {
         // fp64 residual against the kept basis, correction through the same factor; a step whose
-        // predecessor's correction was at most RP_REFINE_STOP_E returns at once on the device
+        // predecessor left an error bound of at most RP_REFINE_TARGET_E returns at once on the device
         KTimer ktr(ctx, KT_REFINE);
         const int64_t M = fit->n_keep, rows = fit->phi_rows, plane = (int64_t)NPg * rows;
         double *e = d_e.as<double>(), *esum = e + M + 2;
         double *u = d_rv.as<double>(), *r = u + N, *r1 = r + N, *z = r1 + nr, *dx = z + nr, *r2 = dx + nr + 1;
         const double *phi = fit->ws[7].as<double>();
-        const int steps = fit->opt.refine_steps > 0 ? fit->opt.refine_steps : 2;
+        const int steps = fit->opt.refine_steps > 0 ? fit->opt.refine_steps : RP_REFINE_STEPS_DEFAULT;
         for (int it = 0; it < steps; ++it) {
             k_ref_expand<<<ceil_div(N, 256), 256, 0, st>>>(d_x.as<double>(), dc_inv_ptr, dc_inv_set, dc_inv_mult, N, u,
                                                            d_cinfo.as<int>(), d_rs);
@@ -1447,7 +1717,7 @@ int run_solve(rp_fit *fit, const rp_constraints *con, double *q_out, double *sol
                                             dx, d_sync.as<unsigned>(), fit->bs_base, ++fit->bs_epoch, spin_limit,
                                             d_cinfo.as<int>() + 1, d_cinfo.as<int>(), r2, &d_rs->go);
             fit->bs_base += (unsigned)nblk;
-            k_ref_update<<<1, 256, 0, st>>>(d_x.as<double>(), dx, nr, RP_REFINE_STOP_E, d_cinfo.as<int>(), d_rs);
+            k_ref_update<<<1, 256, 0, st>>>(d_x.as<double>(), dx, nr, RP_REFINE_TARGET_E, d_cinfo.as<int>(), d_rs);
         }
     }
     k_publish<<<ceil_div(N + 1, 256), 256, 0, st>>>(d_x.as<double>(), dc_exp, N, n, d_cinfo.as<int>(), d_rs, h_out_dev);
@@ -1455,13 +1725,15 @@ int run_solve(rp_fit *fit, const rp_constraints *con, double *q_out, double *sol
     RP_HIP(ctx, hipEventRecord(e1, st));
     RP_HIP(ctx, hipStreamSynchronize(st));
     const int cinfo = (int)h_out[0];
+    fit->solver = RP_SOLVER_CHOLESKY;
     if (cinfo == 0) {
         std::copy(h_out + 1, h_out + 1 + N, q_out);
         if (solution_out) std::copy(h_out + 1 + N, h_out + 1 + N + n, solution_out);
+        fit->refine_steps_done = refine ? (int32_t)h_out[1 + N + n] : 0;
+        fit->refine_last_dq = refine ? h_out[2 + N + n] : 0.0;
+        fit->refine_bound = refine ? h_out[3 + N + n] : -1.0;
     }
     fit->lu_fallback = cinfo != 0;
-    fit->refine_steps_done = refine && cinfo == 0 ? (int32_t)h_out[1 + N + n] : 0;
-    fit->refine_last_dq = refine && cinfo == 0 ? h_out[2 + N + n] : 0.0;
     if (cinfo < 0) {
         // a backward-substitution wait gave up: start the tickets and flags afresh for the next solve
         const int words = (int)(d_sync.bytes / sizeof(unsigned));
@@ -1470,23 +1742,97 @@ int run_solve(rp_fit *fit, const rp_constraints *con, double *q_out, double *sol
         fit->bs_epoch = 0;
     }
     if (cinfo != 0) {
-        // indefinite or singular A (or a backward-substitution wait that gave up): LU with partial
-        // pivoting on the bordered system
-        const int rc = lu_solve_augmented(ctx, d_K2.as<double>(), ld, n, d_piv.as<int>(), d_info.as<int>(),
-                                          d_x.as<double>(), st);
+        // indefinite or singular A (or a triangular-solve wait that gave up): LU with partial pivoting on
+        // the bordered system; on an exactly zero pivot the minimum-norm least-squares solution, as the
+        // reference's lstsq branch (repeat.py:2306-2310).  The refined form then refines that solution
+        // with the same corrections against the kept fp64 basis, each solved with the fallback's factors.
+        int rc = lu_solve_augmented(ctx, d_K2.as<double>(), ld, n, d_piv.as<int>(), d_info.as<int>(),
+                                    d_x.as<double>(), st);
+        fit->solver = RP_SOLVER_LU;
+        DBuf &d_W = fit->ws[11], &d_jw = fit->ws[14];
+        double *jy = nullptr, *jcoef = nullptr, *jscal = nullptr;
+        if (rc == RP_ESINGULAR) {
+            fit->solver = RP_SOLVER_LSTSQ;
+            const int ne = n + (n & 1);
+            const size_t pair_bytes = round_up((int64_t)sizeof(JacPair) * (ne / 2), 256);
+            RP_CHECK(d_W.grow(ctx, sizeof(double) * (size_t)n * n));
+            RP_CHECK(d_jw.grow(ctx, pair_bytes + sizeof(double) * (2 * (size_t)n + 4) + 256));
+            JacPair *pairs = d_jw.as<JacPair>();
+            jy = reinterpret_cast<double *>(d_jw.as<char>() + pair_bytes);
+            jcoef = jy + n;
+            jscal = jcoef + n;
+            int *d_rot = reinterpret_cast<int *>(jscal + 2);
+            // the bordered system again (the LU overwrote its copy)
+            k_assemble<<<ceil_div((int64_t)n * ncols, 256), 256, 0, st>>>(
+                fit->d_gram.as<double>(), NPg, N, dc_ptr, dc_src, dc_mult, dc_w, con->diag_add ? dc_diag : nullptr,
+                con->rhs_add ? dc_rhs : nullptr, con->q_tot, nr, d_K.as<double>(), d_K2.as<double>(), ld,
+                d_wc.as<double>(), d_bc.as<double>(), d_cinfo.as<int>(), d_rs);
+            int h_rot = 0;
+            rc = jacobi_eigen(ctx, d_K2.as<double>(), ld, n, d_W.as<double>(), pairs, d_rot, &h_rot, jscal, st);
+            if (rc == RP_OK) {
+                // right-hand side: column n of the bordered system, its last row q_tot
+                k_gather_rhs<<<ceil_div(n, 256), 256, 0, st>>>(d_K2.as<double>() + n, ld,
+                                                              d_K2.as<double>() + (int64_t)(n - 1) * ld + n, n, jy,
+                                                              nullptr, nullptr);
+                k_pinv_coef<<<n, 256, 0, st>>>(d_W.as<double>(), d_K2.as<double>(), ld, n, jy, jscal, jcoef, nullptr,
+                                               nullptr);
+                k_pinv_combine<<<ceil_div(n, 256), 256, 0, st>>>(d_W.as<double>(), n, jcoef, d_x.as<double>(), nullptr,
+                                                                 nullptr);
+                RP_HIP(ctx, hipGetLastError());
+                std::fprintf(stderr, "Warning: Matrix singular, using least squares fallback.\n");
+            }
+        }
         if (rc != RP_OK) {
             (void)hipEventDestroy(e0);
             (void)hipEventDestroy(e1);
             return rc;
         }
-        k_expand<<<ceil_div(N, 256), 256, 0, st>>>(d_x.as<double>(), dc_exp, N, d_q.as<double>());
+        fit->refine_steps_done = 0;
+        fit->refine_last_dq = 0.0;
+        fit->refine_bound = -1.0;
+        if (refine) {
+            KTimer ktr(ctx, KT_REFINE);
+            const int64_t M = fit->n_keep, rows = fit->phi_rows, plane = (int64_t)NPg * rows;
+            double *e = d_e.as<double>(), *esum = e + M + 2;
+            double *u = d_rv.as<double>(), *r = u + N, *r1 = r + N, *z = r1 + nr, *dx = z + nr, *r2 = dx + nr + 1;
+            const double *phi = fit->ws[7].as<double>();
+            const int *ci = d_cinfo.as<int>();
+            const int steps = fit->opt.refine_steps > 0 ? fit->opt.refine_steps : RP_REFINE_STEPS_DEFAULT;
+            k_ref_reset<<<1, 1, 0, st>>>(d_cinfo.as<int>(), d_rs);
+            for (int it = 0; it < steps; ++it) {
+                k_ref_expand<<<ceil_div(N, 256), 256, 0, st>>>(d_x.as<double>(), dc_inv_ptr, dc_inv_set, dc_inv_mult, N,
+                                                               u, ci, d_rs);
+                k_ref_rows<<<n_esum, 256, 0, st>>>(phi, rows, plane, N, M, fit->d_v.as<double>(), u, e, esum, ci, d_rs);
+                k_ref_cols<<<N, 256, 0, st>>>(phi, rows, plane, M, e, esum, n_esum, r, ci, d_rs);
+                k_ref_rhs<<<1, 256, 0, st>>>(r, dc_ptr, dc_src, dc_mult, dc_w, con->diag_add ? dc_diag : nullptr,
+                                             con->rhs_add ? dc_rhs : nullptr, con->q_tot, nr, d_x.as<double>(), r1, r2,
+                                             ci, d_rs);
+                if (fit->solver == RP_SOLVER_LU) {
+                    k_lu_getrs<<<1, 1024, sizeof(double) * n, st>>>(d_K2.as<double>(), ld, n, d_piv.as<int>(), r1, r2,
+                                                                     dx, ci, d_rs);
+                } else {
+                    k_gather_rhs<<<ceil_div(n, 256), 256, 0, st>>>(r1, 1, r2, n, jy, ci, d_rs);
+                    k_pinv_coef<<<n, 256, 0, st>>>(d_W.as<double>(), d_K2.as<double>(), ld, n, jy, jscal, jcoef, ci,
+                                                   d_rs);
+                    k_pinv_combine<<<ceil_div(n, 256), 256, 0, st>>>(d_W.as<double>(), n, jcoef, dx, ci, d_rs);
+                }
+                k_ref_update<<<1, 256, 0, st>>>(d_x.as<double>(), dx, nr, RP_REFINE_TARGET_E, ci, d_rs);
+            }
+        }
+        k_publish<<<ceil_div(N + 1, 256), 256, 0, st>>>(d_x.as<double>(), dc_exp, N, n, d_cinfo.as<int>(), d_rs,
+                                                        h_out_dev);
         RP_HIP(ctx, hipGetLastError());
         RP_HIP(ctx, hipEventRecord(e1, st));
-        RP_HIP(ctx, hipMemcpyAsync(q_out, d_q.p, sizeof(double) * N, hipMemcpyDeviceToHost, st));
-        if (solution_out) RP_HIP(ctx, hipMemcpyAsync(solution_out, d_x.p, sizeof(double) * n, hipMemcpyDeviceToHost, st));
         RP_HIP(ctx, hipStreamSynchronize(st));
+        std::copy(h_out + 1, h_out + 1 + N, q_out);
+        if (solution_out) std::copy(h_out + 1 + N, h_out + 1 + N + n, solution_out);
+        if (refine) {
+            fit->refine_steps_done = (int32_t)h_out[1 + N + n];
+            fit->refine_last_dq = h_out[2 + N + n];
+            fit->refine_bound = h_out[3 + N + n];
+        }
     }
-    if (used_fallback) *used_fallback = fit->lu_fallback ? 1 : 0;
+    if (used_fallback) *used_fallback = fit->solver;
     ctx->collect();
     fit->resolve_timings(false);
     float ms = 0.f;

@@ -191,16 +191,37 @@ def test_fit_type0_matches_oracle(oracle, gpu_ctx):
     calc.close()
 
 
-def test_singular_kkt_reports(gpu_ctx):
-    """Two atoms at the same place make the KKT matrix exactly singular (the reference's lstsq branch)."""
+def _duplicate_atom_case(oracle):
+    """Three atoms, the second at the first's place: identical basis columns, an exactly singular KKT
+    matrix.  Its minimum-norm solution (the reference's lstsq branch) is the fit without the second atom,
+    the first atom's charge split equally between the two."""
     s = synth.tiny(seed=14, n_atoms=3, n=12, L=9.0)
     pos = s.atom_pos.copy()
     pos[1] = pos[0]
-    calc = ChargeCalculator(cube={"n_atoms": 3, "n_grid": s.n_grid, "atom_index": s.atomic_numbers, "atom_pos": pos,
-                                  "axis_vector": s.axis, "esp_cube": s.esp_cube}, precision="fp64", context=gpu_ctx)
-    with pytest.raises(_abi.SingularKKTError):
-        calc.fit_charges()
+    keep = [0, 2]
+    merged = oracle.fit(s.n_grid, s.axis, s.atomic_numbers[keep], pos[keep], s.esp_flat())
+    expect = np.array([merged.charges[0] / 2, merged.charges[0] / 2, merged.charges[1]])
+    cube = {"n_atoms": 3, "n_grid": s.n_grid, "atom_index": s.atomic_numbers, "atom_pos": pos, "axis_vector": s.axis,
+            "esp_cube": s.esp_cube}
+    return cube, expect
+
+
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-6), ("split", 1e-5)])
+def test_singular_kkt_least_squares(oracle, gpu_ctx, capfd, precision, tol):
+    """An exactly singular KKT matrix (two atoms at one place) is solved by the minimum-norm least
+    squares, as the reference's lstsq fallback (repeat.py:2306-2310), with its warning."""
+    cube, expect = _duplicate_atom_case(oracle)
+    print(f"singular KKT, {precision}: lstsq fallback", flush=True)
+    calc = ChargeCalculator(cube=cube, precision=precision, context=gpu_ctx)
+    q = calc.fit_charges()
+    rep = calc._fit.solve_report()
     calc.close()
+    print(f"  q {q}, expected {expect}, report {rep}", flush=True)
+    assert rep["solver"] == _abi.RP_SOLVER_LSTSQ
+    assert "least squares fallback" in capfd.readouterr().err
+    np.testing.assert_allclose(q, expect, rtol=0, atol=tol)
+    assert abs(q[0] - q[1]) <= 1e-9
+    assert abs(q.sum()) < 1e-9
 
 
 def test_backsub_wait_limit_falls_back_to_lu(gpu_ctx, tmp_path):
@@ -236,6 +257,46 @@ def test_backsub_wait_limit_falls_back_to_lu(gpu_ctx, tmp_path):
         out[spin] = (np.load(path), r.stdout.split()[-1])
     assert out["0"][1] == "1" and out[None][1] == "0"
     assert np.max(np.abs(out["0"][0] - out[None][0])) <= 1e-8
+
+
+def test_backsub_wait_limit_refined_refines_the_lu(gpu_ctx, tmp_path):
+    """Refined form with every triangular-solve wait giving up (RP_BACKSUB_SPIN=0, a fresh process): the
+    solve falls back to LU, and the LU solution of the 3-slice system is refined against the kept fp64
+    basis with LU corrections, so the charges still meet the fp64 path's (instead of the 3-slice
+    system's 1e-4 e)."""
+    import subprocess
+    import sys
+    script = tmp_path / "spin_refined.py"
+    script.write_text(
+        "import sys, numpy as np\n"
+        f"sys.path.insert(0, {str(ROOT)!r})\n"
+        "from pyrepeat_amd import _abi, params, synth\n"
+        "s = synth.crystal200()\n"
+        "ctx = _abi.Context(0, 0)\n"
+        "con = params.constraint_arrays(s.n_atoms, s.atomic_numbers, 0.0, None, None, None)\n"
+        "out = []\n"
+        "for prec in (_abi.RP_PREC_OZAKI_REFINED, _abi.RP_PREC_FP64):\n"
+        "    f = _abi.Fit(ctx, s.n_grid, s.axis, s.atomic_numbers, s.atom_pos, s.esp_cube, _abi.RP_ORDER_CUBE, 1, prec)\n"
+        "    f.accumulate()\n"
+        "    q, _ = f.solve(con)\n"
+        "    out.append(q)\n"
+        "    if prec == _abi.RP_PREC_OZAKI_REFINED:\n"
+        "        rep = f.solve_report()\n"
+        "    f.close()\n"
+        "np.save(sys.argv[1], np.array(out))\n"
+        "print('report', rep)\n"
+        "print('solver', rep['solver'], 'steps', rep['refine_steps'], 'within', int(rep['within_target']))\n")
+    env = dict(os.environ)
+    env["RP_BACKSUB_SPIN"] = "0"
+    path = tmp_path / "q_refined.npy"
+    print("refined, backsub wait limit 0", flush=True)
+    r = subprocess.run([sys.executable, str(script), str(path)], env=env, capture_output=True, text=True, timeout=240)
+    assert r.returncode == 0, r.stderr[-2000:]
+    print(r.stdout, flush=True)
+    tail = r.stdout.split()
+    assert tail[-5] == str(_abi.RP_SOLVER_LU) and int(tail[-3]) >= 1 and tail[-1] == "1"
+    q_ref, q64 = np.load(path)
+    assert np.max(np.abs(q_ref - q64)) <= 1e-8
 
 
 def test_solve_large_n_matches_dense_solve(oracle, gpu_ctx):

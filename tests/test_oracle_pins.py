@@ -306,6 +306,21 @@ def test_kkt_solution_properties(oracle, small_fit):
     np.testing.assert_allclose(r.kkt_solution, np.linalg.solve(K, rhs), rtol=1e-10, atol=1e-12)
 
 
+def test_kkt_lstsq_branch_on_exact_zero_pivot(oracle):
+    """solve_kkt takes the reference's lstsq branch (repeat.py:2306-2310) exactly when LAPACK's LU meets a
+    zero pivot.  Small integer entries keep the elimination exact: rows 0 and 1 of the bordered matrix
+    are equal, the unit pivot eliminates row 1 to exact zeros, the LU raises, and lstsq returns the
+    minimum-norm solution, worked by hand: q0 + q1 + lam = 1, q2 + lam = 2, q0 + q1 + q2 = 0 give
+    lam = 1.5, q2 = 0.5, q0 + q1 = -0.5, split equally by the minimum norm."""
+    A = np.array([[1.0, 1.0, 0.0], [1.0, 1.0, 0.0], [0.0, 0.0, 1.0]])
+    sol, fallback = oracle.solve_kkt(A, np.array([1.0, 1.0, 2.0]), np.ones(3), 0.0)
+    assert fallback
+    np.testing.assert_allclose(sol, [-0.25, -0.25, 0.5, 1.5], rtol=0, atol=1e-14)
+    # a regular system does not take it
+    sol2, fallback2 = oracle.solve_kkt(A + np.eye(3), np.array([1.0, 1.0, 2.0]), np.ones(3), 0.0)
+    assert not fallback2
+
+
 def test_self_consistent_recovery_and_rms(oracle):
     """V := Phi q* + c on the kept points -> the fit returns q* and RMS 0."""
     s = synth.tiny(seed=8, n_atoms=4, n=12, L=9.0)
