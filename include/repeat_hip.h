@@ -58,6 +58,10 @@ extern "C" {
                             the last two corrections (RP_REFINE_RHO_PRIOR after the first); steps
                             run while that bound exceeds RP_REFINE_TARGET_E, at most
                             rp_options.refine_steps (rp_fit_solve_report gives the bound). */
+#define RP_PREC_FP64_FACT 3  /* fp64 throughout with the split form's factorized reciprocal sum: the fp64
+                            basis of RP_PREC_OZAKI, its Gram [Phi | V | 1] on fp64 MFMA, no slices, no
+                            refinement; the same normal equations as RP_PREC_FP64 to fp64 rounding,
+                            without the K-wide trig tables and GEMM */
 #define RP_REFINE_TARGET_E 1e-6
 #define RP_REFINE_RHO_PRIOR 1e-2     /* twice the largest contraction measured (5.3e-3, cond(A) 4.9e10) */
 #define RP_REFINE_STEPS_DEFAULT 4
@@ -85,7 +89,7 @@ typedef struct rp_system {
 /* Command-line options of repeat.py that reach the fit path (repeat.py:2655-2660, 2680). */
 typedef struct rp_options {
     int32_t fit_type;            /* --fit-type: 1 Ewald (default), 0 bare 1/r */
-    int32_t precision;           /* RP_PREC_FP64 (--fp64) | RP_PREC_OZAKI | RP_PREC_OZAKI_REFINED */
+    int32_t precision;           /* RP_PREC_FP64 (--fp64) | RP_PREC_OZAKI | RP_PREC_OZAKI_REFINED | RP_PREC_FP64_FACT */
     double vdw_factor;           /* --vdw-factor, default 1.0 */
     double vdw_max;              /* --vdw-max, default 1000.0 */
     double r_cutoff;             /* --cutoff, default 20.0 */
@@ -246,10 +250,14 @@ typedef struct rp_solve_report {
     int32_t solver;              /* RP_SOLVER_* */
     int32_t refine_steps;        /* refinement steps run (RP_PREC_OZAKI_REFINED) */
     double last_correction;      /* max |dq| of the last correction (e) */
-    double error_bound;          /* refined: estimated max |q - q_fp64| left (e); -1: no estimate (other forms) */
+    double error_bound;          /* refined: estimated max |q - q_converged| left by the steps (e); -1: none */
     int32_t within_target;       /* refined: error_bound <= RP_REFINE_TARGET_E; other forms: 1 */
-    int32_t reserved;
+    int32_t well_conditioned;    /* sensitivity known and <= RP_SENSITIVITY_LIMIT_E */
+    double sensitivity;          /* max |dq| (e) when every entry of the solved system moves by 8 ulps
+                                    (signs pseudo-random): how far fp64 rounding alone can move these charges,
+                                    whatever the implementation; -1 after an LU or least-squares fallback */
 } rp_solve_report;
+#define RP_SENSITIVITY_LIMIT_E 1e-5
 int rp_fit_solve_report(const rp_fit *fit, rp_solve_report *out);
 
 /*

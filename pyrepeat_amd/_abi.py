@@ -26,7 +26,7 @@ RP_EAGAIN = -8
 
 RP_F64, RP_F32 = 0, 1
 RP_ORDER_CUBE, RP_ORDER_FLAT = 0, 1
-RP_PREC_FP64, RP_PREC_OZAKI, RP_PREC_OZAKI_REFINED = 0, 1, 2
+RP_PREC_FP64, RP_PREC_OZAKI, RP_PREC_OZAKI_REFINED, RP_PREC_FP64_FACT = 0, 1, 2, 3
 
 EXPORTED = (
     "rp_ctx_create", "rp_ctx_destroy", "rp_last_error", "rp_ctx_hbm_in_use",
@@ -40,6 +40,7 @@ EXPORTED = (
 )
 RP_SOLVER_CHOLESKY, RP_SOLVER_LU, RP_SOLVER_LSTSQ = 0, 1, 2
 RP_REFINE_TARGET_E = 1e-6
+RP_SENSITIVITY_LIMIT_E = 1e-5
 KERNEL_KINDS = {"kspace": 0, "gram": 1, "filter": 2, "solve": 3, "phase": 4, "slice": 5, "reduce": 6, "refine": 7}
 
 
@@ -105,7 +106,8 @@ class RpCubeHeader(ctypes.Structure):
 
 class RpSolveReport(ctypes.Structure):
     _fields_ = [("solver", ctypes.c_int32), ("refine_steps", ctypes.c_int32), ("last_correction", ctypes.c_double),
-                ("error_bound", ctypes.c_double), ("within_target", ctypes.c_int32), ("reserved", ctypes.c_int32)]
+                ("error_bound", ctypes.c_double), ("within_target", ctypes.c_int32), ("well_conditioned", ctypes.c_int32),
+                ("sensitivity", ctypes.c_double)]
 
 
 class RpChunkPlanInfo(ctypes.Structure):
@@ -412,7 +414,8 @@ class Fit:
         check(load().rp_fit_solve_report(self.handle, ctypes.byref(r)), self.ctx.handle)
         return {"solver": int(r.solver), "refine_steps": int(r.refine_steps),
                 "last_correction": float(r.last_correction), "error_bound": float(r.error_bound),
-                "within_target": bool(r.within_target)}
+                "within_target": bool(r.within_target), "sensitivity": float(r.sensitivity),
+                "well_conditioned": bool(r.well_conditioned)}
 
     def refit(self, con, c_prepared=None):
         """Whole fit (filter, accumulate, solve) from the device-resident cube."""

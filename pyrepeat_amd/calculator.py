@@ -20,6 +20,8 @@ accumulation, fp64 across chunks and in the solve.
 ``precision="refined"`` is the split form with 3 slices followed by iterative
 refinement of the solve against the fp64 basis (kept in HBM): its charges
 converge to those of the fp64 normal equations.
+``precision="fp64-factorized"`` is fp64 throughout with the split form's
+factorized basis and an fp64 Gram (no slices, no refinement).
 Every computation runs in ``librepeat_hip.so``; nothing falls back to the CPU.
 """
 from __future__ import annotations
@@ -34,7 +36,8 @@ from .cube import read_data, v_pot_view
 from .params import constraint_arrays, read_qeq_params, read_resp_params, read_symmetry
 from .tables import ATOM_SYMBOLS, DEFAULT_QEQ_PARAMS
 
-PRECISIONS = {"fp64": _abi.RP_PREC_FP64, "split": _abi.RP_PREC_OZAKI, "refined": _abi.RP_PREC_OZAKI_REFINED}
+PRECISIONS = {"fp64": _abi.RP_PREC_FP64, "split": _abi.RP_PREC_OZAKI, "refined": _abi.RP_PREC_OZAKI_REFINED,
+              "fp64-factorized": _abi.RP_PREC_FP64_FACT}
 
 
 class ChargeCalculator:
@@ -112,6 +115,12 @@ class ChargeCalculator:
             self._fit.accumulate()
             q, sol = self._fit.solve(self.constraints())
             self.solve_report = dict(self._fit.solve_report(), refined_bound_e=bound)
+        sens = self.solve_report["sensitivity"]
+        if sens > _abi.RP_SENSITIVITY_LIMIT_E:
+            # ill-conditioned: fp64 rounding of the system alone moves these charges by about sens, so no
+            # fp64 implementation (the reference's included) determines them more closely
+            print(f"Warning: ill-conditioned fit; rounding the system in fp64 moves the charges by up to "
+                  f"{sens:.1e} e.", file=sys.stderr)
         _, _, sum_phi, sum_v = self._fit.normal_equations()
         # The reference scatters the *unreduced* sum_phi as if it were reduced (repeat.py:2317-2320, 2491-2509).
         self._sum_phi = _expand_sum_phi_quirk(sum_phi, self.symm_data, self.n_atoms) if self.symm_data else sum_phi

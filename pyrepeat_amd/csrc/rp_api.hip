@@ -256,7 +256,10 @@ int run_accumulate(rp_fit *fit) {
     fit->acc_timing_pending = false;
     RP_HIP(ctx, hipEventRecord(fit->ev_acc0, st));
     k_zero<<<ceil_div(g2, 256), 256, 0, st>>>(fit->d_gram.as<double>(), g2);
-    int rc = fit->opt.precision == RP_PREC_FP64 ? accumulate_f64(fit, dv) : accumulate_split(fit, dv);
+    const int prec = fit->opt.precision;
+    int rc = prec == RP_PREC_FP64        ? accumulate_f64(fit, dv)
+             : prec == RP_PREC_FP64_FACT ? accumulate_f64_factorized(fit, dv)
+                                         : accumulate_split(fit, dv);
     if (rc != RP_OK) return rc;
     k_symmetrize<<<ceil_div(g2, 256), 256, 0, st>>>(fit->d_gram.as<double>(), dv.NP, 128);
     RP_HIP(ctx, hipGetLastError());
@@ -354,7 +357,8 @@ int load_system(rp_fit *fit, const rp_system *sys, const rp_options *opt) {
         return ctx->fail(RP_EINVAL, "rp_fit_create: system needs atoms, positions and ESP");
     if (sys->esp_dtype != RP_F64 && sys->esp_dtype != RP_F32) return ctx->fail(RP_EINVAL, "bad esp_dtype");
     if (sys->esp_order != RP_ORDER_CUBE && sys->esp_order != RP_ORDER_FLAT) return ctx->fail(RP_EINVAL, "bad esp_order");
-    if (opt->precision != RP_PREC_FP64 && opt->precision != RP_PREC_OZAKI && opt->precision != RP_PREC_OZAKI_REFINED)
+    if (opt->precision != RP_PREC_FP64 && opt->precision != RP_PREC_OZAKI && opt->precision != RP_PREC_OZAKI_REFINED &&
+        opt->precision != RP_PREC_FP64_FACT)
         return ctx->fail(RP_EINVAL, "bad precision");
     fit->opt = *opt;
     fit->n_atoms = sys->n_atoms;
@@ -370,6 +374,7 @@ int load_system(rp_fit *fit, const rp_system *sys, const rp_options *opt) {
     fit->refine_steps_done = 0;
     fit->refine_last_dq = 0.0;
     fit->refine_bound = -1.0;
+    fit->sensitivity = -1.0;
     fit->solver = RP_SOLVER_CHOLESKY;
     fit->na = 0;
     if (fit->dev) fit->dev->ready = false;
@@ -638,7 +643,8 @@ int rp_fit_solve_report(const rp_fit *fit, rp_solve_report *out) {
     out->last_correction = fit->refine_last_dq;
     out->error_bound = refined ? fit->refine_bound : -1.0;
     out->within_target = refined ? (fit->refine_bound >= 0.0 && fit->refine_bound <= RP_REFINE_TARGET_E) : 1;
-    out->reserved = 0;
+    out->sensitivity = fit->sensitivity;
+    out->well_conditioned = fit->sensitivity >= 0.0 && fit->sensitivity <= RP_SENSITIVITY_LIMIT_E;
     return RP_OK;
 }
 
@@ -652,7 +658,7 @@ int rp_fit_timings(const rp_fit *fit, double *t3) {
 }
 
 int rp_chunk_plan(int64_t n_keep, int32_t n_atoms, int32_t n_k, int32_t precision, rp_chunk_plan_info *out) {
-    if (!out || n_keep < 1 || n_atoms < 1 || n_k < 0 || precision < RP_PREC_FP64 || precision > RP_PREC_OZAKI_REFINED)
+    if (!out || n_keep < 1 || n_atoms < 1 || n_k < 0 || precision < RP_PREC_FP64 || precision > RP_PREC_FP64_FACT)
         return RP_EINVAL;
     const ChunkPlan c = plan_chunks(n_keep, n_atoms, n_k, precision);
     out->rows = c.rows;
@@ -661,8 +667,9 @@ int rp_chunk_plan(int64_t n_keep, int32_t n_atoms, int32_t n_k, int32_t precisio
     out->kp = c.KP;
     out->splits = c.splits;
     out->n_tiles = c.n_tiles;
-    out->slices = precision == RP_PREC_FP64 ? 0 : c.slices;
-    out->stage_rows = precision == RP_PREC_FP64 ? f64_gram_stage_rows : gram_stage_rows(c.slices);
+    const bool f64_gram = precision == RP_PREC_FP64 || precision == RP_PREC_FP64_FACT;
+    out->slices = f64_gram ? 0 : c.slices;
+    out->stage_rows = f64_gram ? f64_gram_stage_rows : gram_stage_rows(c.slices);
     return RP_OK;
 }
 

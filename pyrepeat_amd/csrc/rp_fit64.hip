@@ -178,7 +178,25 @@ __global__ __launch_bounds__(256) void k_phi_gemm_f64(const double *__restrict__
         }
 }
 
-// Upper-triangular tiles of Phi^T Phi over rows [split*rps, ...), one 128x128 tile per block.
+// Upper-triangular tiles of Phi^T Phi over rows [split*rps, ...), one 128x128 tile per block.  Phi is a
+// chunk of `rows` rows, row-major [rows][NP] (the fp64 GEMM's output) or column-major [NP][rows] (the
+// factorized basis kernel's output, ColMajor).
+template <bool ColMajor>
+__device__ __forceinline__ void syrk_load(const double *Phi, int NP, int64_t rows, int64_t r, int col0, double (&v)[8]) {
+    if constexpr (ColMajor)
+        TileLoader::load_a(Phi, rows, col0, (int)r, v);
+    else
+        TileLoader::load_b(Phi, NP, r, col0, v);
+}
+template <bool ColMajor>
+__device__ __forceinline__ void syrk_store(double (*S)[BM], const double (&v)[8]) {
+    if constexpr (ColMajor)
+        TileLoader::store_a(S, v);
+    else
+        TileLoader::store_b(S, v);
+}
+
+template <bool ColMajor>
 __global__ __launch_bounds__(256) void k_syrk_f64(const double *__restrict__ Phi, int NP, int64_t rows, int rps,
                                                   const int2 *__restrict__ tiles, int n_tiles, double *__restrict__ out,
                                                   int direct) {
@@ -193,23 +211,24 @@ __global__ __launch_bounds__(256) void k_syrk_f64(const double *__restrict__ Phi
 #pragma unroll
         for (int b = 0; b < 4; ++b) acc[a][b] = dvec4{0.0, 0.0, 0.0, 0.0};
     const int nk = (int)((r1 - r0) / BK);
+    static_assert(BM == BN, "square tiles: one loader serves both operands");
     if (nk > 0) {
         double ra[8], rb[8];
-        TileLoader::load_b(Phi, NP, r0, tile.x * BM, ra);
-        TileLoader::load_b(Phi, NP, r0, tile.y * BN, rb);
-        TileLoader::store_b(reinterpret_cast<double(*)[BN]>(As[0]), ra);
-        TileLoader::store_b(Bs[0], rb);
+        syrk_load<ColMajor>(Phi, NP, rows, r0, tile.x * BM, ra);
+        syrk_load<ColMajor>(Phi, NP, rows, r0, tile.y * BN, rb);
+        syrk_store<ColMajor>(As[0], ra);
+        syrk_store<ColMajor>(Bs[0], rb);
         __syncthreads();
         for (int kt = 0; kt < nk; ++kt) {
             const bool more = kt + 1 < nk;
             if (more) {
-                TileLoader::load_b(Phi, NP, r0 + (int64_t)(kt + 1) * BK, tile.x * BM, ra);
-                TileLoader::load_b(Phi, NP, r0 + (int64_t)(kt + 1) * BK, tile.y * BN, rb);
+                syrk_load<ColMajor>(Phi, NP, rows, r0 + (int64_t)(kt + 1) * BK, tile.x * BM, ra);
+                syrk_load<ColMajor>(Phi, NP, rows, r0 + (int64_t)(kt + 1) * BK, tile.y * BN, rb);
             }
             mma_tile(As[kt & 1], Bs[kt & 1], acc);
             if (more) {
-                TileLoader::store_b(reinterpret_cast<double(*)[BN]>(As[(kt + 1) & 1]), ra);
-                TileLoader::store_b(Bs[(kt + 1) & 1], rb);
+                syrk_store<ColMajor>(As[(kt + 1) & 1], ra);
+                syrk_store<ColMajor>(Bs[(kt + 1) & 1], rb);
             }
             __syncthreads();
         }
@@ -337,7 +356,7 @@ int accumulate_f64(rp_fit *fit, FitDevice &dv) {
         }
         dim3 gs(n_tiles, n_splits);
         KTimer kt(ctx, KT_GRAM);
-        k_syrk_f64<<<gs, 256, 0, st>>>(d_phi.as<double>(), NP, rows, rps, dv.d_tiles.as<int2>(), n_tiles,
+        k_syrk_f64<false><<<gs, 256, 0, st>>>(d_phi.as<double>(), NP, rows, rps, dv.d_tiles.as<int2>(), n_tiles,
                                        n_splits > 1 ? d_part.as<double>() : fit->d_gram.as<double>(), n_splits > 1 ? 0 : 1);
     }
     if (n_splits > 1) {
@@ -348,6 +367,78 @@ int accumulate_f64(rp_fit *fit, FitDevice &dv) {
     RP_HIP(ctx, hipGetLastError());
     RP_HIP(ctx, hipStreamSynchronize(st));  // chunk buffers are released on return
     ctx->collect();
+    return RP_OK;
+}
+
+namespace {
+
+// Rows [valid, rows) of every column of a column-major chunk: the factorized basis kernel writes only the
+// chunk's kept points, and the Gram reads whole stages.
+__global__ __launch_bounds__(256) void k_zero_tail(double *__restrict__ phi, int64_t rows, int64_t valid) {
+    const int64_t r = valid + (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (r < rows) phi[(int64_t)blockIdx.y * rows + r] = 0.0;
+}
+
+__global__ __launch_bounds__(256) void k_zero_f64(double *__restrict__ p, int64_t n) {
+    const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (t < n) p[t] = 0.0;
+}
+
+}  // namespace
+
+// RP_PREC_FP64_FACT: the basis from the factorized reciprocal sum (k_sep_phi, fp64, column-major chunks)
+// and the Gram of [Phi | V | 1] on fp64 MFMA: fp64 throughout, like the plain path, without its
+// K-wide trig tables and GEMM.
+int accumulate_f64_factorized(rp_fit *fit, FitDevice &dv) {
+    Ctx *ctx = fit->ctx;
+    hipStream_t st = ctx->stream;
+    const int N = fit->n_atoms, NP = dv.NP;
+    const int64_t M = fit->n_keep;
+    const ChunkPlan cp = plan_chunks(M, N, dv.K, RP_PREC_FP64_FACT);
+    const int64_t rows = cp.rows;
+    const int n_tiles = (int)dv.tiles.size();
+    const int n_splits = cp.splits;
+    if (cp.NP != NP || cp.n_tiles != n_tiles || rows % (n_splits * BK) != 0)
+        return ctx->fail(RP_EINVAL, "fp64 factorized path: chunk plan does not match the Gram tiling");
+    DBuf &d_phi = fit->ws[2], &d_part = fit->ws[3], &d_colmax = fit->ws[5];
+    SepPlan plan;
+    {
+        KTimer kt(ctx, KT_PHASE);
+        RP_CHECK(sep_prepare(fit, dv, rows, plan));
+    }
+    RP_CHECK(d_phi.grow(ctx, sizeof(double) * (size_t)NP * rows));
+    RP_CHECK(d_colmax.grow(ctx, sizeof(unsigned long long) * NP));
+    const int64_t part_n = (int64_t)n_splits * n_tiles * BM * BN;
+    if (n_splits > 1) {
+        RP_CHECK(d_part.grow(ctx, sizeof(double) * (size_t)part_n));
+        k_zero_f64<<<ceil_div(part_n, 256), 256, 0, st>>>(d_part.as<double>(), part_n);
+    }
+    const int rps = (int)(rows / n_splits);
+    int chunk = 0;
+    for (int64_t r0 = 0; r0 < M; r0 += rows, ++chunk) {
+        const int64_t valid = std::min<int64_t>(rows, M - r0);
+        {
+            KTimer kt(ctx, KT_KSPACE);
+            sep_phi_chunk(fit, dv, plan, chunk, r0, valid, rows, d_phi.as<double>(),
+                          d_colmax.as<unsigned long long>(), st);
+        }
+        if (valid < rows) {
+            dim3 gz((unsigned)ceil_div(rows - valid, 256), (unsigned)NP);
+            k_zero_tail<<<gz, 256, 0, st>>>(d_phi.as<double>(), rows, valid);
+        }
+        dim3 gs(n_tiles, n_splits);
+        KTimer kt(ctx, KT_GRAM);
+        k_syrk_f64<true><<<gs, 256, 0, st>>>(d_phi.as<double>(), NP, rows, rps, dv.d_tiles.as<int2>(), n_tiles,
+                                             n_splits > 1 ? d_part.as<double>() : fit->d_gram.as<double>(),
+                                             n_splits > 1 ? 0 : 1);
+    }
+    if (n_splits > 1) {
+        KTimer kt(ctx, KT_REDUCE);
+        k_reduce_splits<<<n_tiles, 256, 0, st>>>(d_part.as<double>(), n_splits, n_tiles, dv.d_tiles.as<int2>(),
+                                                 fit->d_gram.as<double>(), NP);
+    }
+    RP_HIP(ctx, hipGetLastError());
+    fit->phi_kept = false;
     return RP_OK;
 }
 

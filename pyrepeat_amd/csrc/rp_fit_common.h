@@ -129,6 +129,7 @@ int prepare_device(rp_fit *fit, FitDevice &dv, int precision);
 int device_setup(rp_fit *fit, int precision, FitDevice *&dv);
 int accumulate_f64(rp_fit *fit, FitDevice &dv);
 int accumulate_split(rp_fit *fit, FitDevice &dv);
+int accumulate_f64_factorized(rp_fit *fit, FitDevice &dv);
 
 // Factorized reciprocal sum of the split form (rp_kspace.hip).
 struct SepPlan {

@@ -162,6 +162,17 @@ inline ChunkPlan plan_chunks(int64_t M, int N, int K, int precision) {
         c.n_tiles = nt * (nt + 1) / 2;
         while ((int64_t)c.n_tiles * c.splits < 512 && c.rows / (c.splits * 2) >= 1024 && (c.rows / (c.splits * 2)) % 64 == 0)
             c.splits *= 2;
+    } else if (precision == RP_PREC_FP64_FACT) {
+        // factorized basis (chunks of whole 512-row blocks, column-major) and the fp64 Gram's tiles
+        c.slices = 0;
+        int64_t rows = std::max<int64_t>(512, std::min<int64_t>(65536, round_up(M, 512)));
+        while (rows > 512 && rows * 8 * c.NP > ((int64_t)1 << 31)) rows /= 2;
+        c.rows = std::max<int64_t>(512, rows - rows % 512);
+        const int nt = c.NP / 128;
+        c.n_tiles = nt * (nt + 1) / 2;
+        while ((int64_t)c.n_tiles * c.splits < 512 && c.rows / (c.splits * 2) >= 1024 &&
+               (c.rows / (c.splits * 2)) % f64_gram_stage_rows == 0)
+            c.splits *= 2;
     } else {
         c.KP = 0;  // the factorized reciprocal sum needs no k-space operand
         // refined: 3 slices (the refinement removes their error); otherwise 4
@@ -292,6 +303,7 @@ struct rp_fit {
     int32_t refine_steps_done = 0;
     double refine_last_dq = 0.0;
     double refine_bound = -1.0;     // estimated error left by the refinement (e), -1: none
+    double sensitivity = -1.0;      // max |dq| (e) from a perturbation of the system by 8 ulps, -1: none
     int32_t solver = RP_SOLVER_CHOLESKY;   // how the last solve went
     rp::FitDevice *dev = nullptr;   // persistent device copy of the Ewald setup
     ~rp_fit();

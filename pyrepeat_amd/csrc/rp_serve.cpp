@@ -124,12 +124,17 @@ uint64_t fit_bytes_bound(const Setup &s, int n_atoms, int esp_dtype, int precisi
     uint64_t sep = 0;
     if (precision != RP_PREC_FP64) {
         const ChunkPlan c = plan_chunks(M, n_atoms, (int)K, precision);
-        if (precision == RP_PREC_OZAKI) at(2, 8 * (uint64_t)c.NP * c.rows);   // the refined form keeps ws[7]
-        at(3, 8 * split_cap(c.rows, c.n_tiles) * c.n_tiles * 64 * 128);
-        at(4, (uint64_t)c.slices * c.NP * c.rows);
+        // one fp64 chunk buffer (the refined form keeps every chunk in ws[7] instead)
+        if (precision != RP_PREC_OZAKI_REFINED) at(2, 8 * (uint64_t)c.NP * c.rows);
         at(5, 8 * (uint64_t)c.NP);
-        at(6, 8 * (uint64_t)c.NP);
-        at(9, 8 * (uint64_t)c.n_tiles);
+        if (precision == RP_PREC_FP64_FACT) {
+            at(3, 8 * split_cap(c.rows, c.n_tiles) * c.n_tiles * 128 * 128);   // fp64 Gram partials
+        } else {
+            at(3, 8 * split_cap(c.rows, c.n_tiles) * c.n_tiles * 64 * 128);
+            at(4, (uint64_t)c.slices * c.NP * c.rows);
+            at(6, 8 * (uint64_t)c.NP);
+            at(9, 8 * (uint64_t)c.n_tiles);
+        }
         // factorized reciprocal sum: k box, T1 per plane (8 x 13 complex per plane and atom),
         // line table of the kept points, per-chunk line ranges
         const uint64_t nb = (uint64_t)ceil_div(M, 256), n_chunks = (uint64_t)ceil_div(M, c.rows);

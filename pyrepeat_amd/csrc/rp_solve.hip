@@ -53,6 +53,8 @@ struct RefineState {
 // Contraction assumed for the bound after a single step, where no ratio of corrections exists yet:
 // twice the largest per-step contraction measured (5.3e-3, a 5000-atom cell with cond(A) = 4.9e10).
 constexpr double RHO_PRIOR = 1e-2;
+// Relative perturbation of the sensitivity estimate: 8 units in the last place of an fp64 entry.
+constexpr double SENS_EPS = 8.0 * 2.220446049250313e-16;
 
 __global__ __launch_bounds__(256) void k_assemble(const double *__restrict__ gram, int NP, int N,
                                                   const int *__restrict__ red_ptr, const int *__restrict__ red_src,
@@ -1204,6 +1206,45 @@ __global__ __launch_bounds__(256) void k_ref_update(double *__restrict__ x, cons
     }
 }
 
+// Sensitivity of the charges to the rounding of the system in fp64: the right-hand side
+// z_i = +-eps sum_j |K_ij| |x_j| (eps = 8 units in the last place, signs pseudo-random), whose solution
+// with the same factor is the charge change that a perturbation of every matrix entry by eps relative
+// causes; one work-group per row.  Also arms the state that gates that solve.
+__global__ __launch_bounds__(256) void k_sens_rhs(const double *__restrict__ K, int ld, int nr, const double *__restrict__ x,
+                                                  double eps, double *__restrict__ z, const int *__restrict__ cinfo,
+                                                  RefineState *__restrict__ sens) {
+    __shared__ double red[4];
+    if (cinfo[0] != 0 || cinfo[1] != 0) return;
+    const int i = blockIdx.x;
+    double acc = 0.0;
+    for (int j = threadIdx.x; j < nr; j += 256) acc = fma(fabs(K[(int64_t)i * ld + j]), fabs(x[j]), acc);
+    const double sum = block_sum_256(acc, red);
+    if (threadIdx.x == 0) {
+        const unsigned h = (unsigned)i * 2654435761u;
+        z[i] = ((h >> 15) & 1u ? eps : -eps) * sum;
+        if (i == 0) {
+            sens->go = 1;
+            sens->steps = 0;
+            sens->last_dq = 0.0;
+            sens->bound = -1.0;
+        }
+    }
+}
+
+// *out = max |dq| of the sensitivity solve (-1 when the solve did not run); one work-group.
+__global__ __launch_bounds__(256) void k_sens_max(const double *__restrict__ dq, int nr, const int *__restrict__ cinfo,
+                                                  double *__restrict__ out) {
+    __shared__ double red[4];
+    double m = 0.0;
+    for (int i = threadIdx.x; i < nr; i += 256) m = fmax(m, fabs(dq[i]));
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o));
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0)
+        *out = (cinfo[0] != 0 || cinfo[1] != 0) ? -1.0 : fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
+}
+
 // Restarts the refinement after a fallback solve: clears the pivot / wait flags and the step state.
 __global__ void k_ref_reset(int *__restrict__ cinfo, RefineState *__restrict__ rs) {
     cinfo[0] = 0;
@@ -1555,7 +1596,7 @@ int run_solve(rp_fit *fit, const rp_constraints *con, double *q_out, double *sol
     const size_t n_int = (size_t)(nr + 1) + nsrc + N + (size_t)(N + 1) + nsrc;
     const size_t off_d = round_up((int64_t)(4 * n_int), 8);
     const size_t stage = round_up((int64_t)(off_d + 8 * (2 * (size_t)nsrc + 3 * (size_t)nr)), 16);
-    const size_t out_bytes = sizeof(double) * (4 + (size_t)N + n);
+    const size_t out_bytes = sizeof(double) * (5 + (size_t)N + n);
     if (fit->h_stage_bytes < stage + out_bytes) {
         if (fit->h_stage) (void)hipHostFree(fit->h_stage);
         fit->h_stage = nullptr;
@@ -1620,8 +1661,9 @@ int run_solve(rp_fit *fit, const rp_constraints *con, double *q_out, double *sol
     const bool refine = fit->opt.precision == RP_PREC_OZAKI_REFINED && fit->phi_kept;
     DBuf &d_rv = fit->ws[10], &d_e = fit->ws[8];
     const size_t rv_doubles = 2 * (size_t)N + 3 * (size_t)nr + 2;
-    RP_CHECK(d_rv.grow(ctx, sizeof(double) * round_up((int64_t)rv_doubles, 2) + sizeof(RefineState)));
+    RP_CHECK(d_rv.grow(ctx, sizeof(double) * round_up((int64_t)rv_doubles, 2) + 2 * sizeof(RefineState)));
     RefineState *d_rs = reinterpret_cast<RefineState *>(d_rv.as<double>() + round_up((int64_t)rv_doubles, 2));
+    RefineState *d_sens = d_rs + 1;   // gates the sensitivity solve
     const int n_esum = (int)ceil_div(fit->n_keep, 512);
     if (refine) RP_CHECK(d_e.grow(ctx, sizeof(double) * ((size_t)fit->n_keep + 2 + (size_t)n_esum)));
 
@@ -1720,6 +1762,23 @@ int run_solve(rp_fit *fit, const rp_constraints *con, double *q_out, double *sol
             k_ref_update<<<1, 256, 0, st>>>(d_x.as<double>(), dx, nr, RP_REFINE_TARGET_E, d_cinfo.as<int>(), d_rs);
         }
     }
+    {
+        // sensitivity to the fp64 rounding of the system: one more solve with the same factor (the
+        // refinement's vectors are free again: r1 <- z, z <- L^-1 z, dx <- the charge change)
+        KTimer kts2(ctx, KT_SOLVE);
+        double *u = d_rv.as<double>(), *r = u + N, *sz = r + N, *szz = sz + nr, *sdx = szz + nr;
+        k_sens_rhs<<<nr, 256, 0, st>>>(d_K2.as<double>(), ld, nr, d_x.as<double>(), SENS_EPS, sz, d_cinfo.as<int>(),
+                                       d_sens);
+        k_fwdsub<<<nblk, 256, 0, st>>>(d_K.as<double>(), ld, nr, d_dinv.as<double>(), sz, szz, d_sync.as<unsigned>(),
+                                       fit->bs_base, ++fit->bs_epoch, spin_limit, d_cinfo.as<int>() + 1,
+                                       d_cinfo.as<int>(), d_sens);
+        fit->bs_base += (unsigned)nblk;
+        k_backsub<<<nblk, 256, 0, st>>>(d_K.as<double>(), ld, nr, d_dinv.as<double>(), d_wc.as<double>(), szz, 0.0, sdx,
+                                        d_sync.as<unsigned>(), fit->bs_base, ++fit->bs_epoch, spin_limit,
+                                        d_cinfo.as<int>() + 1, d_cinfo.as<int>(), nullptr, &d_sens->go);
+        fit->bs_base += (unsigned)nblk;
+        k_sens_max<<<1, 256, 0, st>>>(sdx, nr, d_cinfo.as<int>(), h_out_dev + 4 + N + n);
+    }
     k_publish<<<ceil_div(N + 1, 256), 256, 0, st>>>(d_x.as<double>(), dc_exp, N, n, d_cinfo.as<int>(), d_rs, h_out_dev);
     RP_HIP(ctx, hipGetLastError());
     RP_HIP(ctx, hipEventRecord(e1, st));
@@ -1732,6 +1791,9 @@ int run_solve(rp_fit *fit, const rp_constraints *con, double *q_out, double *sol
         fit->refine_steps_done = refine ? (int32_t)h_out[1 + N + n] : 0;
         fit->refine_last_dq = refine ? h_out[2 + N + n] : 0.0;
         fit->refine_bound = refine ? h_out[3 + N + n] : -1.0;
+        fit->sensitivity = h_out[4 + N + n];
+    } else {
+        fit->sensitivity = -1.0;   // not estimated after a fallback
     }
     fit->lu_fallback = cinfo != 0;
     if (cinfo < 0) {

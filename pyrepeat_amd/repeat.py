@@ -2,7 +2,7 @@
 
     python -m pyrepeat_amd system.cube [--fit-type 1] [--vdw-factor 1.0] [--vdw-max 1000]
         [--cutoff 20] [--total-charge 0] [--symm-file F] [--resp-file F] [--qeq-file F]
-        [--stats] [--output F] [--fp64] [--precision fp64|split|refined] [--charge repeat|qeq]
+        [--stats] [--output F] [--fp64] [--precision fp64|split|refined|fp64-factorized] [--charge repeat|qeq]
 
 Every fit runs on the GPU through ``librepeat_hip.so``.  As with the reference's
 ``--gpu``, the default precision is a split (mixed) form: ``refined`` (int8-sliced
@@ -41,7 +41,7 @@ def build_parser():
     p.add_argument("--no-jit", action="store_true")
     p.add_argument("--gpu", nargs="?", const="0", default=None)
     p.add_argument("--fp64", action="store_true", help="plain fp64 path")
-    p.add_argument("--precision", choices=["fp64", "split", "refined"], default=None,
+    p.add_argument("--precision", choices=["fp64", "split", "refined", "fp64-factorized"], default=None,
                    help="explicit precision form (overrides --fp64)")
     p.add_argument("--output", type=str, default=None)
     p.add_argument("--charge", type=str, default="repeat", choices=["repeat", "qeq"])

@@ -339,4 +339,17 @@ def tiny(seed=0, n_atoms=3, n=10, L=9.0, triclinic=False):
     return _finish(f"tiny{seed}", rng, n_grid, box / n_grid[:, None], [Z_OF[e] for e in el], pos, q, np.float64)
 
 
+def cluster(n_atoms=400, sub=18.0, L=36.0, n=64, seed=1):
+    """A compact cluster of ``n_atoms`` (C/H/O, min separation 1.6 Bohr) in the middle of an ``L`` Bohr
+    cubic box: the grid points lie outside the cluster, so the buried atoms' basis columns are nearly
+    dependent (the classic ill-conditioned ESP fit; cond(A) ~ 1e11-1e12 at 200-400 atoms)."""
+    rng = np.random.default_rng([77, seed])
+    pos = place_atoms(rng, np.eye(3) * sub, n_atoms, 1.6) + (L - sub) / 2
+    el = element_list(rng, n_atoms, {"C": 0.5, "H": 0.3, "O": 0.2})
+    q = rng.uniform(-0.5, 0.5, n_atoms)
+    q -= q.mean()
+    return _finish(f"cluster{n_atoms}", rng, np.array([n, n, n]), np.eye(3) * (L / n), [Z_OF[e] for e in el], pos, q,
+                   np.float64)
+
+
 WORKLOADS = {"water": water, "crystal200": crystal200, "mof1000": mof1000, "large5000": large5000}

@@ -191,6 +191,31 @@ def test_fit_type0_matches_oracle(oracle, gpu_ctx):
     calc.close()
 
 
+@pytest.mark.parametrize("name", ["water", "crystal200"])
+def test_fp64_factorized_matches_fp64(gpu_ctx, tmp_path, name):
+    """The fp64 form with the factorized basis: its normal equations equal the plain fp64 path's to fp64
+    rounding (the same basis by another summation), its charges the golden charges within 1e-6 e."""
+    g = _golden(name)
+    s = getattr(synth, name)()
+    print(f"fp64-factorized {name}", flush=True)
+    out = {}
+    for prec in (_abi.RP_PREC_FP64, _abi.RP_PREC_FP64_FACT):
+        f = _abi.Fit(gpu_ctx, s.n_grid, s.axis, s.atomic_numbers, s.atom_pos, s.esp_cube, _abi.RP_ORDER_CUBE, 1, prec)
+        f.accumulate()
+        out[prec] = f.normal_equations()
+        f.close()
+    (a64, b64, s64, v64), (af, bf, sf, vf) = out[_abi.RP_PREC_FP64], out[_abi.RP_PREC_FP64_FACT]
+    np.testing.assert_allclose(af, a64, rtol=1e-11, atol=1e-13 * np.abs(a64).max())
+    np.testing.assert_allclose(bf, b64, rtol=1e-11, atol=1e-13 * np.abs(b64).max())
+    np.testing.assert_allclose(sf, s64, rtol=1e-11, atol=1e-13 * np.abs(s64).max())
+    calc = _calc(gpu_ctx, s, "fp64-factorized", tmp_path)
+    q = calc.fit_charges()
+    calc.close()
+    dq = np.max(np.abs(q - np.array(g["charges"])))
+    print(f"  max|dq| vs golden {dq:.2e}", flush=True)
+    assert dq <= 1e-6
+
+
 def _duplicate_atom_case(oracle):
     """Three atoms, the second at the first's place: identical basis columns, an exactly singular KKT
     matrix.  Its minimum-norm solution (the reference's lstsq branch) is the fit without the second atom,

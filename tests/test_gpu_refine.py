@@ -94,6 +94,47 @@ def test_refined_small_serving_cells(oracle, gpu_ctx, tmp_path, n_atoms, r):
     assert abs(out["refined"].sum() - s.q_tot) < 1e-9
 
 
+def test_refined_flags_ill_conditioned_cell(gpu_ctx, capfd):
+    """A 400-atom cluster in a 36 Bohr box: points only outside it, cond(A) ~ 2e12, unrestrained charges of
+    hundreds of e.  Any two fp64 solutions of this system differ far above 1e-5 e (the refined and the
+    plain fp64 path by ~1e-2 e), so the refined charges either meet the fp64 path's within 1e-5 e or the
+    solve report flags the fit: its sensitivity (charge change under an 8-ulp perturbation of the system)
+    must cover the gap, and the calculator warns."""
+    s = synth.cluster(400)
+    con = params.constraint_arrays(s.n_atoms, s.atomic_numbers, 0.0, None, None, None)
+    print("ill-conditioned cluster: 400 atoms, 64^3 grid", flush=True)
+    out = {}
+    for prec in (_abi.RP_PREC_FP64, _abi.RP_PREC_OZAKI_REFINED):
+        f = _abi.Fit(gpu_ctx, s.n_grid, s.axis, s.atomic_numbers, s.atom_pos, s.esp_cube, _abi.RP_ORDER_CUBE, 1, prec)
+        f.accumulate()
+        q, _ = f.solve(con)
+        out[prec] = (q, f.solve_report())
+        if prec == _abi.RP_PREC_FP64:
+            ata, _, sp, _ = f.normal_equations()
+            ev = np.linalg.eigvalsh(ata - np.outer(sp, sp) / f.info.n_keep)
+        f.close()
+    cond = ev[-1] / ev[0]
+    dq = np.max(np.abs(out[_abi.RP_PREC_FP64][0] - out[_abi.RP_PREC_OZAKI_REFINED][0]))
+    rep = out[_abi.RP_PREC_OZAKI_REFINED][1]
+    print(f"  cond(A) {cond:.2e}, refined - fp64 {dq:.2e}, report {rep}", flush=True)
+    assert cond >= 1e11
+    assert dq <= 1e-5 or not rep["well_conditioned"] or not rep["within_target"]
+    assert rep["sensitivity"] >= dq
+    calc = ChargeCalculator(cube=cube_from_system(s), precision="refined", context=gpu_ctx)
+    calc.fit_charges()
+    calc.close()
+    assert "ill-conditioned fit" in capfd.readouterr().err
+    # a well-conditioned cell is not flagged
+    w = synth.crystal200()
+    f = _abi.Fit(gpu_ctx, w.n_grid, w.axis, w.atomic_numbers, w.atom_pos, w.esp_cube, _abi.RP_ORDER_CUBE, 1,
+                 _abi.RP_PREC_OZAKI_REFINED)
+    f.accumulate()
+    f.solve(params.constraint_arrays(w.n_atoms, w.atomic_numbers, 0.0, None, None, None))
+    rep = f.solve_report()
+    f.close()
+    assert rep["well_conditioned"] and rep["within_target"] and 0.0 <= rep["sensitivity"] <= 1e-8
+
+
 def test_refined_deterministic_and_basis_kept(gpu_ctx, tmp_path):
     """Two fits are bitwise equal; the --stats matvec (its own chunk buffer) between two solves leaves
     the kept basis intact, so the second solve repeats the first bitwise; refit does the same."""

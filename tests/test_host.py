@@ -134,7 +134,8 @@ def test_request_bound_on_host_and_invalid_requests():
     assert e.value.code == _abi.RP_EINVAL
 
 
-@pytest.mark.parametrize("precision", [_abi.RP_PREC_FP64, _abi.RP_PREC_OZAKI, _abi.RP_PREC_OZAKI_REFINED])
+@pytest.mark.parametrize("precision", [_abi.RP_PREC_FP64, _abi.RP_PREC_OZAKI, _abi.RP_PREC_OZAKI_REFINED,
+                                       _abi.RP_PREC_FP64_FACT])
 def test_chunk_plan_row_splits_hold_whole_gram_stages(precision):
     """Every row split of every plan is whole stages of its Gram kernel (128 rows for 3 slices,
     64 for 4, 16 for fp64) and short enough for exact int32 slice groups.  The kept counts of small
@@ -149,7 +150,9 @@ def test_chunk_plan_row_splits_hold_whole_gram_stages(precision):
             assert c["rows"] % (c["splits"] * c["stage_rows"]) == 0, (n_atoms, m, c)
             assert rps >= c["stage_rows"] and c["n_chunks"] * c["rows"] >= m
             assert c["np"] % 128 == 0 and c["np"] >= n_atoms + 2
-            if precision != _abi.RP_PREC_FP64:
+            if precision == _abi.RP_PREC_FP64_FACT:
+                assert c["slices"] == 0 and c["rows"] % 512 == 0 and c["rows"] * 8 * c["np"] <= 2**31
+            elif precision != _abi.RP_PREC_FP64:
                 p = c["slices"]
                 assert p == (3 if precision == _abi.RP_PREC_OZAKI_REFINED else 4)
                 assert c["rows"] % 512 == 0
