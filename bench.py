@@ -5,10 +5,13 @@
 A step is one whole charge fit from a device-resident cube (vdW filter, basis
 streaming with normal-equation accumulation, constrained solve), the hot path
 the reference runs as ``_filter_grid`` + ``fit_charges``.  The chosen form is
-the split precision path (int8 fixed-point slices on MFMA with exact int32
-accumulation, fp64 across tiles and in the solve); the plain fp64 GPU path
-runs on the same inputs as the denominator of the speed ratio.  Inputs are the
-seeded synthetic workload (exact Ewald ESP plus noise) named in ``--workload``.
+the refined split form (int8 fixed-point slices on MFMA with exact int32
+accumulation, fp64 across tiles and in the solve, fp64 iterative refinement
+against the kept basis); the plain fp64 GPU path (the reference's cos/sin
+GEMM formulation) runs on the same inputs as the denominator of the speed
+ratio, and the fp64 form with the factorized basis beside it, so the gain of
+the precision form itself shows.  Inputs are the seeded synthetic workload
+(exact Ewald ESP plus noise) named in ``--workload``.
 
 Prints one JSON line (rank 0, N = 1).
 
@@ -16,10 +19,9 @@ Prints one JSON line (rank 0, N = 1).
 
 runs the serving stream instead: seeded requests of 20-800 atoms arrive as a
 Poisson process at 20 requests/s and are fitted concurrently by the serving
-loop (split precision); the line reports open-loop fits/s with p50/p99
-latency (completion - arrival), the saturated capacity (every request queued
-at once), and the charges of a sample against the same requests run alone and
-in plain fp64.
+loop; the line reports open-loop fits/s with p50/p99 latency (completion -
+arrival), the saturated capacity (every request queued at once), and the
+charges of a sample against the same requests run alone and in plain fp64.
 """
 from __future__ import annotations
 
@@ -72,6 +74,17 @@ def traffic_from_profile(kernel, workload):
     return None
 
 
+def flops_from_profile(kernel, workload):
+    """Executed fp64 flops per launch of `kernel` from the committed counter summary, if there is one."""
+    path = os.path.join(ROOT, "profiles", f"{kernel}_{workload}_fp64_flops.txt")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("executed_flops_per_launch:"):
+            return float(line.split(":", 1)[1])
+    return None
+
+
 def _workload(name):
     from pyrepeat_amd import synth
     if name.startswith("stream"):
@@ -97,7 +110,8 @@ def _constraints(system, tmpdir):
 
 def run_gpu(ctx, system, con, precision, steps, warmup):
     from pyrepeat_amd import _abi
-    prec = {"fp64": _abi.RP_PREC_FP64, "split": _abi.RP_PREC_OZAKI, "refined": _abi.RP_PREC_OZAKI_REFINED}[precision]
+    prec = {"fp64": _abi.RP_PREC_FP64, "split": _abi.RP_PREC_OZAKI, "refined": _abi.RP_PREC_OZAKI_REFINED,
+            "fp64-factorized": _abi.RP_PREC_FP64_FACT}[precision]
     fit = _abi.Fit(ctx, system.n_grid, system.axis, system.atomic_numbers, system.atom_pos, system.esp_cube,
                    _abi.RP_ORDER_CUBE, 1, prec, 1.0, 1000.0, 20.0)
     prepared = fit._constraints(con)
@@ -121,21 +135,31 @@ def run_gpu(ctx, system, con, precision, steps, warmup):
         fit.refit(con, prepared)
     stats = ctx.kernel_stats()
     ctx.profile(False)
+    report = fit.solve_report()
     info = fit.info
     M = int(info.n_keep)
     N = int(fit.n_atoms)
-    np_ = -(-(N + 2) // 128) * 128
-    slices = 3 if prec == _abi.RP_PREC_OZAKI_REFINED else 4
-    if prec != _abi.RP_PREC_FP64:     # chunk plan of the split path: <= 65536 rows, ~2 GiB of chunk buffers
-        rows = max(512, min(65536, -(-M // 512) * 512))
-        while rows > 512 and rows * (8 + slices) * np_ > 1 << 31:
-            rows //= 2
-        rows -= rows % 512
-    else:
-        rows = min(65536, -(-M // 128) * 128)
+    plan = _abi.chunk_plan(M, N, int(info.n_k), prec)
+    rows = plan["rows"]
+    # grid lines holding kept points, per chunk (a line cut by a chunk boundary counts in both chunks,
+    # as the basis kernel forms its reciprocal table in both)
+    _, _, idx = fit.grid()
+    key = idx // int(system.n_grid[0])
+    lines = [int(np.unique(key[c * rows:(c + 1) * rows]).size) for c in range(plan["n_chunks"])]
+    # end to end from host arrays: the cube upload, the host Ewald setup and the filter (rp_fit_reset), the
+    # accumulate and the solve, i.e. a whole fit as a caller holding the ESP in host memory sees it
+    e2e = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        fit.reset(system.n_grid, system.axis, system.atomic_numbers, system.atom_pos, system.esp_cube,
+                  _abi.RP_ORDER_CUBE, 1, prec, 1.0, 1000.0, 20.0)
+        fit.accumulate()
+        fit.solve(con)
+        e2e.append(time.perf_counter() - t0)
     out = {"q": q, "lat": np.array(lat), "total": total, "kernels": stats, "breakdown_fits": breakdown_fits,
-           "dominant": dominant, "M": M, "rows": rows, "slices": slices,
-           "N": int(fit.n_atoms), "K": int(info.n_k), "timings": fit.timings(),
+           "dominant": dominant, "M": M, "rows": rows, "plan": plan, "lines_per_chunk": lines,
+           "slices": plan["slices"], "N": N, "K": int(info.n_k), "volume": abs(float(info.volume)),
+           "timings": fit.timings(), "report": report, "e2e": np.array(e2e),
            "refine": fit.refine_stats() if prec == _abi.RP_PREC_OZAKI_REFINED else None}
     fit.close()
     return out
@@ -160,20 +184,25 @@ def cpu_baseline(system, con, target_seconds=15.0):
     t0 = time.perf_counter()
     O.phi_matrix(pos[probe], system.atom_pos, ew, nthreads=threads)
     per_row = (time.perf_counter() - t0) / len(probe)
-    n_sample = int(min(M, max(256, target_seconds / max(per_row, 1e-9))))
-    sample = np.sort(rng.choice(M, size=n_sample, replace=False))
-    t0 = time.perf_counter()
-    phi = O.phi_matrix(pos[sample], system.atom_pos, ew, nthreads=threads)
-    t_phi = time.perf_counter() - t0
-    t0 = time.perf_counter()
-    _ = phi.T @ phi
-    _ = phi.T @ v[sample]
-    t_gram = time.perf_counter() - t0
-    t_fit = t_filter + (t_phi + t_gram) * (M / n_sample)
+    # three runs on independent seeded samples, each ~1/3 of the time budget; the median per-row cost
+    runs = 3
+    n_sample = int(min(M, max(256, target_seconds / runs / max(per_row, 1e-9))))
+    per_point = []
+    for _ in range(runs):
+        sample = np.sort(rng.choice(M, size=n_sample, replace=False))
+        t0 = time.perf_counter()
+        phi = O.phi_matrix(pos[sample], system.atom_pos, ew, nthreads=threads)
+        _ = phi.T @ phi
+        _ = phi.T @ v[sample]
+        per_point.append((time.perf_counter() - t0) / n_sample)
+    t_fit = t_filter + float(np.median(per_point)) * M
     return {"value": round(1.0 / t_fit, 6), "unit": "fits/s", "cores": threads, "kind": "port",
             "sample": (f"oracle Phi (direct cos over K={len(ew.kcoefs)} k, erfc over {len(ew.shifts)} shifts) "
-                       f"+ NumPy Gram on {n_sample} of M={M} kept points ({t_phi + t_gram:.1f} s), "
-                       f"filter on the full cube ({t_filter:.1f} s), scaled to M"),
+                       f"+ NumPy Gram, {runs} runs on seeded samples of {n_sample} of M={M} kept points "
+                       f"(median {1e6 * float(np.median(per_point)):.1f} us per point, spread "
+                       f"{1e6 * min(per_point):.1f}-{1e6 * max(per_point):.1f}), filter on the full cube "
+                       f"({t_filter:.1f} s), scaled to M; strict IEEE libm cos/erfc, where the reference's "
+                       f"numba path uses fastmath"),
             "seconds_per_fit": round(t_fit, 2)}
 
 
@@ -325,6 +354,9 @@ def main():
     _log("plain fp64 path ...")
     f64 = run_gpu(ctx, system, con, "fp64", bsteps, min(args.warmup, 2))
     _log(f"  {bsteps / f64['total']:.2f} fits/s, p50 {1e3 * np.median(f64['lat']):.2f} ms")
+    _log("fp64 path with the factorized basis ...")
+    f64f = run_gpu(ctx, system, con, "fp64-factorized", bsteps, min(args.warmup, 2))
+    _log(f"  {bsteps / f64f['total']:.2f} fits/s, p50 {1e3 * np.median(f64f['lat']):.2f} ms")
     ctx.close()
 
     dq_vs_fp64 = float(np.max(np.abs(split["q"] - f64["q"])))
@@ -334,40 +366,63 @@ def main():
         g = json.load(open(golden))
         ref = np.array(g["charges"])
         err_oracle = {"chosen_max_dq_e": float(np.max(np.abs(split["q"] - ref))),
-                      "fp64_max_dq_e": float(np.max(np.abs(f64["q"] - ref)))}
+                      "fp64_max_dq_e": float(np.max(np.abs(f64["q"] - ref))),
+                      "fp64_factorized_max_dq_e": float(np.max(np.abs(f64f["q"] - ref)))}
 
     fits_s = args.steps / split["total"]
     fits_s64 = bsteps / f64["total"]
+    fits_s64f = bsteps / f64f["total"]
     M, N, K = split["M"], split["N"], split["K"]
-    # dominant kernel of the chosen path (the basis, k_sep_phi: the most time per fit) and its roofline
+    # dominant kernel of the chosen path: the basis (k_sep_phi, the most time per fit), bound by its fp64
+    # vector arithmetic.  Its algorithmic fp64 flops per launch, from the factorized sum's units:
+    #   per (grid line, atom)   728: T2 for h = 0..6, 13 complex multiply-adds (8 flops) each
+    #   per (kept point, atom)   86: Re sum_h T2 X^h by Horner (5 complex steps + the last, 44) and the
+    #                                nearest-image test and displacement (42)
+    #   per real-space term      47: erfc(sqrt(a) r)/r = 1/r - sqrt(a) P(u), P of degree 21 by Horner
+    #                                (u, 21 FMAs, the last FMA) plus the reciprocal square root
+    # with the terms inside the cutoff counted as the sphere's share of the cell, (4/3) pi Rc^3 / V.
     ms, launches = split["dominant"]
-    rows = split["rows"]
     launches_per_fit = launches / args.steps
     t_launch = ms / launches * 1e-3
-    # F_k = 4 M N K per fit (the k-space contraction of the reference's formulation, counted once); one
-    # launch covers `rows` points
-    fk_per_launch = 4.0 * M * N * K / launches_per_fit
-    achieved = fk_per_launch / t_launch / 1e12
-    roof = {"bound": "mfma", "kernel": "k_sep_phi", "achieved": round(achieved, 3), "peak": PEAK_FP64_TFLOPS,
-            "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP64_TFLOPS, 4),
-            "traffic": traffic_from_profile("k_sep_phi", args.workload),
+    rows, lines = split["rows"], split["lines_per_chunk"]
+    frac_in = min(1.0, 4.0 / 3.0 * np.pi * 20.0 ** 3 / split["volume"])
+    alg_fit = 728.0 * sum(lines) * N + 86.0 * M * N + 47.0 * frac_in * M * N
+    alg_launch = alg_fit / launches_per_fit
+    achieved = alg_launch / t_launch / 1e12
+    executed = flops_from_profile("k_sep_phi", args.workload)
+    traffic = traffic_from_profile("k_sep_phi", args.workload)
+    roof = {"bound": "mfma", "compute_unit": "fp64 vector ALU (v_fma_f64); the MI355X fp64 vector and matrix "
+                                              "peaks are both 78.6 TFLOP/s",
+            "kernel": "k_sep_phi", "achieved": round(achieved, 3), "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+            "frac": round(achieved / PEAK_FP64_TFLOPS, 4), "traffic": traffic,
             "launch_ms": round(ms / launches, 4), "rows_per_launch": rows,
-            "note": "achieved = algorithmic k-space flops F_k = 4 M N K (the reference's cos/sin GEMM "
-                    "formulation) of the launch's rows / mean launch time (HIP events on the fit's stream); "
-                    "peak = fp64, the arithmetic of the kernel; frac > 1 because the factorized sum does about "
-                    "1/25 of those flops (plus the real-space sum) on the fp64 vector units"}
+            "algorithmic_flops_per_launch": round(alg_launch), "lines_per_fit": int(sum(lines)),
+            "executed_flops_per_launch": executed,
+            "executed_tflops": round(executed / t_launch / 1e12, 3) if executed else None,
+            "executed_frac": round(executed / t_launch / 1e12 / PEAK_FP64_TFLOPS, 4) if executed else None,
+            "hbm_gbs": round(traffic / t_launch / 1e9, 1) if traffic else None,
+            "hbm_frac": round(traffic / t_launch / 8e12, 4) if traffic else None,
+            "fk_equivalent_tflops": round(4.0 * M * N * K / launches_per_fit / t_launch / 1e12, 1),
+            "note": "achieved = the factorized sum's algorithmic fp64 flops of one launch (728 per line and atom, "
+                    "86 per kept point and atom, 47 per real-space term) / mean launch time (HIP events on the "
+                    "fit's stream); executed = rocprof SQ_INSTS_VALU_{FMA,MUL,ADD,TRANS}_F64 x 64 lanes (idle "
+                    "lanes included); traffic = calibrated FETCH_SIZE + WRITE_SIZE; fk_equivalent counts "
+                    "4 M N K, the reference's cos/sin GEMM formulation, which this kernel does not execute"}
     # the int8 Gram (MFMA), from the per-kernel breakdown fits after the timed region
     gms, glaunch = split["kernels"]["gram"]
     g_launch = gms / glaunch * 1e-3
     alg_per_launch = (M * N * (N + 1) + 3.0 * M * N) / (glaunch / split["breakdown_fits"])
-    np_ = ((N + 2 + 127) // 128) * 128
-    tiles = sum(1 for bi in range(np_ // 64) for bj in range(np_ // 128) if bi * 64 <= bj * 128 + 127)
+    np_ = split["plan"]["np"]
+    tiles = split["plan"]["n_tiles"]
     P = split["slices"]
     executed = 2.0 * rows * tiles * 64 * 128 * P * P / g_launch / 1e12    # int8 ops issued, P^2 slice products
+    g_traffic = traffic_from_profile("k_oz_gram", args.workload)
+    g_alg_bytes = P * np_ * rows + 2 * 8.0 * split["plan"]["splits"] * tiles * 64 * 128
     roof_gram = {"bound": "mfma", "kernel": "k_oz_gram", "achieved": round(alg_per_launch / g_launch / 1e12, 3),
                  "peak": PEAK_INT8_TOPS, "unit": "TFLOP/s",
                  "frac": round(alg_per_launch / g_launch / 1e12 / PEAK_INT8_TOPS, 5),
-                 "traffic": traffic_from_profile("k_oz_gram", args.workload),
+                 "traffic": g_traffic, "algorithmic_bytes": round(g_alg_bytes),
+                 "traffic_over_algorithmic": round(g_traffic / g_alg_bytes, 3) if g_traffic else None,
                  "executed_int8_tops": round(executed, 1), "executed_frac": round(executed / PEAK_INT8_TOPS, 4),
                  "launch_ms": round(gms / glaunch, 4), "slices": P,
                  "note": "achieved = fp64-equivalent Gram flops F_G = M N (N + 1) + 3 M N of the launch's rows / "
@@ -399,10 +454,21 @@ def main():
         "fp64_fits_per_s": round(fits_s64, 4),
         "fp64_p50_ms": round(1e3 * float(np.median(f64["lat"])), 3),
         "speedup_vs_fp64_gpu": round(fits_s / fits_s64, 3),
+        "fp64_factorized_fits_per_s": round(fits_s64f, 4),
+        "fp64_factorized_p50_ms": round(1e3 * float(np.median(f64f["lat"])), 3),
+        "speedup_vs_fp64_factorized": round(fits_s / fits_s64f, 3),
         "max_dq_vs_fp64_e": dq_vs_fp64, "within_1e-5": dq_vs_fp64 <= 1e-5,
+        "max_dq_fp64_factorized_vs_fp64_e": float(np.max(np.abs(f64f["q"] - f64["q"]))),
         "error_vs_oracle": err_oracle,
+        "end_to_end_p50_ms": {"chosen": round(1e3 * float(np.median(split["e2e"])), 2),
+                              "fp64": round(1e3 * float(np.median(f64["e2e"])), 2),
+                              "fp64_factorized": round(1e3 * float(np.median(f64f["e2e"])), 2),
+                              "note": "from host arrays: cube upload, host Ewald setup, filter, accumulate and "
+                                      "solve (rp_fit_reset + rp_fit_accumulate + rp_fit_solve); cube text "
+                                      "parsing excluded"},
         "refine_steps_last_correction_e": split["refine"],
-        "kernels_chosen": kms, "kernels_fp64": kms64,
+        "solve_report": split["report"],
+        "kernels_chosen": kms, "kernels_fp64": kms64, "kernels_fp64_factorized": per_fit(f64f),
         "roofline": roof,
         "roofline_gram": roof_gram,
         "cpu_baseline": cpu,

@@ -95,8 +95,14 @@ typedef struct rp_options {
     double r_cutoff;             /* --cutoff, default 20.0 */
     int32_t refine_steps;        /* RP_PREC_OZAKI_REFINED: at most this many refinement steps
                                     (<= 0: RP_REFINE_STEPS_DEFAULT); no counterpart in the reference */
-    int32_t reserved;            /* 0 */
+    int32_t flags;               /* RP_OPT_* bits, 0 by default; no counterpart in the reference */
 } rp_options;
+
+/* RP_PREC_OZAKI_REFINED keeps the fp64 basis of every kept point in HBM (M x (N + 2) doubles) for its
+ * refinement when that fits the context's budget and the device's free memory, and otherwise streams:
+ * each refinement step recomputes the basis chunk by chunk (one basis pass more per step, one chunk of
+ * memory).  Both give the same charges, bitwise.  This flag forces the streamed form. */
+#define RP_OPT_STREAM_REFINE 1
 
 /* Sizes of a prepared fit. */
 typedef struct rp_fit_info {
@@ -206,8 +212,10 @@ int rp_fit_solve(rp_fit *fit, const rp_constraints *con, double *q_out, double *
  */
 int rp_fit_refit(rp_fit *fit, const rp_constraints *con, double *q_out);
 
-/* compute_stats (repeat.py:2326-2441): v_coul[m] = (Phi q)[m] for every kept point, streamed (from the
- * basis that RP_PREC_OZAKI_REFINED keeps after rp_fit_accumulate, otherwise recomputed in fp64). */
+/* compute_stats (repeat.py:2326-2441): v_coul[m] = (Phi q)[m] for every kept point, streamed: from the
+ * basis that RP_PREC_OZAKI_REFINED keeps after rp_fit_accumulate; after an accumulate of the other
+ * factorized forms (or a refined one that streamed), the factorized fp64 basis recomputed chunk by
+ * chunk; otherwise the fp64 basis of the plain path. */
 int rp_fit_phi_matvec(rp_fit *fit, const double *q, double *v_coul);
 
 /*

@@ -63,7 +63,7 @@ class RpSystem(ctypes.Structure):
 class RpOptions(ctypes.Structure):
     _fields_ = [("fit_type", ctypes.c_int32), ("precision", ctypes.c_int32), ("vdw_factor", ctypes.c_double),
                 ("vdw_max", ctypes.c_double), ("r_cutoff", ctypes.c_double), ("refine_steps", ctypes.c_int32),
-                ("reserved", ctypes.c_int32)]
+                ("flags", ctypes.c_int32)]
 
 
 class RpFitInfo(ctypes.Structure):
@@ -308,20 +308,25 @@ def make_system(n_grid, axis, atomic_numbers, atom_xyz, esp, esp_order=RP_ORDER_
     return sysd, (Z, xyz, esp)
 
 
-def make_options(fit_type=1, precision=RP_PREC_FP64, vdw_factor=1.0, vdw_max=1000.0, r_cutoff=20.0, refine_steps=0):
+RP_OPT_STREAM_REFINE = 1
+
+
+def make_options(fit_type=1, precision=RP_PREC_FP64, vdw_factor=1.0, vdw_max=1000.0, r_cutoff=20.0, refine_steps=0,
+                 flags=0):
     return RpOptions(int(fit_type), int(precision), float(vdw_factor), float(vdw_max), float(r_cutoff),
-                     int(refine_steps), 0)
+                     int(refine_steps), int(flags))
 
 
 class Fit:
     """Device-resident fit (filter done at construction)."""
 
     def __init__(self, ctx: Context, n_grid, axis, atomic_numbers, atom_xyz, esp, esp_order=RP_ORDER_CUBE,
-                 fit_type=1, precision=RP_PREC_FP64, vdw_factor=1.0, vdw_max=1000.0, r_cutoff=20.0, refine_steps=0):
+                 fit_type=1, precision=RP_PREC_FP64, vdw_factor=1.0, vdw_max=1000.0, r_cutoff=20.0, refine_steps=0,
+                 flags=0):
         L = load()
         self.ctx = ctx
         sysd, self._arrays = make_system(n_grid, axis, atomic_numbers, atom_xyz, esp, esp_order)
-        opt = make_options(fit_type, precision, vdw_factor, vdw_max, r_cutoff, refine_steps)
+        opt = make_options(fit_type, precision, vdw_factor, vdw_max, r_cutoff, refine_steps, flags)
         h = ctypes.c_void_p()
         check(L.rp_fit_create(ctx.handle, ctypes.byref(sysd), ctypes.byref(opt), ctypes.byref(h)), ctx.handle)
         self.handle = h

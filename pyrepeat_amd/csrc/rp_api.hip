@@ -371,6 +371,7 @@ int load_system(rp_fit *fit, const rp_system *sys, const rp_options *opt) {
     fit->accumulated = false;
     fit->lu_fallback = false;
     fit->phi_kept = false;
+    fit->refine_stream = false;
     fit->refine_steps_done = 0;
     fit->refine_last_dq = 0.0;
     fit->refine_bound = -1.0;

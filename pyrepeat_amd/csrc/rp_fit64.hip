@@ -401,7 +401,7 @@ int accumulate_f64_factorized(rp_fit *fit, FitDevice &dv) {
     if (cp.NP != NP || cp.n_tiles != n_tiles || rows % (n_splits * BK) != 0)
         return ctx->fail(RP_EINVAL, "fp64 factorized path: chunk plan does not match the Gram tiling");
     DBuf &d_phi = fit->ws[2], &d_part = fit->ws[3], &d_colmax = fit->ws[5];
-    SepPlan plan;
+    SepPlan &plan = dv.sep_plan;
     {
         KTimer kt(ctx, KT_PHASE);
         RP_CHECK(sep_prepare(fit, dv, rows, plan));
@@ -439,6 +439,7 @@ int accumulate_f64_factorized(rp_fit *fit, FitDevice &dv) {
     }
     RP_HIP(ctx, hipGetLastError());
     fit->phi_kept = false;
+    fit->phi_rows = rows;
     return RP_OK;
 }
 
@@ -460,7 +461,34 @@ int run_phi_matvec(rp_fit *fit, const double *q, double *v_coul) {
         RP_HIP(ctx, hipStreamSynchronize(st));
         return RP_OK;
     }
-    // the matvec always evaluates the basis in fp64 (KP padded for the fp64 GEMM)
+    const int prec = fit->opt.precision;
+    if (fit->accumulated && fit->dev && fit->phi_rows > 0 &&
+        (prec == RP_PREC_OZAKI || prec == RP_PREC_OZAKI_REFINED || prec == RP_PREC_FP64_FACT)) {
+        // after an accumulate of the factorized forms: their fp64 basis again, chunk by chunk into the
+        // chunk buffer, each chunk's rows times q
+        FitDevice &dv = *fit->dev;
+        const int N = fit->n_atoms, NP = dv.NP;
+        const int64_t M = fit->n_keep, rows = fit->phi_rows;
+        DBuf &d_phi = fit->ws[2], &d_colmax = fit->ws[5], &d_q = fit->ws[22], &d_out = fit->ws[23];
+        RP_CHECK(d_phi.grow(ctx, sizeof(double) * (size_t)NP * rows));
+        RP_CHECK(d_colmax.grow(ctx, sizeof(unsigned long long) * NP));
+        RP_CHECK(d_q.grow(ctx, sizeof(double) * N));
+        RP_CHECK(d_out.grow(ctx, sizeof(double) * M));
+        RP_HIP(ctx, hipMemcpyAsync(d_q.p, q, sizeof(double) * N, hipMemcpyHostToDevice, st));
+        int chunk = 0;
+        for (int64_t r0 = 0; r0 < M; r0 += rows, ++chunk) {
+            const int64_t valid = std::min<int64_t>(rows, M - r0);
+            sep_phi_chunk(fit, dv, dv.sep_plan, chunk, r0, valid, rows, d_phi.as<double>(),
+                          d_colmax.as<unsigned long long>(), st);
+            k_kept_matvec<<<ceil_div(valid, 512), 256, 0, st>>>(d_phi.as<double>(), rows, (int64_t)NP * rows, N, valid,
+                                                                d_q.as<double>(), d_out.as<double>() + r0);
+        }
+        RP_HIP(ctx, hipGetLastError());
+        RP_HIP(ctx, hipMemcpyAsync(v_coul, d_out.p, sizeof(double) * M, hipMemcpyDeviceToHost, st));
+        RP_HIP(ctx, hipStreamSynchronize(st));
+        return RP_OK;
+    }
+    // otherwise the basis in fp64 by the plain path's GEMM (KP padded for the fp64 GEMM)
     FitDevice *dvp = nullptr;
     RP_CHECK(device_setup(fit, RP_PREC_FP64, dvp));
     FitDevice &dv = *dvp;

@@ -99,6 +99,11 @@ struct EpiArgs {
     }
 };
 
+// Factorized reciprocal sum of the split form (rp_kspace.hip).
+struct SepPlan {
+    std::vector<int> first_line, end_line;  // per chunk: lines [first, end) holding its rows
+};
+
 // Device copies of the Ewald setup and the chunking plan of one fit.
 struct FitDevice {
     int K = 0, KP = 0, NP = 0, S = 0;
@@ -115,6 +120,7 @@ struct FitDevice {
     // table of grid lines holding kept points
     DBuf sep_kbox, sep_t1, line_row, line_key, line_cnt, line_off, chunk_lines;
     bool sep_kbox_ok = false;   // sep_kbox holds the index box of the current k-set
+    SepPlan sep_plan;           // chunk line ranges of the last split accumulate (the streamed refinement's)
     hipEvent_t ev_plan = nullptr;   // marks the chunk line ranges on the stream
     FitDevice() = default;
     FitDevice(const FitDevice &) = delete;
@@ -131,16 +137,15 @@ int accumulate_f64(rp_fit *fit, FitDevice &dv);
 int accumulate_split(rp_fit *fit, FitDevice &dv);
 int accumulate_f64_factorized(rp_fit *fit, FitDevice &dv);
 
-// Factorized reciprocal sum of the split form (rp_kspace.hip).
-struct SepPlan {
-    std::vector<int> first_line, end_line;  // per chunk: lines [first, end) holding its rows
-};
 // Per accumulate: the k-index box, T1 for every plane, and the line table of the kept points.
 int sep_prepare(rp_fit *fit, FitDevice &dv, int64_t rows, SepPlan &plan);
 // Phi rows [r0, r0 + valid) of one chunk (k-space + real space + augmented columns), column-major
 // [NP][rows] fp64, with the column maxima of |Phi| (bit patterns) into colmax.
+// gate (nullable): the launch does nothing unless gate[0] != 0 and gate_flags[0] == gate_flags[1] == 0 (a
+// refinement step the device has decided to skip).
 void sep_phi_chunk(rp_fit *fit, FitDevice &dv, const SepPlan &plan, int chunk, int64_t r0, int64_t valid,
-                   int64_t rows, double *phi, unsigned long long *colmax, hipStream_t st);
+                   int64_t rows, double *phi, unsigned long long *colmax, hipStream_t st, const int *gate = nullptr,
+                   const int *gate_flags = nullptr);
 // Column-major Phi chunk -> int8 slices [P][NP][rows] (P = 3 or 4) with per-column power-of-two scales.
 void sep_slice_chunk(const double *phi, int NP, int64_t rows, int64_t valid, const unsigned long long *colmax,
                      int8_t *slices, double *scale_out, int P, hipStream_t st);

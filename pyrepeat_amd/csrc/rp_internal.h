@@ -7,6 +7,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
+#include <functional>
 #include <string>
 #include <vector>
 
@@ -55,6 +56,9 @@ struct Ctx {
         hipEvent_t a, b;
     };
     std::vector<Pending> pending;
+    // Asked for more budget when a fit could use it (the refined form's kept basis); returns whether the
+    // budget grew by that much.  Unset: the budget is fixed.
+    std::function<bool(uint64_t)> grow_budget;
     int fail(int code, const std::string &msg) {
         err = msg;
         return code;
@@ -299,6 +303,7 @@ struct rp_fit {
     // refinement (RP_PREC_OZAKI_REFINED): the accumulate keeps every chunk of the fp64 basis in ws[7]
     // ([chunk][NP][phi_rows] doubles); phi_kept says it matches the fit's current system
     bool phi_kept = false;
+    bool refine_stream = false;     // refined form without the kept basis: each step recomputes it
     int64_t phi_rows = 0;
     int32_t refine_steps_done = 0;
     double refine_last_dq = 0.0;

@@ -218,6 +218,7 @@ struct SepArgs {
     int64_t rows;               // leading dimension of phi
     unsigned long long *colmax;
     int items, items_per_xcd;   // line groups of the chunk; line groups per XCD
+    const int *gate, *gate_flags;   // nullable: run only while gate[0] != 0 and both flags are 0
 };
 
 // One wave takes 8 atoms of the work-group's lines; lane = (atom a_l = lane / 8, s = lane % 8).
@@ -239,6 +240,7 @@ __global__ __launch_bounds__(SEP_THREADS) void k_sep_phi(SepArgs p) {
     const int item = xcd * p.items_per_xcd + jj % p.items_per_xcd;
     const int ablk = jj / p.items_per_xcd;
     if (item >= p.items) return;
+    if (p.gate && (p.gate[0] == 0 || p.gate_flags[0] != 0 || p.gate_flags[1] != 0)) return;
     const int a = ablk * SEP_ATOMS + wave * 8 + a_l;
     const EpiArgs &e = p.e;
     const int nx = e.geom.nx, ny = e.geom.ny;
@@ -502,8 +504,10 @@ int sep_prepare(rp_fit *fit, FitDevice &dv, int64_t rows, SepPlan &plan) {
 }
 
 void sep_phi_chunk(rp_fit *fit, FitDevice &dv, const SepPlan &plan, int chunk, int64_t r0, int64_t valid, int64_t rows,
-                   double *phi, unsigned long long *colmax, hipStream_t st) {
+                   double *phi, unsigned long long *colmax, hipStream_t st, const int *gate, const int *gate_flags) {
     SepArgs a{};
+    a.gate = gate;
+    a.gate_flags = gate_flags;
     a.rows = rows;
     a.line_row = dv.line_row.as<int>();
     a.line_key = dv.line_key.as<int>();

@@ -244,8 +244,27 @@ __global__ void k_zero_u64(unsigned long long *p, int n) {
 
 namespace {
 
+// Whether the refined form can keep its basis (`need` bytes in ws[7]): within the context's budget (which
+// its grow hook may raise) and the device's free memory with 1 GiB to spare; an existing ws[7] is freed
+// before a larger one is taken.
+bool keep_basis_fits(rp_fit *fit, uint64_t need) {
+    Ctx *ctx = fit->ctx;
+    if (fit->opt.flags & RP_OPT_STREAM_REFINE) return false;
+    DBuf &kept = fit->ws[7];
+    if (kept.p && kept.bytes >= need && kept.ctx == ctx) return true;
+    const uint64_t held = kept.p ? kept.bytes : 0;
+    size_t free_b = 0, total_b = 0;
+    if (hipMemGetInfo(&free_b, &total_b) == hipSuccess && need + (1ull << 30) > (uint64_t)free_b + held) return false;
+    if (ctx->budget) {
+        const uint64_t after = ctx->in_use - held + need;
+        if (after > ctx->budget && !(ctx->grow_budget && ctx->grow_budget(after - ctx->budget))) return false;
+    }
+    return true;
+}
+
 // P slices per element; keep_phi: every chunk of the fp64 basis stays in ws[7] ([chunk][NP][rows]) for
-// the refinement of the solve, otherwise one chunk buffer (ws[2]) is reused.
+// the refinement of the solve, otherwise one chunk buffer (ws[2]) is reused.  The refined form asks for
+// the kept basis and streams its refinement instead when the basis does not fit.
 template <int P>
 int accumulate_split_p(rp_fit *fit, FitDevice &dv, bool keep_phi) {
     Ctx *ctx = fit->ctx;
@@ -256,10 +275,16 @@ int accumulate_split_p(rp_fit *fit, FitDevice &dv, bool keep_phi) {
     const int64_t rows = cp.rows;
     if (cp.NP != NP || cp.slices != P || TM != 64 || TN != 128 || rows % 512 != 0)
         return ctx->fail(RP_EINVAL, "split path: chunk plan does not match the kernel tiling");
+    const int64_t n_chunks = ceil_div(M, rows);
+    const bool refined = keep_phi;
+    if (refined) {
+        keep_phi = keep_basis_fits(fit, sizeof(double) * (uint64_t)NP * rows * n_chunks);
+        if (!keep_phi) fit->ws[7].release();   // the memory is short: give back a basis kept before
+    }
+    fit->refine_stream = refined && !keep_phi;
     DBuf &d_phi = fit->ws[keep_phi ? 7 : 2], &d_part = fit->ws[3], &d_slices = fit->ws[4], &d_colmax = fit->ws[5],
          &d_scale = fit->ws[6], &d_tiles = fit->ws[9];
-    const int64_t n_chunks = ceil_div(M, rows);
-    SepPlan plan;
+    SepPlan &plan = dv.sep_plan;
     {
         KTimer kt(ctx, KT_PHASE);
         RP_CHECK(sep_prepare(fit, dv, rows, plan));

@@ -111,9 +111,10 @@ uint64_t fit_bytes_bound(const Setup &s, int n_atoms, int esp_dtype, int precisi
     };
     const ChunkPlan c64 = plan_chunks(M, n_atoms, (int)K, RP_PREC_FP64);
     const uint64_t NP = c64.NP;
-    // fp64 basis chunks: the fp64 accumulate, and the matvec of the fp64 and split forms (the refined
-    // form's matvec streams the basis it keeps)
-    if (precision != RP_PREC_OZAKI_REFINED) {
+    // fp64 basis chunks of the plain path's GEMM: its accumulate and its matvec (the factorized forms'
+    // matvec after their accumulate recomputes their own basis into their chunk buffer, or streams the
+    // basis the refined form keeps)
+    if (precision == RP_PREC_FP64) {
         at(0, 8 * (uint64_t)c64.rows * c64.KP);
         at(1, 8 * (uint64_t)c64.KP * NP);
         at(2, 8 * (uint64_t)c64.rows * NP);
@@ -124,8 +125,8 @@ uint64_t fit_bytes_bound(const Setup &s, int n_atoms, int esp_dtype, int precisi
     uint64_t sep = 0;
     if (precision != RP_PREC_FP64) {
         const ChunkPlan c = plan_chunks(M, n_atoms, (int)K, precision);
-        // one fp64 chunk buffer (the refined form keeps every chunk in ws[7] instead)
-        if (precision != RP_PREC_OZAKI_REFINED) at(2, 8 * (uint64_t)c.NP * c.rows);
+        // one fp64 chunk buffer (the refined form's streamed refinement recomputes into it too)
+        at(2, 8 * (uint64_t)c.NP * c.rows);
         at(5, 8 * (uint64_t)c.NP);
         if (precision == RP_PREC_FP64_FACT) {
             at(3, 8 * split_cap(c.rows, c.n_tiles) * c.n_tiles * 128 * 128);   // fp64 Gram partials
@@ -141,10 +142,11 @@ uint64_t fit_bytes_bound(const Setup &s, int n_atoms, int esp_dtype, int precisi
         sep += 4 * 8 * 13 * 13 + (K > 0 ? 16 * 8 * 13 * (uint64_t)s.nz * N : 0);
         sep += 4 * nb + 8 * (nb + 1) + 4 * (M + 1) + 4 * M + 8 * n_chunks + 16 * 7;
         if (precision == RP_PREC_OZAKI_REFINED) {
-            // every basis chunk kept for the refinement (fewer than M + rows rows in all), the
-            // residual e over the kept points and its block sums
-            at(7, 8 * (uint64_t)c.NP * (M + (uint64_t)c.rows));
+            // the streamed refinement: the residual e over the kept points and its block sums, the column
+            // pass's partial sums.  Keeping the basis instead (ws[7]) is not in the bound: a fit takes it
+            // only when its context's budget grows by that much (the serving loop asks its own budget).
             at(8, 8 * (M + 2 + (uint64_t)ceil_div(M, 512)));
+            at(15, 8 * 4 * 256 * N);
         }
     }
     // bordered solve
@@ -217,6 +219,20 @@ struct rp_server {
 
 void rp_server::run_job(Slot *slot, Job &job, bool trim_first) {
     rp_ctx *ctx = slot->ctx;
+    // the slot's context may hold what admission reserved for it; a refined fit asks for more to keep
+    // its basis, granted while the server's budget holds it
+    {
+        std::lock_guard<std::mutex> lk(mu);
+        ctx->budget = slot->reserved;
+    }
+    ctx->grow_budget = [this, slot, ctx](uint64_t extra) {
+        std::lock_guard<std::mutex> lk(mu);
+        if (reserved_total() + extra > budget) return false;
+        slot->reserved += extra;
+        ctx->budget += extra;
+        peak_reserved = std::max(peak_reserved, reserved_total());
+        return true;
+    };
     if (trim_first && slot->fit) {
         rp_fit_destroy(slot->fit);
         slot->fit = nullptr;

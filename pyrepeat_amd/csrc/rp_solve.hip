@@ -53,6 +53,8 @@ struct RefineState {
 // Contraction assumed for the bound after a single step, where no ratio of corrections exists yet:
 // twice the largest per-step contraction measured (5.3e-3, a 5000-atom cell with cond(A) = 4.9e10).
 constexpr double RHO_PRIOR = 1e-2;
+// Refinement steps queued with the solve; more only after a synchronisation shows they are needed.
+constexpr int REFINE_EAGER = 2;
 // Relative perturbation of the sensitivity estimate: 8 units in the last place of an fp64 entry.
 constexpr double SENS_EPS = 8.0 * 2.220446049250313e-16;
 
@@ -1073,6 +1075,54 @@ __global__ __launch_bounds__(256) void k_ref_cols(const double *__restrict__ phi
     if (t == 0) r[j] = G - S * (Et / (double)M);
 }
 
+// k_ref_cols over one chunk of the basis at a time (the streamed refinement, which recomputes the chunks):
+// each thread's four partial sums carry over between chunks in acc ([N][256][4]), so the sums run in
+// k_ref_cols's order and the residual is bitwise the same.  The last chunk finishes r_j.
+__global__ __launch_bounds__(256) void k_ref_cols_chunk(const double *__restrict__ col0, int64_t rows, int64_t valid,
+                                                        const double *__restrict__ ec, double *__restrict__ acc,
+                                                        int first, int last, const double *__restrict__ esum,
+                                                        int n_esum, int64_t M, double *__restrict__ r,
+                                                        const int *__restrict__ cinfo,
+                                                        const RefineState *__restrict__ rs) {
+    __shared__ double red[4];
+    if (!refine_active(cinfo, rs)) return;
+    const int j = blockIdx.x, t = threadIdx.x;
+    double *a = acc + ((int64_t)j * 256 + t) * 4;
+    double g0 = 0.0, g1 = 0.0, s0 = 0.0, s1 = 0.0;
+    if (!first) {
+        g0 = a[0];
+        g1 = a[1];
+        s0 = a[2];
+        s1 = a[3];
+    }
+    const double *col = col0 + (int64_t)j * rows;
+    int64_t q = 2 * t;
+    for (; q + 1 < valid; q += 512) {
+        const double2 p = *reinterpret_cast<const double2 *>(col + q);
+        g0 = fma(p.x, ec[q], g0);
+        g1 = fma(p.y, ec[q + 1], g1);
+        s0 += p.x;
+        s1 += p.y;
+    }
+    if (q < valid) {
+        g0 = fma(col[q], ec[q], g0);
+        s0 += col[q];
+    }
+    if (!last) {
+        a[0] = g0;
+        a[1] = g1;
+        a[2] = s0;
+        a[3] = s1;
+        return;
+    }
+    double E = 0.0;
+    for (int i = t; i < n_esum; i += 256) E += esum[i];
+    const double G = block_sum_256(g0 + g1, red);
+    const double S = block_sum_256(s0 + s1, red);
+    const double Et = block_sum_256(E, red);
+    if (t == 0) r[j] = G - S * (Et / (double)M);
+}
+
 // r1 = P^T r + rhs_add - diag_add x - w lambda (into r1), r2 = q_tot - w . x; one work-group.
 __global__ __launch_bounds__(256) void k_ref_rhs(const double *__restrict__ r, const int *__restrict__ red_ptr,
                                                  const int *__restrict__ red_src, const double *__restrict__ red_mult,
@@ -1504,6 +1554,7 @@ __global__ __launch_bounds__(256) void k_publish(const double *__restrict__ x, c
         out[1 + N + n] = (double)rs->steps;
         out[2 + N + n] = rs->last_dq;
         out[3 + N + n] = rs->bound;
+        out[5 + N + n] = (double)rs->go;
     }
     if (i < N) out[1 + i] = src[i] >= 0 ? x[src[i]] : 0.0;
     if (i < n) out[1 + N + i] = x[i];
@@ -1596,7 +1647,7 @@ int run_solve(rp_fit *fit, const rp_constraints *con, double *q_out, double *sol
     const size_t n_int = (size_t)(nr + 1) + nsrc + N + (size_t)(N + 1) + nsrc;
     const size_t off_d = round_up((int64_t)(4 * n_int), 8);
     const size_t stage = round_up((int64_t)(off_d + 8 * (2 * (size_t)nsrc + 3 * (size_t)nr)), 16);
-    const size_t out_bytes = sizeof(double) * (5 + (size_t)N + n);
+    const size_t out_bytes = sizeof(double) * (6 + (size_t)N + n);
     if (fit->h_stage_bytes < stage + out_bytes) {
         if (fit->h_stage) (void)hipHostFree(fit->h_stage);
         fit->h_stage = nullptr;
@@ -1658,7 +1709,7 @@ int run_solve(rp_fit *fit, const rp_constraints *con, double *q_out, double *sol
     RP_CHECK(d_dinv.grow(ctx, sizeof(double) * CB * CB * (size_t)ceil_div(nr, CB)));
     // refinement vectors: u (N), r (N), r1 (nr), z (nr), dx (nr + 1), r2, then the state (16 bytes);
     // e over the kept points and its block sums
-    const bool refine = fit->opt.precision == RP_PREC_OZAKI_REFINED && fit->phi_kept;
+    const bool refine = fit->opt.precision == RP_PREC_OZAKI_REFINED && (fit->phi_kept || fit->refine_stream);
     DBuf &d_rv = fit->ws[10], &d_e = fit->ws[8];
     const size_t rv_doubles = 2 * (size_t)N + 3 * (size_t)nr + 2;
     RP_CHECK(d_rv.grow(ctx, sizeof(double) * round_up((int64_t)rv_doubles, 2) + 2 * sizeof(RefineState)));
@@ -1666,6 +1717,36 @@ int run_solve(rp_fit *fit, const rp_constraints *con, double *q_out, double *sol
     RefineState *d_sens = d_rs + 1;   // gates the sensitivity solve
     const int n_esum = (int)ceil_div(fit->n_keep, 512);
     if (refine) RP_CHECK(d_e.grow(ctx, sizeof(double) * ((size_t)fit->n_keep + 2 + (size_t)n_esum)));
+    DBuf &d_cacc = fit->ws[15];   // streamed refinement: the column pass's per-thread partial sums
+    if (refine && fit->refine_stream) RP_CHECK(d_cacc.grow(ctx, sizeof(double) * 4 * 256 * (size_t)N));
+    // One residual pass of the refinement: u = P x, e = V - Phi u over the kept points and r = Phi_c^T e,
+    // from the kept basis, or (streamed) from each chunk recomputed by the basis kernel into ws[2].
+    auto residual_pass = [&]() {
+        const int64_t M = fit->n_keep, rows = fit->phi_rows, plane = (int64_t)NPg * rows;
+        double *e = d_e.as<double>(), *esum = e + M + 2;
+        double *u = d_rv.as<double>(), *r = u + N;
+        const int *ci = d_cinfo.as<int>();
+        k_ref_expand<<<ceil_div(N, 256), 256, 0, st>>>(d_x.as<double>(), dc_inv_ptr, dc_inv_set, dc_inv_mult, N, u,
+                                                       ci, d_rs);
+        if (!fit->refine_stream) {
+            const double *phi = fit->ws[7].as<double>();
+            k_ref_rows<<<n_esum, 256, 0, st>>>(phi, rows, plane, N, M, fit->d_v.as<double>(), u, e, esum, ci, d_rs);
+            k_ref_cols<<<N, 256, 0, st>>>(phi, rows, plane, M, e, esum, n_esum, r, ci, d_rs);
+            return;
+        }
+        FitDevice &dv = *fit->dev;
+        double *phi = fit->ws[2].as<double>();
+        const int n_chunks = (int)ceil_div(M, rows);
+        for (int c = 0; c < n_chunks; ++c) {
+            const int64_t r0 = (int64_t)c * rows, valid = std::min<int64_t>(rows, M - r0);
+            sep_phi_chunk(fit, dv, dv.sep_plan, c, r0, valid, rows, phi, fit->ws[5].as<unsigned long long>(), st,
+                          &d_rs->go, ci);
+            k_ref_rows<<<ceil_div(valid, 512), 256, 0, st>>>(phi, rows, plane, N, valid, fit->d_v.as<double>() + r0, u,
+                                                             e + r0, esum + r0 / 512, ci, d_rs);
+            k_ref_cols_chunk<<<N, 256, 0, st>>>(phi, rows, valid, e + r0, d_cacc.as<double>(), c == 0,
+                                                c == n_chunks - 1, esum, n_esum, M, r, ci, d_rs);
+        }
+    };
 
     hipEvent_t e0, e1;
     RP_HIP(ctx, hipEventCreate(&e0));
@@ -1733,34 +1814,31 @@ int run_solve(rp_fit *fit, const rp_constraints *con, double *q_out, double *sol
     fit->bs_base += (unsigned)nblk;
     delete kts;
     kts = nullptr;
-    if (refine) {
-        // fp64 residual against the kept basis, correction through the same factor; a step whose
-        // predecessor left an error bound of at most RP_REFINE_TARGET_E returns at once on the device
-        KTimer ktr(ctx, KT_REFINE);
-        const int64_t M = fit->n_keep, rows = fit->phi_rows, plane = (int64_t)NPg * rows;
-        double *e = d_e.as<double>(), *esum = e + M + 2;
+    // refinement: fp64 residual against the kept basis, correction through the same factor; a step whose
+    // predecessor left an error bound of at most RP_REFINE_TARGET_E returns at once on the device.  The
+    // first REFINE_EAGER steps are queued with the solve; further ones (up to the cap) only when the
+    // published state after them still asks for a step.
+    const int refine_cap = fit->opt.refine_steps > 0 ? fit->opt.refine_steps : RP_REFINE_STEPS_DEFAULT;
+    auto refine_step = [&]() {
         double *u = d_rv.as<double>(), *r = u + N, *r1 = r + N, *z = r1 + nr, *dx = z + nr, *r2 = dx + nr + 1;
-        const double *phi = fit->ws[7].as<double>();
-        const int steps = fit->opt.refine_steps > 0 ? fit->opt.refine_steps : RP_REFINE_STEPS_DEFAULT;
-        for (int it = 0; it < steps; ++it) {
-            k_ref_expand<<<ceil_div(N, 256), 256, 0, st>>>(d_x.as<double>(), dc_inv_ptr, dc_inv_set, dc_inv_mult, N, u,
-                                                           d_cinfo.as<int>(), d_rs);
-            k_ref_rows<<<n_esum, 256, 0, st>>>(phi, rows, plane, N, M, fit->d_v.as<double>(), u, e, esum,
-                                               d_cinfo.as<int>(), d_rs);
-            k_ref_cols<<<N, 256, 0, st>>>(phi, rows, plane, M, e, esum, n_esum, r, d_cinfo.as<int>(), d_rs);
-            k_ref_rhs<<<1, 256, 0, st>>>(r, dc_ptr, dc_src, dc_mult, dc_w, con->diag_add ? dc_diag : nullptr,
-                                         con->rhs_add ? dc_rhs : nullptr, con->q_tot, nr, d_x.as<double>(), r1, r2,
-                                         d_cinfo.as<int>(), d_rs);
-            k_fwdsub<<<nblk, 256, 0, st>>>(d_K.as<double>(), ld, nr, d_dinv.as<double>(), r1, z, d_sync.as<unsigned>(),
-                                           fit->bs_base, ++fit->bs_epoch, spin_limit, d_cinfo.as<int>() + 1,
-                                           d_cinfo.as<int>(), d_rs);
-            fit->bs_base += (unsigned)nblk;
-            k_backsub<<<nblk, 256, 0, st>>>(d_K.as<double>(), ld, nr, d_dinv.as<double>(), d_wc.as<double>(), z, 0.0,
-                                            dx, d_sync.as<unsigned>(), fit->bs_base, ++fit->bs_epoch, spin_limit,
-                                            d_cinfo.as<int>() + 1, d_cinfo.as<int>(), r2, &d_rs->go);
-            fit->bs_base += (unsigned)nblk;
-            k_ref_update<<<1, 256, 0, st>>>(d_x.as<double>(), dx, nr, RP_REFINE_TARGET_E, d_cinfo.as<int>(), d_rs);
-        }
+        residual_pass();
+        k_ref_rhs<<<1, 256, 0, st>>>(r, dc_ptr, dc_src, dc_mult, dc_w, con->diag_add ? dc_diag : nullptr,
+                                     con->rhs_add ? dc_rhs : nullptr, con->q_tot, nr, d_x.as<double>(), r1, r2,
+                                     d_cinfo.as<int>(), d_rs);
+        k_fwdsub<<<nblk, 256, 0, st>>>(d_K.as<double>(), ld, nr, d_dinv.as<double>(), r1, z, d_sync.as<unsigned>(),
+                                       fit->bs_base, ++fit->bs_epoch, spin_limit, d_cinfo.as<int>() + 1,
+                                       d_cinfo.as<int>(), d_rs);
+        fit->bs_base += (unsigned)nblk;
+        k_backsub<<<nblk, 256, 0, st>>>(d_K.as<double>(), ld, nr, d_dinv.as<double>(), d_wc.as<double>(), z, 0.0,
+                                        dx, d_sync.as<unsigned>(), fit->bs_base, ++fit->bs_epoch, spin_limit,
+                                        d_cinfo.as<int>() + 1, d_cinfo.as<int>(), r2, &d_rs->go);
+        fit->bs_base += (unsigned)nblk;
+        k_ref_update<<<1, 256, 0, st>>>(d_x.as<double>(), dx, nr, RP_REFINE_TARGET_E, d_cinfo.as<int>(), d_rs);
+    };
+    int refine_queued = 0;
+    if (refine) {
+        KTimer ktr(ctx, KT_REFINE);
+        for (; refine_queued < std::min(refine_cap, REFINE_EAGER); ++refine_queued) refine_step();
     }
     {
         // sensitivity to the fp64 rounding of the system: one more solve with the same factor (the
@@ -1783,6 +1861,18 @@ int run_solve(rp_fit *fit, const rp_constraints *con, double *q_out, double *sol
     RP_HIP(ctx, hipGetLastError());
     RP_HIP(ctx, hipEventRecord(e1, st));
     RP_HIP(ctx, hipStreamSynchronize(st));
+    if (refine && h_out[0] == 0.0 && h_out[5 + N + n] != 0.0 && refine_queued < refine_cap) {
+        // the eager steps left the bound above the target: the rest of the allowed steps
+        {
+            KTimer ktr(ctx, KT_REFINE);
+            for (; refine_queued < refine_cap; ++refine_queued) refine_step();
+        }
+        k_publish<<<ceil_div(N + 1, 256), 256, 0, st>>>(d_x.as<double>(), dc_exp, N, n, d_cinfo.as<int>(), d_rs,
+                                                        h_out_dev);
+        RP_HIP(ctx, hipGetLastError());
+        RP_HIP(ctx, hipEventRecord(e1, st));
+        RP_HIP(ctx, hipStreamSynchronize(st));
+    }
     const int cinfo = (int)h_out[0];
     fit->solver = RP_SOLVER_CHOLESKY;
     if (cinfo == 0) {
@@ -1854,18 +1944,13 @@ int run_solve(rp_fit *fit, const rp_constraints *con, double *q_out, double *sol
         fit->refine_bound = -1.0;
         if (refine) {
             KTimer ktr(ctx, KT_REFINE);
-            const int64_t M = fit->n_keep, rows = fit->phi_rows, plane = (int64_t)NPg * rows;
-            double *e = d_e.as<double>(), *esum = e + M + 2;
             double *u = d_rv.as<double>(), *r = u + N, *r1 = r + N, *z = r1 + nr, *dx = z + nr, *r2 = dx + nr + 1;
-            const double *phi = fit->ws[7].as<double>();
+            (void)z;
             const int *ci = d_cinfo.as<int>();
             const int steps = fit->opt.refine_steps > 0 ? fit->opt.refine_steps : RP_REFINE_STEPS_DEFAULT;
             k_ref_reset<<<1, 1, 0, st>>>(d_cinfo.as<int>(), d_rs);
             for (int it = 0; it < steps; ++it) {
-                k_ref_expand<<<ceil_div(N, 256), 256, 0, st>>>(d_x.as<double>(), dc_inv_ptr, dc_inv_set, dc_inv_mult, N,
-                                                               u, ci, d_rs);
-                k_ref_rows<<<n_esum, 256, 0, st>>>(phi, rows, plane, N, M, fit->d_v.as<double>(), u, e, esum, ci, d_rs);
-                k_ref_cols<<<N, 256, 0, st>>>(phi, rows, plane, M, e, esum, n_esum, r, ci, d_rs);
+                residual_pass();
                 k_ref_rhs<<<1, 256, 0, st>>>(r, dc_ptr, dc_src, dc_mult, dc_w, con->diag_add ? dc_diag : nullptr,
                                              con->rhs_add ? dc_rhs : nullptr, con->q_tot, nr, d_x.as<double>(), r1, r2,
                                              ci, d_rs);

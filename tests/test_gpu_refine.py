@@ -135,6 +135,41 @@ def test_refined_flags_ill_conditioned_cell(gpu_ctx, capfd):
     assert rep["well_conditioned"] and rep["within_target"] and 0.0 <= rep["sensitivity"] <= 1e-8
 
 
+def test_refined_streamed_equals_kept(tmp_path):
+    """The refined form without the kept basis (each step recomputes the basis chunk by chunk): forced by
+    the option flag, and chosen by the fit itself under a budget that cannot hold the kept basis.  Four
+    chunks of 65 536 points (mof1000): the charges equal the kept form's bitwise, within less memory."""
+    s = synth.mof1000()
+    symm = resp = None
+    if s.resp_text:
+        resp = tmp_path / "resp.txt"
+        resp.write_text(s.resp_text)
+    con = params.constraint_arrays(s.n_atoms, s.atomic_numbers, s.q_tot, None,
+                                   params.read_resp_params(str(resp)) if resp else None, None)
+    out = {}
+    for mode in ("kept", "forced", "budget"):
+        print(f"refined mof1000, basis {mode}", flush=True)
+        budget = 0 if mode != "budget" else out["forced"][2] + (256 << 20)
+        ctx = _abi.Context(0, budget)
+        f = _abi.Fit(ctx, s.n_grid, s.axis, s.atomic_numbers, s.atom_pos, s.esp_cube, _abi.RP_ORDER_CUBE, 1,
+                     _abi.RP_PREC_OZAKI_REFINED, flags=_abi.RP_OPT_STREAM_REFINE if mode == "forced" else 0)
+        f.accumulate()
+        q, _ = f.solve(con)
+        rep = f.solve_report()
+        out[mode] = (q, rep, ctx.hbm_in_use())
+        print(f"  device bytes {out[mode][2] / 1e9:.2f} GB, report {rep}", flush=True)
+        f.close()
+        ctx.close()
+    plan = _abi.chunk_plan(252456, s.n_atoms, 754, _abi.RP_PREC_OZAKI_REFINED)
+    kept_bytes = 8 * plan["np"] * plan["rows"] * plan["n_chunks"]
+    assert plan["n_chunks"] == 4
+    for mode in ("forced", "budget"):
+        np.testing.assert_array_equal(out[mode][0], out["kept"][0])
+        assert out[mode][1]["refine_steps"] == out["kept"][1]["refine_steps"] >= 1
+        assert out[mode][2] <= out["kept"][2] - kept_bytes // 2
+    assert out["budget"][2] <= out["forced"][2] + (256 << 20)
+
+
 def test_refined_deterministic_and_basis_kept(gpu_ctx, tmp_path):
     """Two fits are bitwise equal; the --stats matvec (its own chunk buffer) between two solves leaves
     the kept basis intact, so the second solve repeats the first bitwise; refit does the same."""
@@ -154,7 +189,9 @@ def test_refined_deterministic_and_basis_kept(gpu_ctx, tmp_path):
 
 
 def test_refined_step_limit(gpu_ctx):
-    """refine_steps bounds the steps; a step runs only while the last correction exceeded 3e-4 e."""
+    """refine_steps bounds the steps; a step runs only while the error bound the previous ones left
+    (rho / (1 - rho) |dq|, rho = 1e-2 after the first step) exceeds 1e-6 e.  A 400-atom cluster with
+    cond(A) ~ 2e12 needs more than the two steps queued with the solve, so the host queues the rest."""
     s = synth.water()
     con = params.constraint_arrays(s.n_atoms, s.atomic_numbers, 0.0, None, None, None)
     out = {}
@@ -164,14 +201,27 @@ def test_refined_step_limit(gpu_ctx):
                      _abi.RP_PREC_OZAKI_REFINED, refine_steps=steps)
         f.accumulate()
         q, _ = f.solve(con)
-        out[steps] = (q, *f.refine_stats())
+        out[steps] = (q, f.solve_report())
         f.close()
-    assert out[1][1] == 1
-    n3, last3 = out[3][1], out[3][2]
-    assert 1 <= n3 <= 3
-    assert n3 == 3 or last3 <= 3e-4
-    # steps after the first only shrink a correction that was already at most 3e-4 e
-    assert np.max(np.abs(out[1][0] - out[3][0])) <= 3e-4
+    assert out[1][1]["refine_steps"] == 1
+    r3 = out[3][1]
+    assert 1 <= r3["refine_steps"] <= 3
+    assert r3["refine_steps"] == 3 or r3["error_bound"] <= 1e-6
+    assert np.max(np.abs(out[1][0] - out[3][0])) <= 1e-4
+    c = synth.cluster(400)
+    con = params.constraint_arrays(c.n_atoms, c.atomic_numbers, 0.0, None, None, None)
+    reps = {}
+    for steps in (2, 6):
+        print(f"refined step limit {steps}: 400-atom cluster", flush=True)
+        f = _abi.Fit(gpu_ctx, c.n_grid, c.axis, c.atomic_numbers, c.atom_pos, c.esp_cube, _abi.RP_ORDER_CUBE, 1,
+                     _abi.RP_PREC_OZAKI_REFINED, refine_steps=steps)
+        f.accumulate()
+        f.solve(con)
+        reps[steps] = f.solve_report()
+        f.close()
+        print(f"  {reps[steps]}", flush=True)
+    assert reps[2]["refine_steps"] == 2 and not reps[2]["within_target"]
+    assert 2 < reps[6]["refine_steps"] <= 6 and reps[6]["within_target"]
 
 
 def test_refined_large_n_matches_fp64(gpu_ctx):

@@ -87,21 +87,31 @@ def test_request_bound_covers_device_memory(stream12):
                            (REFINED, _abi.RP_PREC_OZAKI_REFINED)):
             print(f"bound case {s.name} precision={prec}", flush=True)
             bound = _abi.Server.request_bytes(_args(s), con, opts)
-            ctx = _abi.Context(0, 0)
+            # a context capped at the bound: the fit must run within it (the refined form streams its
+            # refinement when keeping the basis would not fit), --stats path included
+            ctx = _abi.Context(0, bound)
             f = _abi.Fit(ctx, *_args(s), _abi.RP_ORDER_CUBE, 1, prec)
             f.accumulate()
             q, _ = f.solve(con)
-            f.phi_matvec(q)                   # --stats path on the same fit
+            f.phi_matvec(q)
             used = ctx.hbm_in_use()
             f.close()
             ctx.close()
             assert used <= bound, (s.name, prec, used, bound)
-            # and not absurdly loose; the refined form keeps the basis of every kept point, which the
-            # bound (fixed before the filter) takes for the whole grid
-            kept = 0
+            # and not absurdly loose (the bound takes the kept count as the whole grid)
+            assert bound < 4 * used + (64 << 20), (s.name, prec, used, bound)
             if prec == _abi.RP_PREC_OZAKI_REFINED:
-                kept = 8 * (-(-(s.n_atoms + 2) // 128) * 128) * int(np.prod(s.n_grid))
-            assert bound < 4 * used + kept + (64 << 20), (s.name, prec, used, bound)
+                # uncapped, the refined form keeps its basis on top of the bound's buffers
+                ctx = _abi.Context(0, 0)
+                f = _abi.Fit(ctx, *_args(s), _abi.RP_ORDER_CUBE, 1, prec)
+                f.accumulate()
+                q2, _ = f.solve(con)
+                plan = _abi.chunk_plan(f.info.n_keep, s.n_atoms, f.info.n_k, prec)
+                kept = 8 * plan["np"] * plan["rows"] * plan["n_chunks"]
+                assert ctx.hbm_in_use() <= bound + kept
+                f.close()
+                ctx.close()
+                np.testing.assert_array_equal(q2, q)
 
 
 def test_server_mix_equals_alone(stream12):
