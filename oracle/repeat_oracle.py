@@ -164,6 +164,66 @@ def filter_mask_cdist(n_grid, axis, eff_pos, eff_rmin, eff_rmax):
     return (valid & near).astype(np.uint8)
 
 
+def filter_mask_cpu_slab(n_grid, axis, eff_pos, eff_rmin, eff_rmax, global_max_r, n_workers):
+    """The reference's CPU filter as it runs (``repeat.py:2034-2079, 2115-2180``), for small grids.
+
+    The grid is cut into z-chunks of whole layers (one chunk below 50 000 points, otherwise
+    ``n_workers`` of them, the worker count the reference derives from the host's cores), each chunk
+    into batches of ``512 MB // (8 E)`` points in flat order, and each batch tests only the atoms
+    with ``|z_atom - mid| < half + max_r + 1``, where ``mid`` and ``half`` come from the batch's first
+    and last layer index times ``axis[2][2]`` alone.  Points: ``base_xy + k * axis[2]`` with
+    ``base_xy = i * axis[0] + j * axis[1]`` per component; distances by ``cdist``.
+    """
+    from scipy.spatial.distance import cdist
+    nx, ny, nz = (int(v) for v in n_grid)
+    axis = np.asarray(axis, dtype=np.float64)
+    eff_pos = np.asarray(eff_pos, dtype=np.float64)
+    eff_rmin = np.asarray(eff_rmin, dtype=np.float64)
+    eff_rmax = np.asarray(eff_rmax, dtype=np.float64)
+    j_grid, i_grid = np.meshgrid(np.arange(ny, dtype=np.int32), np.arange(nx, dtype=np.int32), indexing="ij")
+    ig, jg = i_grid.ravel(), j_grid.ravel()
+    base_xy = np.zeros((nx * ny, 3), dtype=np.float64)
+    for c in range(3):
+        base_xy[:, c] = ig * axis[0][c] + jg * axis[1][c]
+    layer = nx * ny
+    n_total = layer * nz
+    if n_total < 50000:
+        tasks = [(0, nz)]
+    else:
+        w = min(int(n_total / 50000), max(1, min(int(n_workers), nz)))
+        size = (nz + w - 1) // w
+        tasks = [(a * size, min((a + 1) * size, nz)) for a in range(w) if a * size < min((a + 1) * size, nz)]
+    n_eff = len(eff_pos)
+    batch = max(1, (512 * 1024 * 1024) // (n_eff * 8)) if n_eff > 0 else None
+    z_vec = axis[2]
+    valid_all, near_all = [], []
+    for z0, z1 in tasks:
+        total = (z1 - z0) * layer
+        valid = np.empty(total, dtype=bool)
+        near = np.empty(total, dtype=bool)
+        step = batch or total
+        for s0 in range(0, total, step):
+            s1 = min(s0 + step, total)
+            idx = np.arange(s0, s1)
+            kg = z0 + idx // layer
+            coords = base_xy[idx % layer] + (kg[:, None] * z_vec)
+            if n_eff == 0:
+                valid[s0:s1], near[s0:s1] = True, False
+                continue
+            zmin, zmax = kg[0] * z_vec[2], kg[-1] * z_vec[2]
+            mid, half = (zmin + zmax) / 2.0, (zmax - zmin) / 2.0
+            sel = np.abs(eff_pos[:, 2] - mid) < (half + global_max_r + 1.0)
+            if not np.any(sel):
+                valid[s0:s1], near[s0:s1] = True, False
+                continue
+            d = cdist(coords, eff_pos[sel])
+            valid[s0:s1] = np.all(d > eff_rmin[sel], axis=1)
+            near[s0:s1] = np.any(d <= eff_rmax[sel], axis=1)
+        valid_all.append(valid)
+        near_all.append(near)
+    return (np.concatenate(valid_all) & np.concatenate(near_all)).astype(np.uint8)
+
+
 def compact(keep, esp_flat, n_grid, axis):
     """Kept positions in ascending flat index and mean-subtracted ESP (``repeat.py:2081-2113``).
 

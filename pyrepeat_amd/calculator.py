@@ -33,7 +33,8 @@ import numpy as np
 
 from . import _abi
 from .cube import read_data, v_pot_view
-from .params import constraint_arrays, read_qeq_params, read_resp_params, read_symmetry
+from .params import (constraint_arrays, read_qeq_params, read_resp_params, read_symmetry,
+                     reference_cpu_filter_is_exact)
 from .tables import ATOM_SYMBOLS, DEFAULT_QEQ_PARAMS
 
 PRECISIONS = {"fp64": _abi.RP_PREC_FP64, "split": _abi.RP_PREC_OZAKI, "refined": _abi.RP_PREC_OZAKI_REFINED,
@@ -59,6 +60,11 @@ class ChargeCalculator:
                              self.cube["atom_pos"], self.cube["esp_cube"],
                              self.cube.get("esp_order", _abi.RP_ORDER_CUBE), self.fit_flag,
                              PRECISIONS[precision], self.vdw_factor, self.vdw_max, self.R_cutoff)
+        if not reference_cpu_filter_is_exact(self.cube["axis_vector"], self.cube["n_grid"], self.vdw_factor,
+                                             self.vdw_max):
+            print("Note: for this grid the reference's CPU filter prefilters atoms by a z-slab that ignores the "
+                  "z components of the first two grid vectors; its mask then depends on its worker count.  This "
+                  "build applies the exact keep predicate (as the reference's GPU filter does).", file=sys.stderr)
         self.symm_data = read_symmetry(symm_file) if symm_file else None
         self.resp_data = read_resp_params(resp_file) if resp_file else None
         self.qeq_data = read_qeq_params(qeq_file) if qeq_file else None

@@ -146,3 +146,27 @@ def constraint_arrays(n_atoms, atom_types, q_tot=0.0, symm=None, resp=None, qeq=
             "red_src": np.asarray(red_src, dtype=np.int32), "red_mult": np.asarray(red_mult, dtype=np.float64),
             "weights": np.asarray(weights, dtype=np.float64), "diag_add": diag, "rhs_add": rhs,
             "expand_src": src, "independent": np.asarray(independent, dtype=np.int64)}
+
+
+def reference_cpu_filter_is_exact(axis, n_grid, vdw_factor, vdw_max):
+    """Whether the reference's CPU filter gives the exact keep mask for this grid and these options.
+
+    That filter (``repeat.py:2115-2180``) tests a batch of points only against the effective atoms with
+    ``|z_atom - mid| < half + max_r + 1``, where ``mid`` and ``half`` come from the batch's layer indices
+    times ``axis[2][2]`` alone, and its batches follow its worker count (the host's cores).  A point's z
+    is ``x + k axis[2][2]`` with ``x = i axis[0][2] + j axis[1][2]``.  With ``axis[2][2] >= 0`` every
+    ``k axis[2][2]`` of a batch lies within ``half`` of ``mid``, so an atom the test drops is farther than
+    ``max_r + 1 - |x|`` from every point of the batch in z alone; if ``|x| < 1`` over the whole grid and
+    ``r_min <= r_max <= max_r`` (``vdw_factor <= vdw_max``), it is neither within ``r_min`` nor within
+    ``r_max`` of any of them, and dropping it changes nothing (a batch with no atom left keeps nothing, as
+    the full test would not either).  Then the CPU mask equals the exact predicate that this build and
+    the reference's GPU filter (``repeat.py:349-437``) apply, whatever the worker count.  Otherwise the
+    reference's own CPU mask can differ from that, and depends on its worker count.
+    """
+    axis = np.asarray(axis, dtype=np.float64)
+    nx, ny = int(n_grid[0]), int(n_grid[1])
+    if not vdw_factor <= vdw_max or axis[2][2] < 0.0:
+        return False
+    corners = [0.0, (nx - 1) * axis[0][2], (ny - 1) * axis[1][2], (nx - 1) * axis[0][2] + (ny - 1) * axis[1][2]]
+    return max(abs(c) for c in corners) < 0.999
+

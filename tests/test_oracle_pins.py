@@ -396,3 +396,48 @@ def test_compaction_order_and_centring(oracle):
     f = flat[len(flat) // 2]
     i, j, k = f % nx, (f // nx) % ny, f // (nx * ny)
     np.testing.assert_array_equal(pos[len(flat) // 2], (i * s.axis[0] + j * s.axis[1]) + k * s.axis[2])
+
+
+# ------------------------------------------------------------------ the reference's CPU filter batches
+
+def _slab_case(oracle, axis, n, n_atoms, vdw_max, seed):
+    rng = np.random.default_rng(seed)
+    axis = np.asarray(axis, dtype=np.float64)
+    box = axis * n
+    pos = synth.place_atoms(rng, box, n_atoms, 2.0)
+    Z = rng.choice([1, 6, 7, 8], size=n_atoms)
+    eff, rmin, rmax, gmax = oracle.effective_atoms(pos, Z, axis, box, 1.0, vdw_max)
+    exact = oracle.filter_mask([n, n, n], axis, eff, rmin, rmax)
+    slab = {w: oracle.filter_mask_cpu_slab([n, n, n], axis, eff, rmin, rmax, gmax, w) for w in (1, 2)}
+    return exact, slab
+
+
+@pytest.mark.parametrize("triclinic,vdw_max", [(False, 1000.0), (True, 1.6), (True, 3.0)])
+def test_reference_cpu_slab_is_exact_on_standard_cells(oracle, triclinic, vdw_max):
+    """With the first two grid vectors in the xy plane (every configuration here: cubes written with a along
+    x and b in xy) the reference's CPU z-slab prefilter changes nothing: its mask equals the exact predicate
+    for one worker and for two (48^3 > 100 000 points, so two z-chunks), and the host's check agrees."""
+    from pyrepeat_amd.params import reference_cpu_filter_is_exact
+    box = synth.triclinic_box(18.0, 19.0, 17.0, 80.0, 97.0, 104.0) if triclinic else np.eye(3) * 18.0
+    axis = box / 48
+    assert reference_cpu_filter_is_exact(axis, [48, 48, 48], 1.0, vdw_max)
+    exact, slab = _slab_case(oracle, axis, 48, 40, vdw_max, 3)
+    assert exact.sum() > 0
+    for w, m in slab.items():
+        np.testing.assert_array_equal(m, exact)
+
+
+def test_reference_cpu_slab_differs_on_tilted_cells(oracle):
+    """A grid whose first vector points down in z (x_p = i axis[0][2] reaches -21 Bohr) at vdw-max 1.6: with
+    one z-chunk the slab keeps every effective atom and the mask is exact; with two chunks (a host with two
+    or more cores) the second chunk's slab drops atoms near its tilted points, and the reference's CPU
+    mask differs from the exact predicate on hundreds of points (kept inside a dropped atom's r_min, or
+    dropped for want of a near atom).  The host's check flags the cell; the device applies the exact
+    predicate."""
+    from pyrepeat_amd.params import reference_cpu_filter_is_exact
+    axis = [[0.5, 0.0, -0.45], [0.05, 0.5, 0.0], [0.0, 0.0, 0.5]]
+    assert not reference_cpu_filter_is_exact(axis, [48, 48, 48], 1.0, 1.6)
+    exact, slab = _slab_case(oracle, axis, 48, 150, 1.6, 5)
+    np.testing.assert_array_equal(slab[1], exact)
+    assert (slab[2] != exact).sum() > 100
+
