@@ -243,7 +243,7 @@ def serve_main(args):
     from pyrepeat_amd import _abi, synth
     arrivals_all, _ = synth.stream_schedule()
     R = args.requests
-    arrivals = arrivals_all[:R]
+    arrivals = arrivals_all[:R] * (20.0 / args.rate)
     t0 = time.perf_counter()
     systems = synth.stream_requests(R)
     _log(f"{R} stream requests generated in {time.perf_counter() - t0:.1f} s")
@@ -283,13 +283,13 @@ def serve_main(args):
     queue_wait = np.array([r["t_admit"] - r["arrival"] for _, r in res])
     n_atoms = np.array([s.n_atoms for s in systems])
     line = {
-        "metric": "charge fits/s + p50/p99 latency, serving stream (Poisson 20 req/s, 20-800 atoms)",
+        "metric": f"charge fits/s + p50/p99 latency, serving stream (Poisson {args.rate:g} req/s, 20-800 atoms)",
         "value": round(R / span, 4), "unit": "fits/s", "n_gpus": 1, "steps": R, "warmup": args.slots,
         "ms_per_step": round(1e3 * span / R, 3), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f64 basis/solve/refinement, i8 slices -> i32 exact Gram" if args.chosen == "refined" else
                  "f64 basis/solve, i8 slices -> i32 exact Gram",
         "data": "synthetic stream (seeded Ewald ESP + 1e-5 noise, fp32 cubes)",
-        "config": {"workload": f"stream{R}", "requests": R, "rate_per_s": 20.0, "slots": args.slots,
+        "config": {"workload": f"stream{R}", "requests": R, "rate_per_s": args.rate, "slots": args.slots,
                    "hbm_budget_gb": args.budget_gb, "atoms_min": int(n_atoms.min()), "atoms_max": int(n_atoms.max()),
                    "grid_points_total": int(sum(int(np.prod(s.n_grid)) for s in systems)),
                    "precision": "split (int8 Ozaki, 4 slices)" if args.chosen == "split" else
@@ -323,6 +323,9 @@ def main():
     ap.add_argument("--requests", type=int, default=512)
     ap.add_argument("--slots", type=int, default=4)
     ap.add_argument("--budget-gb", type=float, default=200.0)
+    ap.add_argument("--rate", type=float, default=20.0,
+                    help="serving: arrival rate of the open-loop stream (requests/s); the seeded exponential gaps "
+                         "of the 20 req/s schedule are scaled to it")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="first run one device copy of this many MiB (counter calibration under rocprofv3 --pmc)")
     args = ap.parse_args()

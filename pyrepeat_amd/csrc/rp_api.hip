@@ -195,6 +195,7 @@ int prepare_device(rp_fit *fit, FitDevice &dv, int precision) {
             dv.epi.h[i] = std::fabs(det) / std::sqrt(cr[0] * cr[0] + cr[1] * cr[1] + cr[2] * cr[2]) * (1.0 - 1e-12);
         }
         std::copy(B, B + 9, dv.epi.box);
+        for (int i = 0; i < 3; ++i) dv.epi.nmax[i] = s.nmax[i];
         dv.epi.inv_n[0] = 1.0 / s.nx;
         dv.epi.inv_n[1] = 1.0 / s.ny;
         dv.epi.inv_n[2] = 1.0 / s.nz;

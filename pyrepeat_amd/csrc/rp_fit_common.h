@@ -43,8 +43,44 @@ struct EpiArgs {
     int S;
     double R2, sqrt_alpha, r_cut;
     double box[9];         // cell rows
-    double h[3];           // interplanar spacings V / |a_j x a_k|
+    double h[3];           // interplanar spacings V / |a_j x a_k|, shrunk by a relative 1e-12
     double inv_n[3];       // 1 / n_grid
+    int nmax[3];           // extent of the full shift list, |n_i| <= nmax[i] (repeat.py:815-822)
+
+    // The shift-list sum for one (point, atom) pair, visiting only the shifts that can lie within Rc:
+    // image n is at distance >= |d_i - n_i| h_i from the point along plane family i (d = g_f - a_f, the
+    // fractional displacement), so n_i runs over [d_i - Rc/h_i, d_i + Rc/h_i] within the list's extent.
+    // The candidates are visited in the list's order (n_0 outer, n_2 inner), each shift vector rounded as
+    // the list's ((n0 b0 + n1 b1) + n2 b2), so the terms inside Rc and their order are those of the loop
+    // over the list: the same sum.  Every shift the list's AABB pruning drops is farther than Rc from
+    // every kept point, so it adds nothing either way.
+    template <class Term>
+    __device__ __forceinline__ double image_sum(double d0, double d1, double d2, double gx, double gy, double gz,
+                                                double ax, double ay, double az, Term term) const {
+        int lo[3], hi[3];
+        const double dd[3] = {d0, d1, d2};
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            const double reach = r_cut / h[i] + 1e-9;
+            lo[i] = max(-nmax[i], (int)ceil(dd[i] - reach));
+            hi[i] = min(nmax[i], (int)floor(dd[i] + reach));
+        }
+        double sum = 0.0;
+        for (int n0 = lo[0]; n0 <= hi[0]; ++n0)
+            for (int n1 = lo[1]; n1 <= hi[1]; ++n1)
+                for (int n2 = lo[2]; n2 <= hi[2]; ++n2) {
+                    const double sx = ((double)n0 * box[0] + (double)n1 * box[3]) + (double)n2 * box[6];
+                    const double sy = ((double)n0 * box[1] + (double)n1 * box[4]) + (double)n2 * box[7];
+                    const double sz = ((double)n0 * box[2] + (double)n1 * box[5]) + (double)n2 * box[8];
+                    const double dx = gx - (ax + sx), dy = gy - (ay + sy), dz = gz - (az + sz);
+                    const double r2 = (dx * dx + dy * dy) + dz * dz;
+                    if (r2 <= R2) {
+                        if (sx == 0.0 && sy == 0.0 && sz == 0.0 && r2 <= 1e-20) continue;
+                        sum += term(r2);
+                    }
+                }
+        return sum;
+    }
 
     // Nearest image of atom a seen from grid point (i, j, k): shift index n0 = rint(g_f - a_f) and the
     // lower bound min_i (1 - |w_i|) h_i on the distance to every other image (w = g_f - a_f - n0).
@@ -83,6 +119,18 @@ struct EpiArgs {
             if (sx == 0.0 && sy == 0.0 && sz == 0.0 && r2 <= 1e-20) return 0.0;
             const double r = sqrt(r2);
             return erfc(sqrt_alpha * r) / r;
+        }
+        if (f >= 0) {
+            // several images may lie within Rc: the candidate shifts only
+            int i, j, k;
+            geom.ijk(f, i, j, k);
+            const double d0 = (double)i * inv_n[0] - afrac[3 * a], d1 = (double)j * inv_n[1] - afrac[3 * a + 1],
+                         d2 = (double)k * inv_n[2] - afrac[3 * a + 2];
+            const double sa = sqrt_alpha;
+            return image_sum(d0, d1, d2, gx, gy, gz, ax, ay, az, [sa](double r2) {
+                const double r = sqrt(r2);
+                return erfc(sa * r) / r;
+            });
         }
         double sum = 0.0;
         for (int s = 0; s < S; ++s) {

@@ -350,19 +350,14 @@ __global__ __launch_bounds__(SEP_THREADS) void k_sep_phi(SepArgs p) {
                     if (r2 <= e.R2 && !(n0 == 0.0 && n1 == 0.0 && n2 == 0.0 && r2 <= 1e-20))
                         real = erfc_over_r(r2, alpha, e.sqrt_alpha);
                 } else {
-                    // several images within Rc: the shift-list sum with the reference roundings
+                    // several images may lie within Rc: the shift-list sum over the candidate shifts only,
+                    // with the reference roundings and order (EpiArgs::image_sum)
                     const double gx = (id * e.geom.ax[0] + jd * e.geom.ax[3]) + md * e.geom.ax[6];
                     const double gy = (id * e.geom.ax[1] + jd * e.geom.ax[4]) + md * e.geom.ax[7];
                     const double gz = (id * e.geom.ax[2] + jd * e.geom.ax[5]) + md * e.geom.ax[8];
-                    double sum = 0.0;
-                    for (int t = 0; t < e.S; ++t) {
-                        const double sx = e.shifts[3 * t], sy = e.shifts[3 * t + 1], sz = e.shifts[3 * t + 2];
-                        const double dx = gx - (Rx + sx), dy = gy - (Ry + sy), dz = gz - (Rz + sz);
-                        const double r2 = (dx * dx + dy * dy) + dz * dz;
-                        if (r2 <= e.R2 && !(sx == 0.0 && sy == 0.0 && sz == 0.0 && r2 <= 1e-20))
-                            sum += erfc_over_r(r2, alpha, e.sqrt_alpha);
-                    }
-                    real = sum;
+                    const double sa = e.sqrt_alpha;
+                    real = e.image_sum(d0, d1, d2, gx, gy, gz, Rx, Ry, Rz,
+                                       [alpha, sa](double r2) { return erfc_over_r(r2, alpha, sa); });
                 }
             }
             val = acc + real;

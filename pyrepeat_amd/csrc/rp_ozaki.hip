@@ -244,10 +244,12 @@ __global__ void k_zero_u64(unsigned long long *p, int n) {
 
 namespace {
 
-// Whether the refined form can keep its basis (`need` bytes in ws[7]): within the context's budget (which
-// its grow hook may raise) and the device's free memory with 1 GiB to spare; an existing ws[7] is freed
-// before a larger one is taken.
-bool keep_basis_fits(rp_fit *fit, uint64_t need) {
+// Whether the refined form can keep its basis (`need` bytes in ws[7]), decided after every other buffer of
+// the accumulate is allocated.  The device's free memory must hold it with 1 GiB to spare.  Under a budget
+// with a grow hook (a serving slot, whose reservation already covers everything but the kept basis) the
+// hook must grant its size; under a plain budget it must fit beside what the solve still allocates
+// (solve_reserve).  An existing ws[7] is freed before a larger one is taken.
+bool keep_basis_fits(rp_fit *fit, uint64_t need, uint64_t solve_reserve) {
     Ctx *ctx = fit->ctx;
     if (fit->opt.flags & RP_OPT_STREAM_REFINE) return false;
     DBuf &kept = fit->ws[7];
@@ -256,10 +258,19 @@ bool keep_basis_fits(rp_fit *fit, uint64_t need) {
     size_t free_b = 0, total_b = 0;
     if (hipMemGetInfo(&free_b, &total_b) == hipSuccess && need + (1ull << 30) > (uint64_t)free_b + held) return false;
     if (ctx->budget) {
-        const uint64_t after = ctx->in_use - held + need;
-        if (after > ctx->budget && !(ctx->grow_budget && ctx->grow_budget(after - ctx->budget))) return false;
+        if (ctx->grow_budget) return ctx->grow_budget(need - held);
+        if (ctx->in_use - held + need + solve_reserve > ctx->budget) return false;
     }
     return true;
+}
+
+// Device bytes the solve allocates at most after an accumulate with N atoms and M kept points (N' <= N):
+// the bordered system and its copy, the diagonal-block inverses, the least-squares eigenvectors, the
+// refinement vectors and the residual over the kept points, with 1 MiB for the small arrays.
+uint64_t solve_reserve_bytes(int N, int64_t M) {
+    const uint64_t n = (uint64_t)N + 1, ld = (uint64_t)round_up((int64_t)n + 1, 4);
+    return 8 * (2 * n * ld + n * n + 64 * 64 * (uint64_t)ceil_div(N, 64) + 4 * 256 * (uint64_t)N) +
+           8 * ((uint64_t)M + 2 + (uint64_t)ceil_div(M, 512)) + (1u << 20);
 }
 
 // P slices per element; keep_phi: every chunk of the fp64 basis stays in ws[7] ([chunk][NP][rows]) for
@@ -276,20 +287,13 @@ int accumulate_split_p(rp_fit *fit, FitDevice &dv, bool keep_phi) {
     if (cp.NP != NP || cp.slices != P || TM != 64 || TN != 128 || rows % 512 != 0)
         return ctx->fail(RP_EINVAL, "split path: chunk plan does not match the kernel tiling");
     const int64_t n_chunks = ceil_div(M, rows);
-    const bool refined = keep_phi;
-    if (refined) {
-        keep_phi = keep_basis_fits(fit, sizeof(double) * (uint64_t)NP * rows * n_chunks);
-        if (!keep_phi) fit->ws[7].release();   // the memory is short: give back a basis kept before
-    }
-    fit->refine_stream = refined && !keep_phi;
-    DBuf &d_phi = fit->ws[keep_phi ? 7 : 2], &d_part = fit->ws[3], &d_slices = fit->ws[4], &d_colmax = fit->ws[5],
-         &d_scale = fit->ws[6], &d_tiles = fit->ws[9];
+    DBuf &d_part = fit->ws[3], &d_slices = fit->ws[4], &d_colmax = fit->ws[5], &d_scale = fit->ws[6],
+         &d_tiles = fit->ws[9];
     SepPlan &plan = dv.sep_plan;
     {
         KTimer kt(ctx, KT_PHASE);
         RP_CHECK(sep_prepare(fit, dv, rows, plan));
     }
-    RP_CHECK(d_phi.grow(ctx, sizeof(double) * NP * rows * (keep_phi ? n_chunks : 1)));
     RP_CHECK(d_slices.grow(ctx, (size_t)P * NP * rows));
     RP_CHECK(d_colmax.grow(ctx, sizeof(unsigned long long) * NP));
     RP_CHECK(d_scale.grow(ctx, sizeof(double) * NP));
@@ -316,6 +320,15 @@ int accumulate_split_p(rp_fit *fit, FitDevice &dv, bool keep_phi) {
         fit->split_tiles_np = NP;
     }
     RP_CHECK(d_part.grow(ctx, sizeof(double) * (size_t)splits * n_tiles * TM * TN));
+    // the basis: every chunk kept (refined, when it fits) or one chunk buffer
+    const bool refined = keep_phi;
+    if (refined) {
+        keep_phi = keep_basis_fits(fit, sizeof(double) * (uint64_t)NP * rows * n_chunks, solve_reserve_bytes(N, M));
+        if (!keep_phi) fit->ws[7].release();   // the memory is short: give back a basis kept before
+    }
+    fit->refine_stream = refined && !keep_phi;
+    DBuf &d_phi = fit->ws[keep_phi ? 7 : 2];
+    RP_CHECK(d_phi.grow(ctx, sizeof(double) * NP * rows * (keep_phi ? n_chunks : 1)));
 
     const size_t lds_bytes = gram_lds_bytes_dma<P>();
     RP_HIP(ctx, hipFuncSetAttribute(reinterpret_cast<const void *>(&k_oz_gram<P>),
