@@ -312,16 +312,15 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="mof1000")
-    ap.add_argument("--chosen", choices=["split", "refined"], default=None,
-                    help="the chosen path: split (4 slices) or refined (3 slices + fp64 refinement); "
-                         "default refined for a single fit, split for the serving stream (small requests, "
-                         "where the refinement's passes cost more than the 3-slice Gram saves)")
+    ap.add_argument("--chosen", choices=["split", "refined"], default="refined",
+                    help="the chosen path: split (4 slices) or refined (3 slices + fp64 refinement, the default: "
+                         "fp64-level charges with an error bound per fit)")
     ap.add_argument("--baseline-steps", type=int, default=None)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--serve", action="store_true", help="run the serving stream instead of the single-fit bench")
     ap.add_argument("--requests", type=int, default=512)
-    ap.add_argument("--slots", type=int, default=4)
+    ap.add_argument("--slots", type=int, default=8)
     ap.add_argument("--budget-gb", type=float, default=200.0)
     ap.add_argument("--rate", type=float, default=20.0,
                     help="serving: arrival rate of the open-loop stream (requests/s); the seeded exponential gaps "
@@ -333,8 +332,6 @@ def main():
         print(json.dumps({"error": "this benchmark runs on one GPU: the fit path is measured on one MI355X; "
                                    "multi-GPU sharding is described, not run"}))
         return 2
-    if args.chosen is None:
-        args.chosen = "split" if args.serve else "refined"
     if args.calibrate_copy_mib:
         calibrate_copy(args.calibrate_copy_mib)
     if args.serve:
