@@ -13,8 +13,9 @@
 //     T2[a][h]       = sum_k T1[m][a][h][k] Y(k,j)    once per grid line (j, m)       (13 terms)
 //     phi_rec        = Re sum_h T2[a][h] X(h,i)       once per kept point on the line (7 terms)
 // all in fp64, which replaces the K = 754-term sum per (point, atom) by about 20 multiply-adds
-// per (point, atom) plus 52 per (line, atom).  The phase factors come from the integer-reduced
-// tables of the fp64 path, so the result equals the direct sum to fp64 rounding.
+// per (point, atom) plus 24 per (line, atom, h): the l and k sums are folded by conjugate symmetry.
+// The phase factors come from the integer-reduced tables of the fp64 path, so the result equals the
+// direct sum to fp64 rounding.
 //
 // Phi rows are produced per chunk of kept points (rows in ascending flat index, so a chunk is a
 // run of whole or partial grid lines): one work-group takes up to SEP_LINES consecutive lines and
@@ -80,40 +81,76 @@ constexpr int SEP_THREADS = 64 * SEP_WAVES;
 constexpr int SEP_LINES = 5;   // lines per work-group: 23 KB of T2 rows, 6 work-groups per CU
 constexpr int T1_PLANES = 16;   // planes per T1 work-group
 
-__device__ __forceinline__ double2 cmac(double2 acc, double2 a, double2 b) {
-    acc.x = fma(a.x, b.x, fma(-a.y, b.y, acc.x));
-    acc.y = fma(a.x, b.y, fma(a.y, b.x, acc.y));
-    return acc;
-}
+// Z(-l, m) = conj Z(l, m), so a sum over l = -6..6 folds into
+//     sum_l c(l) Z(l, m) = c(0) + sum_{l=1..6} [P(l) Re Z(l, m) + Q(l) Im Z(l, m)],
+//     P(l) = c(l) + c(-l),  Q(l) = i (c(l) - c(-l)),
+// 4 multiply-adds per l instead of 8 for the pair.  T1 is stored folded the same way over k, so that
+// T2 = T1[0] + sum_{k=1..6} [P_k Re Y(k, j) + Q_k Im Y(k, j)]: slot 0 of row h holds T1[h][0], slot k
+// holds T1[h][k] + T1[h][-k] and slot KF + k holds i (T1[h][k] - T1[h][-k]).  Only the phase rows of
+// l, k >= 0 are read; the table's row of -l equals the conjugate of the row of l to an ulp.
+constexpr int KF = SK / 2;   // folded pairs per row
 
-// T1[m][a][h][k] for planes [blockIdx.y * T1_PLANES, +T1_PLANES); block = atom, lane = (h, k).
-__global__ __launch_bounds__(128) void k_sep_t1(int N, int nz, const int *__restrict__ kbox,
-                                                const double *__restrict__ kvecs, const double *__restrict__ kcoef,
-                                                const double *__restrict__ atoms, const double2 *__restrict__ ez,
-                                                double2 *__restrict__ T1) {
-    const int a = blockIdx.x, t = threadIdx.x;
-    if (t >= SBOX) return;
-    const double ax = atoms[3 * a], ay = atoms[3 * a + 1], az = atoms[3 * a + 2];
-    double2 c[SK];
-#pragma unroll
-    for (int ll = 0; ll < SK; ++ll) {
-        const int p = kbox[t * SK + ll];
-        c[ll] = make_double2(0.0, 0.0);
+// c_a(h, k, l) folded over l into f[0] = c(0), f[l] = P(l), f[KF + l] = Q(l) for one (h, k).
+__device__ __forceinline__ void t1_fold_coef(const int *__restrict__ kbox_row, const double *__restrict__ kvecs,
+                                             const double *__restrict__ kcoef, double ax, double ay, double az,
+                                             double2 (&f)[SK]) {
+    auto coef = [&](int ll) {
+        const int p = kbox_row[ll];
+        double2 c = make_double2(0.0, 0.0);
         if (p >= 0) {
             // the angle with the rounding of the atom tables: (kx x + ky y) + kz z
             const double th = (kvecs[3 * p] * ax + kvecs[3 * p + 1] * ay) + kvecs[3 * p + 2] * az;
             double s, co;
             sincos(th, &s, &co);
             const double w = kcoef[p];
-            c[ll] = make_double2(w * co, -(w * s));
+            c = make_double2(w * co, -(w * s));
         }
+        return c;
+    };
+    f[0] = coef(SOFF);
+#pragma unroll
+    for (int l = 1; l <= KF; ++l) {
+        const double2 cp = coef(SOFF + l), cm = coef(SOFF - l);
+        f[l] = make_double2(cp.x + cm.x, cp.y + cm.y);
+        f[KF + l] = make_double2(cm.y - cp.y, cp.x - cm.x);
     }
+}
+
+__device__ __forceinline__ double2 t1_fold_sum(const double2 (&f)[SK], const double2 *__restrict__ ez, int nz, int m) {
+    double2 acc = f[0];
+#pragma unroll
+    for (int l = 1; l <= KF; ++l) {
+        const double2 z = ez[(l + 7) * nz + m];
+        acc.x = fma(f[l].x, z.x, fma(f[KF + l].x, z.y, acc.x));
+        acc.y = fma(f[l].y, z.x, fma(f[KF + l].y, z.y, acc.y));
+    }
+    return acc;
+}
+
+// Folded T1[m][a][h][.] for planes [blockIdx.y * T1_PLANES, +T1_PLANES); block = atom, lane = (h, k >= 0).
+constexpr int T1_LANES = SH * (KF + 1);
+__global__ __launch_bounds__(64) void k_sep_t1(int N, int nz, const int *__restrict__ kbox,
+                                               const double *__restrict__ kvecs, const double *__restrict__ kcoef,
+                                               const double *__restrict__ atoms, const double2 *__restrict__ ez,
+                                               double2 *__restrict__ T1) {
+    const int a = blockIdx.x, t = threadIdx.x;
+    if (t >= T1_LANES) return;
+    const int h = t / (KF + 1), k = t - h * (KF + 1);
+    const double ax = atoms[3 * a], ay = atoms[3 * a + 1], az = atoms[3 * a + 2];
+    double2 fp[SK], fm[SK];
+    t1_fold_coef(kbox + (h * SK + SOFF + k) * SK, kvecs, kcoef, ax, ay, az, fp);
+    if (k > 0) t1_fold_coef(kbox + (h * SK + SOFF - k) * SK, kvecs, kcoef, ax, ay, az, fm);
     const int m0 = blockIdx.y * T1_PLANES, m1 = min(m0 + T1_PLANES, nz);
     for (int m = m0; m < m1; ++m) {
-        double2 acc = make_double2(0.0, 0.0);
-#pragma unroll
-        for (int ll = 0; ll < SK; ++ll) acc = cmac(acc, c[ll], ez[(ll - SOFF + 7) * nz + m]);
-        T1[((int64_t)m * N + a) * SBOX + t] = acc;
+        double2 *row = T1 + ((int64_t)m * N + a) * SBOX + h * SK;
+        const double2 tp = t1_fold_sum(fp, ez, nz, m);
+        if (k == 0) {
+            row[0] = tp;
+        } else {
+            const double2 tm = t1_fold_sum(fm, ez, nz, m);
+            row[k] = make_double2(tp.x + tm.x, tp.y + tm.y);
+            row[KF + k] = make_double2(tm.y - tp.y, tp.x - tm.x);
+        }
     }
 }
 
@@ -270,9 +307,13 @@ __global__ __launch_bounds__(SEP_THREADS) void k_sep_phi(SepArgs p) {
 #pragma unroll
                 for (int kk = 0; kk < SK; ++kk) t1[kk] = src[kk];
             }
-            double2 t2 = make_double2(0.0, 0.0);
+            double2 t2 = t1[0];
 #pragma unroll
-            for (int kk = 0; kk < SK; ++kk) t2 = cmac(t2, t1[kk], p.ey[(kk - SOFF + 7) * ny + j]);
+            for (int k = 1; k <= KF; ++k) {
+                const double2 y = p.ey[(k + 7) * ny + j];
+                t2.x = fma(t1[k].x, y.x, fma(t1[KF + k].x, y.y, t2.x));
+                t2.y = fma(t1[k].y, y.x, fma(t1[KF + k].y, y.y, t2.y));
+            }
             t2w[wave][li][a_l][s] = t2;
         }
     }
@@ -482,7 +523,7 @@ int sep_prepare(rp_fit *fit, FitDevice &dv, int64_t rows, SepPlan &plan) {
     RP_HIP(ctx, hipEventRecord(dv.ev_plan, st));
     if (K > 0) {
         dim3 g(N, (unsigned)ceil_div(s.nz, T1_PLANES));
-        k_sep_t1<<<g, 128, 0, st>>>(N, s.nz, dv.sep_kbox.as<int>(), dv.kvecs.as<double>(), dv.kcoef.as<double>(),
+        k_sep_t1<<<g, 64, 0, st>>>(N, s.nz, dv.sep_kbox.as<int>(), dv.kvecs.as<double>(), dv.kcoef.as<double>(),
                                     fit->d_atoms.as<double>(), dv.ez.as<double2>(), dv.sep_t1.as<double2>());
         RP_HIP(ctx, hipGetLastError());
     }
