@@ -80,6 +80,7 @@ constexpr int SEP_ATOMS = 8 * SEP_WAVES;     // atoms per work-group, 8 per wave
 constexpr int SEP_THREADS = 64 * SEP_WAVES;
 constexpr int SEP_LINES = 5;   // lines per work-group: 23 KB of T2 rows, 6 work-groups per CU
 constexpr int T1_PLANES = 16;   // planes per T1 work-group
+constexpr int SEP_XMAX = 2048;  // longest axis 0 whose phase row X(1, .) the basis kernel keeps in LDS
 
 // Z(-l, m) = conj Z(l, m), so a sum over l = -6..6 folds into
 //     sum_l c(l) Z(l, m) = c(0) + sum_{l=1..6} [P(l) Re Z(l, m) + Q(l) Im Z(l, m)],
@@ -268,6 +269,7 @@ __global__ __launch_bounds__(SEP_THREADS) void k_sep_phi(SepArgs p) {
     // disjoint LDS banks
     __shared__ double2 t2w[SEP_WAVES][SEP_LINES][8][SH + 1];
     __shared__ int lrow[SEP_LINES + 1], lkey[SEP_LINES], lj[SEP_LINES], lm[SEP_LINES];
+    extern __shared__ double2 xs[];   // X(1, i) for i < nx (when the reciprocal part is there)
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int a_l = lane >> 3, s = lane & 7;
     // Work-groups go round-robin to the 8 XCDs (linear id L -> XCD L % 8): XCD x takes line groups
@@ -292,6 +294,8 @@ __global__ __launch_bounds__(SEP_THREADS) void k_sep_phi(SepArgs p) {
         lm[threadIdx.x] = key / ny;
         lj[threadIdx.x] = key - (key / ny) * ny;
     }
+    if (p.T1 != nullptr)
+        for (int i = threadIdx.x; i < nx; i += SEP_THREADS) xs[i] = p.ex[nx + i];
     __syncthreads();
     // ---- T2 of every line (lane s = h); the line key is uniform, so the Y row comes through the
     // scalar cache
@@ -331,7 +335,6 @@ __global__ __launch_bounds__(SEP_THREADS) void k_sep_phi(SepArgs p) {
         Fz = e.afrac[3 * a + 2];
     }
     const double alpha = e.sqrt_alpha * e.sqrt_alpha;
-    const double2 *x1 = p.ex + nx;   // X(1, i)
     const int64_t rs = max<int64_t>(lrow[0], p.r0);
     const int64_t rend = min<int64_t>(lrow[n_l], p.r0 + p.valid);
     double cmax = 0.0;
@@ -354,7 +357,7 @@ __global__ __launch_bounds__(SEP_THREADS) void k_sep_phi(SepArgs p) {
             double acc = 0.0;
             if (recip) {
                 // Re sum_h T2[h] X(1, i)^h by Horner from h = HMAX
-                const double2 x = x1[i];
+                const double2 x = xs[i];
                 double2 t = t2w[wave][li][a_l][HMAX];
 #pragma unroll
                 for (int q = HMAX - 1; q >= 1; --q) {
@@ -481,6 +484,8 @@ int sep_prepare(rp_fit *fit, FitDevice &dv, int64_t rows, SepPlan &plan) {
     const Setup &s = fit->setup;
     const int N = fit->n_atoms, K = dv.K;
     const int64_t M = fit->n_keep;
+    if (K > 0 && s.nx > SEP_XMAX)
+        return ctx->fail(RP_EINVAL, "grid axis 0 longer than the basis kernel's phase table (2048 points)");
     if (K > 0) {
         if (!dv.sep_kbox_ok) {   // the k-set's index box, once per Ewald setup
             std::vector<int> kbox(SH * SK * SK, -1);
@@ -562,7 +567,7 @@ void sep_phi_chunk(rp_fit *fit, FitDevice &dv, const SepPlan &plan, int chunk, i
     a.items = (int)ceil_div(a.l_end - a.l_first, SEP_LINES);
     a.items_per_xcd = (int)ceil_div(a.items, 8);
     const int blocks = 8 * a.items_per_xcd * (dv.NP / SEP_ATOMS);
-    k_sep_phi<<<blocks, SEP_THREADS, 0, st>>>(a);
+    k_sep_phi<<<blocks, SEP_THREADS, dv.K > 0 ? sizeof(double2) * fit->setup.nx : 0, st>>>(a);
 }
 
 void sep_slice_chunk(const double *phi, int NP, int64_t rows, int64_t valid, const unsigned long long *colmax,
