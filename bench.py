@@ -13,15 +13,17 @@ ratio, and the fp64 form with the factorized basis beside it, so the gain of
 the precision form itself shows.  Inputs are the seeded synthetic workload
 (exact Ewald ESP plus noise) named in ``--workload``.
 
-Prints one JSON line (rank 0, N = 1).
+Prints one JSON line (rank 0, N = 1).  Its "serving" object is the serving
+stream below at 400 requests/s with 8 slots (``--no-serve`` leaves it out).
 
-    python bench.py --serve [--requests 512] [--slots 4] [--budget-gb 200]
+    python bench.py --serve [--requests 512] [--slots 8] [--rate 20] [--budget-gb 200]
 
 runs the serving stream instead: seeded requests of 20-800 atoms arrive as a
-Poisson process at 20 requests/s and are fitted concurrently by the serving
-loop; the line reports open-loop fits/s with p50/p99 latency (completion -
-arrival), the saturated capacity (every request queued at once), and the
-charges of a sample against the same requests run alone and in plain fp64.
+Poisson process (20 requests/s unless ``--rate``) and are fitted concurrently
+by the serving loop; the line reports open-loop fits/s with p50/p99 latency
+(completion - arrival), the saturated capacity (every request queued at once),
+and the charges of a sample against the same requests run alone and in plain
+fp64.
 """
 from __future__ import annotations
 
@@ -239,24 +241,25 @@ def _run_stream(srv, reqs, opts, arrivals):
     return out, t0
 
 
-def serve_main(args):
+def serve_run(R, slots, rate, budget_gb, chosen):
+    """The serving stream: R requests, open loop at ``rate`` req/s, then all queued at once (capacity),
+    then a sample rerun alone and in fp64.  Returns the result line as a dict."""
     from pyrepeat_amd import _abi, synth
     arrivals_all, _ = synth.stream_schedule()
-    R = args.requests
-    arrivals = arrivals_all[:R] * (20.0 / args.rate)
+    arrivals = arrivals_all[:R] * (20.0 / rate)
     t0 = time.perf_counter()
     systems = synth.stream_requests(R)
     _log(f"{R} stream requests generated in {time.perf_counter() - t0:.1f} s")
     import tempfile
     with tempfile.TemporaryDirectory() as td:
         reqs = [(s, _constraints(s, td)) for s in systems]
-    chosen_prec = _abi.RP_PREC_OZAKI_REFINED if args.chosen == "refined" else _abi.RP_PREC_OZAKI
+    chosen_prec = _abi.RP_PREC_OZAKI_REFINED if chosen == "refined" else _abi.RP_PREC_OZAKI
     split = _abi.make_options(1, chosen_prec, 1.0, 1000.0, 20.0)
     f64 = _abi.make_options(1, _abi.RP_PREC_FP64, 1.0, 1000.0, 20.0)
-    srv = _abi.Server(0, args.slots, int(args.budget_gb * 1e9))
+    srv = _abi.Server(0, slots, int(budget_gb * 1e9))
     # warm-up: every slot sees the largest request once, so the timed stream allocates nothing
     big = max(range(R), key=lambda r: int(np.prod(systems[r].n_grid)) * systems[r].n_atoms)
-    tk = [srv.submit(_serve_args(reqs[big][0]), reqs[big][1], split) for _ in range(args.slots)]
+    tk = [srv.submit(_serve_args(reqs[big][0]), reqs[big][1], split) for _ in range(slots)]
     for t in tk:
         srv.poll(t, -1)
     _log("open-loop stream ...")
@@ -283,16 +286,16 @@ def serve_main(args):
     queue_wait = np.array([r["t_admit"] - r["arrival"] for _, r in res])
     n_atoms = np.array([s.n_atoms for s in systems])
     line = {
-        "metric": f"charge fits/s + p50/p99 latency, serving stream (Poisson {args.rate:g} req/s, 20-800 atoms)",
-        "value": round(R / span, 4), "unit": "fits/s", "n_gpus": 1, "steps": R, "warmup": args.slots,
+        "metric": f"charge fits/s + p50/p99 latency, serving stream (Poisson {rate:g} req/s, 20-800 atoms)",
+        "value": round(R / span, 4), "unit": "fits/s", "n_gpus": 1, "steps": R, "warmup": slots,
         "ms_per_step": round(1e3 * span / R, 3), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f64 basis/solve/refinement, i8 slices -> i32 exact Gram" if args.chosen == "refined" else
+        "dtype": "f64 basis/solve/refinement, i8 slices -> i32 exact Gram" if chosen == "refined" else
                  "f64 basis/solve, i8 slices -> i32 exact Gram",
         "data": "synthetic stream (seeded Ewald ESP + 1e-5 noise, fp32 cubes)",
-        "config": {"workload": f"stream{R}", "requests": R, "rate_per_s": args.rate, "slots": args.slots,
-                   "hbm_budget_gb": args.budget_gb, "atoms_min": int(n_atoms.min()), "atoms_max": int(n_atoms.max()),
+        "config": {"workload": f"stream{R}", "requests": R, "rate_per_s": rate, "slots": slots,
+                   "hbm_budget_gb": budget_gb, "atoms_min": int(n_atoms.min()), "atoms_max": int(n_atoms.max()),
                    "grid_points_total": int(sum(int(np.prod(s.n_grid)) for s in systems)),
-                   "precision": "split (int8 Ozaki, 4 slices)" if args.chosen == "split" else
+                   "precision": "split (int8 Ozaki, 4 slices)" if chosen == "split" else
                    "refined (int8 Ozaki, 3 slices, fp64 residual refinement)"},
         "p50_ms": round(1e3 * float(np.median(lat)), 3), "p99_ms": round(1e3 * float(np.percentile(lat, 99)), 3),
         "service_p50_ms": round(1e3 * float(np.median(service)), 3),
@@ -302,7 +305,11 @@ def serve_main(args):
         "open_loop_max_running": int(st_open["max_running"]),
         "mix_vs_alone_max_dq_e": dq_alone, "chosen_vs_fp64_max_dq_e": dq_f64, "sample_requests": sample,
     }
-    print(json.dumps(line))
+    return line
+
+
+def serve_main(args):
+    print(json.dumps(serve_run(args.requests, args.slots, args.rate, args.budget_gb, args.chosen)))
     return 0
 
 
@@ -317,6 +324,8 @@ def main():
                          "fp64-level charges with an error bound per fit)")
     ap.add_argument("--baseline-steps", type=int, default=None)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-serve", action="store_true",
+                    help="skip the serving stream (512 requests at 400 req/s, 8 slots) of the default line")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--serve", action="store_true", help="run the serving stream instead of the single-fit bench")
     ap.add_argument("--requests", type=int, default=512)
@@ -375,7 +384,8 @@ def main():
     M, N, K = split["M"], split["N"], split["K"]
     # dominant kernel of the chosen path: the basis (k_sep_phi, the most time per fit), bound by its fp64
     # vector arithmetic.  Its algorithmic fp64 flops per launch, from the factorized sum's units:
-    #   per (grid line, atom)   728: T2 for h = 0..6, 13 complex multiply-adds (8 flops) each
+    #   per (grid line, atom)   336: T2 for h = 0..6, folded by conjugate symmetry into T1[0] plus 6 pairs
+    #                                (P_k Re Y + Q_k Im Y): 24 multiply-adds (48 flops) per h
     #   per (kept point, atom)   86: Re sum_h T2 X^h by Horner (5 complex steps + the last, 44) and the
     #                                nearest-image test and displacement (42)
     #   per real-space term      47: erfc(sqrt(a) r)/r = 1/r - sqrt(a) P(u), P of degree 21 by Horner
@@ -386,7 +396,7 @@ def main():
     t_launch = ms / launches * 1e-3
     rows, lines = split["rows"], split["lines_per_chunk"]
     frac_in = min(1.0, 4.0 / 3.0 * np.pi * 20.0 ** 3 / split["volume"])
-    alg_fit = 728.0 * sum(lines) * N + 86.0 * M * N + 47.0 * frac_in * M * N
+    alg_fit = 336.0 * sum(lines) * N + 86.0 * M * N + 47.0 * frac_in * M * N
     alg_launch = alg_fit / launches_per_fit
     achieved = alg_launch / t_launch / 1e12
     executed = flops_from_profile("k_sep_phi", args.workload)
@@ -403,7 +413,7 @@ def main():
             "hbm_gbs": round(traffic / t_launch / 1e9, 1) if traffic else None,
             "hbm_frac": round(traffic / t_launch / 8e12, 4) if traffic else None,
             "fk_equivalent_tflops": round(4.0 * M * N * K / launches_per_fit / t_launch / 1e12, 1),
-            "note": "achieved = the factorized sum's algorithmic fp64 flops of one launch (728 per line and atom, "
+            "note": "achieved = the factorized sum's algorithmic fp64 flops of one launch (336 per line and atom, "
                     "86 per kept point and atom, 47 per real-space term) / mean launch time (HIP events on the "
                     "fit's stream); executed = rocprof SQ_INSTS_VALU_{FMA,MUL,ADD,TRANS}_F64 x 64 lanes (idle "
                     "lanes included); traffic = calibrated FETCH_SIZE + WRITE_SIZE; fk_equivalent counts "
@@ -432,6 +442,14 @@ def main():
     if not args.no_cpu:
         _log("cpu baseline (oracle sample) ...")
         cpu = cpu_baseline(system, con, args.cpu_seconds)
+    serving = None
+    if not args.no_serve:
+        _log("serving stream (512 requests, 400 req/s, 8 slots) ...")
+        sv = serve_run(512, 8, 400.0, args.budget_gb, args.chosen)
+        serving = {k: sv[k] for k in ("metric", "value", "unit", "p50_ms", "p99_ms", "service_p50_ms",
+                                      "queue_wait_p99_ms", "capacity_fits_per_s", "max_running",
+                                      "peak_reserved_gb", "mix_vs_alone_max_dq_e", "chosen_vs_fp64_max_dq_e")}
+        serving["config"] = sv["config"]
     def per_fit(run):
         nf = run["breakdown_fits"]
         return {k: {"ms_per_fit": round(v[0] / nf, 4), "launches_per_fit": v[1] / nf}
@@ -473,6 +491,7 @@ def main():
         "roofline_gram": roof_gram,
         "cpu_baseline": cpu,
         "speedup_vs_cpu": round(fits_s / cpu["value"], 1) if cpu else None,
+        "serving": serving,
     }
     print(json.dumps(line))
     return 0
