@@ -196,6 +196,24 @@ int prepare_device(rp_fit *fit, FitDevice &dv, int precision) {
         }
         std::copy(B, B + 9, dv.epi.box);
         for (int i = 0; i < 3; ++i) dv.epi.nmax[i] = s.nmax[i];
+        // the cell's metric in triangular form, for the sphere enumeration of the multi-image sum
+        {
+            double G[3][3];
+            for (int i = 0; i < 3; ++i)
+                for (int j = 0; j < 3; ++j)
+                    G[i][j] = B[3 * i] * B[3 * j] + B[3 * i + 1] * B[3 * j + 1] + B[3 * i + 2] * B[3 * j + 2];
+            EpiArgs &e = dv.epi;
+            e.q00 = G[0][0];
+            e.a01 = G[0][1] / G[0][0];
+            e.a02 = G[0][2] / G[0][0];
+            e.q11 = G[1][1] - G[0][1] * e.a01;
+            e.a12 = (G[1][2] - G[0][2] * e.a01) / e.q11;
+            e.q22 = G[2][2] - e.q00 * e.a02 * e.a02 - e.q11 * e.a12 * e.a12;
+            if (!(e.q00 > 0.0 && e.q11 > 0.0 && e.q22 > 0.0)) return ctx->fail(RP_EINVAL, "degenerate cell");
+            e.inv_q00 = 1.0 / e.q00;
+            e.inv_q11 = 1.0 / e.q11;
+            e.reach2 = s.r_cutoff / std::sqrt(e.q22) + 1e-9;
+        }
         dv.epi.inv_n[0] = 1.0 / s.nx;
         dv.epi.inv_n[1] = 1.0 / s.ny;
         dv.epi.inv_n[2] = 1.0 / s.nz;

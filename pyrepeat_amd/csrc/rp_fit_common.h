@@ -46,6 +46,11 @@ struct EpiArgs {
     double h[3];           // interplanar spacings V / |a_j x a_k|, shrunk by a relative 1e-12
     double inv_n[3];       // 1 / n_grid
     int nmax[3];           // extent of the full shift list, |n_i| <= nmax[i] (repeat.py:815-822)
+    // |w . cell|^2 = q22 w2^2 + q11 (w1 + a12 w2)^2 + q00 (w0 + a01 w1 + a02 w2)^2 for fractional w: the
+    // cell's metric G = B B^T in triangular form (q00 = G00, a0j = G0j / G00, q11 = G11 - G01 a01,
+    // a12 = (G12 - G02 a01) / q11, q22 = G22 - q00 a02^2 - q11 a12^2), inv_q00 = 1 / q00,
+    // inv_q11 = 1 / q11, reach2 = Rc / sqrt(q22) + 1e-9.
+    double q00, q11, q22, a01, a02, a12, inv_q00, inv_q11, reach2;
 
     // The shift-list sum for one (point, atom) pair, visiting only the shifts that can lie within Rc:
     // image n is at distance >= |d_i - n_i| h_i from the point along plane family i (d = g_f - a_f, the
@@ -79,6 +84,51 @@ struct EpiArgs {
                         sum += term(r2);
                     }
                 }
+        return sum;
+    }
+
+    // The same sum over the lattice points of the cutoff sphere only: with w = d - n the metric form
+    // above bounds n2 by q22 w2^2 <= Rc^2, then n1 for each n2 and n0 for each (n2, n1) by what is left
+    // of Rc^2 (the enumeration of Fincke and Pohst), each range widened by 1e-9 and clipped to the list's
+    // extent.  Every candidate is tested exactly as in the list, its shift vector rounded as the list's,
+    // so the terms are those of the list; they are added n2-major, so the sum agrees with the list's
+    // to rounding.  For a wave whose lanes hold neighbouring points of one atom the ranges nearly agree
+    // across lanes; for lanes of unrelated pairs the box above diverges less.
+    template <class Term>
+    __device__ __forceinline__ double image_sum_sphere(double d0, double d1, double d2, double gx, double gy,
+                                                       double gz, double ax, double ay, double az, Term term) const {
+        // sqrt(x) for x >= 0 within a relative 1e-6 (the v_rsq estimate is good to about 2^-23), widened:
+        // only the ranges use it
+        auto root_up = [](double x) { return x > 0.0 ? x * __builtin_amdgcn_rsq(x) * (1.0 + 1e-6) + 1e-9 : 1e-9; };
+        const double R2s = R2 * (1.0 + 1e-12);
+        double sum = 0.0;
+        const int lo2 = max(-nmax[2], (int)ceil(d2 - reach2)), hi2 = min(nmax[2], (int)floor(d2 + reach2));
+        for (int n2 = lo2; n2 <= hi2; ++n2) {
+            const double w2 = d2 - (double)n2;
+            const double rem2 = R2s - q22 * w2 * w2;
+            if (rem2 < 0.0) continue;
+            const double c1 = d1 + a12 * w2, s1 = root_up(rem2 * inv_q11);
+            const int lo1 = max(-nmax[1], (int)ceil(c1 - s1)), hi1 = min(nmax[1], (int)floor(c1 + s1));
+            for (int n1 = lo1; n1 <= hi1; ++n1) {
+                const double w1 = d1 - (double)n1;
+                const double t1 = w1 + a12 * w2;
+                const double rem1 = rem2 - q11 * t1 * t1;
+                if (rem1 < 0.0) continue;
+                const double c0 = d0 + a01 * w1 + a02 * w2, s0 = root_up(rem1 * inv_q00);
+                const int lo0 = max(-nmax[0], (int)ceil(c0 - s0)), hi0 = min(nmax[0], (int)floor(c0 + s0));
+                for (int n0 = lo0; n0 <= hi0; ++n0) {
+                    const double sx = ((double)n0 * box[0] + (double)n1 * box[3]) + (double)n2 * box[6];
+                    const double sy = ((double)n0 * box[1] + (double)n1 * box[4]) + (double)n2 * box[7];
+                    const double sz = ((double)n0 * box[2] + (double)n1 * box[5]) + (double)n2 * box[8];
+                    const double dx = gx - (ax + sx), dy = gy - (ay + sy), dz = gz - (az + sz);
+                    const double r2 = (dx * dx + dy * dy) + dz * dz;
+                    if (r2 <= R2) {
+                        if (sx == 0.0 && sy == 0.0 && sz == 0.0 && r2 <= 1e-20) continue;
+                        sum += term(r2);
+                    }
+                }
+            }
+        }
         return sum;
     }
 
@@ -166,7 +216,7 @@ struct FitDevice {
     std::vector<int2> tiles;
     // split form: factorized reciprocal sum (k-index box, per-plane partial sums) and the
     // table of grid lines holding kept points
-    DBuf sep_kbox, sep_t1, line_row, line_key, line_cnt, line_off, chunk_lines;
+    DBuf sep_kbox, sep_t1, line_row, line_key, line_cnt, line_off, chunk_lines, multi_flag;
     bool sep_kbox_ok = false;   // sep_kbox holds the index box of the current k-set
     SepPlan sep_plan;           // chunk line ranges of the last split accumulate (the streamed refinement's)
     hipEvent_t ev_plan = nullptr;   // marks the chunk line ranges on the stream

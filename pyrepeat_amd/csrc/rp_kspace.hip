@@ -244,6 +244,24 @@ __global__ __launch_bounds__(256) void k_chunk_lines(const int *__restrict__ lin
     out[2 * c + 1] = (int)line_of(r1) + 1;
 }
 
+// Fractional displacement d = g_f - a_f of a grid point (i, j, m) from an atom, its nearest image
+// n = rint(d) and w = d - n; true when only that image can be within Rc, i.e. the bound
+// min_i (1 - |w_i|) h_i on the distance to every other image exceeds Rc.  Written with explicit fma so
+// that the basis kernel and the multi-image kernel classify every (point, atom) pair alike.
+__device__ __forceinline__ bool nearest_only(const EpiArgs &e, double id, double jd, double md, double Fx, double Fy,
+                                             double Fz, double &d0, double &d1, double &d2, double &n0, double &n1,
+                                             double &n2) {
+    d0 = fma(id, e.inv_n[0], -Fx);
+    d1 = fma(jd, e.inv_n[1], -Fy);
+    d2 = fma(md, e.inv_n[2], -Fz);
+    n0 = rint(d0);
+    n1 = rint(d1);
+    n2 = rint(d2);
+    const double b0 = (1.0 - fabs(d0 - n0)) * e.h[0], b1 = (1.0 - fabs(d1 - n1)) * e.h[1],
+                 b2 = (1.0 - fabs(d2 - n2)) * e.h[2];
+    return fmin(fmin(b0, b1), b2) > e.r_cut;
+}
+
 struct SepArgs {
     const int *line_row, *line_key;
     int l_first, l_end;
@@ -255,6 +273,7 @@ struct SepArgs {
     double *phi;                // [NP][rows]: column-major, chunk-local rows
     int64_t rows;               // leading dimension of phi
     unsigned long long *colmax;
+    int *multi_flag;            // set when a pair of the chunk has several images within Rc
     int items, items_per_xcd;   // line groups of the chunk; line groups per XCD
     const int *gate, *gate_flags;   // nullable: run only while gate[0] != 0 and both flags are 0
 };
@@ -338,6 +357,7 @@ __global__ __launch_bounds__(SEP_THREADS) void k_sep_phi(SepArgs p) {
     const int64_t rs = max<int64_t>(lrow[0], p.r0);
     const int64_t rend = min<int64_t>(lrow[n_l], p.r0 + p.valid);
     double cmax = 0.0;
+    bool multi = false;   // a pair of this lane has several images within Rc
     int li = 0, base = lkey[0] * nx, j = lj[0], m = lm[0];
     int64_t r = rs + s;
     const int *idx = e.idx + r;
@@ -353,6 +373,7 @@ __global__ __launch_bounds__(SEP_THREADS) void k_sep_phi(SepArgs p) {
         const int i = f_next - base;
         if (r + SH < rend) f_next = idx[SH];
         double val = 0.0;
+        bool mine = true;   // val is the pair's final value (not left to the multi-image kernel)
         if (atom) {
             double acc = 0.0;
             if (recip) {
@@ -378,15 +399,10 @@ __global__ __launch_bounds__(SEP_THREADS) void k_sep_phi(SepArgs p) {
                 const double d = sqrt((dx * dx + dy * dy) + dz * dz);
                 real = d > 1e-10 ? 1.0 / d : 0.0;
             } else {
-                // nearest image n = rint(g_f - a_f); only it can be within Rc when the distance bound
-                // min_i (1 - |w_i|) h_i to every other image exceeds Rc (w = g_f - a_f - n)
-                const double d0 = id * e.inv_n[0] - Fx, d1 = jd * e.inv_n[1] - Fy, d2 = md * e.inv_n[2] - Fz;
-                const double n0 = rint(d0), n1 = rint(d1), n2 = rint(d2);
-                const double w0 = d0 - n0, w1 = d1 - n1, w2 = d2 - n2;
-                const double bnd = fmin(fmin((1.0 - fabs(w0)) * e.h[0], (1.0 - fabs(w1)) * e.h[1]),
-                                        (1.0 - fabs(w2)) * e.h[2]);
-                if (bnd > e.r_cut) {
-                    // displacement to that image, w . cell
+                double d0, d1, d2, n0, n1, n2;
+                if (nearest_only(e, id, jd, md, Fx, Fy, Fz, d0, d1, d2, n0, n1, n2)) {
+                    // only the nearest image can be within Rc: displacement w . cell to it
+                    const double w0 = d0 - n0, w1 = d1 - n1, w2 = d2 - n2;
                     const double dx = fma(w0, e.box[0], fma(w1, e.box[3], w2 * e.box[6]));
                     const double dy = fma(w0, e.box[1], fma(w1, e.box[4], w2 * e.box[7]));
                     const double dz = fma(w0, e.box[2], fma(w1, e.box[5], w2 * e.box[8]));
@@ -394,14 +410,9 @@ __global__ __launch_bounds__(SEP_THREADS) void k_sep_phi(SepArgs p) {
                     if (r2 <= e.R2 && !(n0 == 0.0 && n1 == 0.0 && n2 == 0.0 && r2 <= 1e-20))
                         real = erfc_over_r(r2, alpha, e.sqrt_alpha);
                 } else {
-                    // several images may lie within Rc: the shift-list sum over the candidate shifts only,
-                    // with the reference roundings and order (EpiArgs::image_sum)
-                    const double gx = (id * e.geom.ax[0] + jd * e.geom.ax[3]) + md * e.geom.ax[6];
-                    const double gy = (id * e.geom.ax[1] + jd * e.geom.ax[4]) + md * e.geom.ax[7];
-                    const double gz = (id * e.geom.ax[2] + jd * e.geom.ax[5]) + md * e.geom.ax[8];
-                    const double sa = e.sqrt_alpha;
-                    real = e.image_sum(d0, d1, d2, gx, gy, gz, Rx, Ry, Rz,
-                                       [alpha, sa](double r2) { return erfc_over_r(r2, alpha, sa); });
+                    // several images may lie within Rc: k_sep_images adds that sum afterwards
+                    mine = false;
+                    multi = true;
                 }
             }
             val = acc + real;
@@ -411,14 +422,84 @@ __global__ __launch_bounds__(SEP_THREADS) void k_sep_phi(SepArgs p) {
             val = 1.0;
         }
         *out = val;
-        cmax = fmax(cmax, fabs(val));
+        if (mine) cmax = fmax(cmax, fabs(val));
     }
+    if (__ballot(multi) != 0 && lane == 0) *p.multi_flag = 1;
     // column maximum over the 8 point slots of the atom
     cmax = fmax(cmax, __shfl_xor(cmax, 1, 64));
     cmax = fmax(cmax, __shfl_xor(cmax, 2, 64));
     cmax = fmax(cmax, __shfl_xor(cmax, 4, 64));
     if (s == 0 && cmax > 0.0 && a < p.NP) atomicMax(p.colmax + a, (unsigned long long)__double_as_longlong(cmax));
 }
+
+// The real-space sum of the (point, atom) pairs with several images within Rc, added to their Phi
+// entries after k_sep_phi (which leaves those sums out and sets *flag).  Work-group = 64 consecutive kept
+// points (lane = point) x 32 atoms, 8 per wave taken one after another: a wave's lanes hold neighbouring
+// points of one atom, so their image ranges (EpiArgs::image_sum_sphere) nearly agree.  The column maxima
+// take the completed entries.
+struct ImageArgs {
+    EpiArgs e;
+    int64_t r0, valid, rows;
+    int N;
+    double *phi;
+    unsigned long long *colmax;
+    const int *flag;
+    const int *gate, *gate_flags;
+};
+__global__ __launch_bounds__(256) void k_sep_images(ImageArgs p) {
+    __shared__ double cmx[32][65];   // per atom of the work-group and lane, the largest completed |entry|
+    if (p.flag[0] == 0) return;
+    if (p.gate && (p.gate[0] == 0 || p.gate_flags[0] != 0 || p.gate_flags[1] != 0)) return;
+    const EpiArgs &e = p.e;
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    const int64_t rl = (int64_t)blockIdx.x * 64 + lane;   // chunk-local row
+    const bool row = rl < p.valid;
+    int i = 0, j = 0, m = 0;
+    if (row) e.geom.ijk(e.idx[p.r0 + rl], i, j, m);
+    const double id = (double)i, jd = (double)j, md = (double)m;
+    const double gx = (id * e.geom.ax[0] + jd * e.geom.ax[3]) + md * e.geom.ax[6];
+    const double gy = (id * e.geom.ax[1] + jd * e.geom.ax[4]) + md * e.geom.ax[7];
+    const double gz = (id * e.geom.ax[2] + jd * e.geom.ax[5]) + md * e.geom.ax[8];
+    const double alpha = e.sqrt_alpha * e.sqrt_alpha, sa = e.sqrt_alpha;
+    const int a0 = (blockIdx.y * 4 + wave) * 8;
+#pragma unroll 1
+    for (int q = 0; q < 8; ++q) {
+        const int a = a0 + q;   // uniform
+        if (a >= p.N) {
+            cmx[wave * 8 + q][lane] = 0.0;
+            continue;
+        }
+        double *o = p.phi + (int64_t)a * p.rows + rl;
+        const double before = row ? *o : 0.0;   // in flight while the sum runs
+        const double Fx = e.afrac[3 * a], Fy = e.afrac[3 * a + 1], Fz = e.afrac[3 * a + 2];
+        double d0, d1, d2, n0, n1, n2;
+        const bool multi = row && !nearest_only(e, id, jd, md, Fx, Fy, Fz, d0, d1, d2, n0, n1, n2);
+        double cm = 0.0;
+        if (multi) {
+            const double real = e.image_sum_sphere(d0, d1, d2, gx, gy, gz, e.atoms[3 * a], e.atoms[3 * a + 1],
+                                                   e.atoms[3 * a + 2],
+                                                   [alpha, sa](double r2) { return erfc_over_r(r2, alpha, sa); });
+            const double v = before + real;
+            *o = v;
+            cm = fabs(v);
+        }
+        cmx[wave * 8 + q][lane] = cm;
+    }
+    // column maxima: thread (atom t / 8, part t % 8) takes 8 lanes' values, then 3 exchanges
+    __syncthreads();
+    const int t = threadIdx.x, ca = t >> 3, part = t & 7;
+    double cm = 0.0;
+#pragma unroll
+    for (int u = 0; u < 8; ++u) cm = fmax(cm, cmx[ca][part * 8 + u]);
+    cm = fmax(cm, __shfl_xor(cm, 1, 64));
+    cm = fmax(cm, __shfl_xor(cm, 2, 64));
+    cm = fmax(cm, __shfl_xor(cm, 4, 64));
+    const int a = blockIdx.y * 32 + ca;
+    if (part == 0 && cm > 0.0 && a < p.N) atomicMax(p.colmax + a, (unsigned long long)__double_as_longlong(cm));
+}
+
+__global__ void k_zero_flag(int *f) { *f = 0; }
 
 // Column-major Phi -> slices [P][NP][rows] of x / S_col, S_col the power of two with
 // max |Phi_col| / S_col <= 127/128; rows at or past `valid` are zero.  Both sides are streams: a
@@ -502,6 +583,7 @@ int sep_prepare(rp_fit *fit, FitDevice &dv, int64_t rows, SepPlan &plan) {
         }
         RP_CHECK(dv.sep_t1.grow(ctx, sizeof(double2) * (size_t)s.nz * N * SBOX));
     }
+    RP_CHECK(dv.multi_flag.grow(ctx, sizeof(int)));
     // line table of the kept points
     const int64_t nb = ceil_div(M, 256);
     RP_CHECK(dv.line_cnt.grow(ctx, sizeof(int) * nb));
@@ -567,7 +649,25 @@ void sep_phi_chunk(rp_fit *fit, FitDevice &dv, const SepPlan &plan, int chunk, i
     a.items = (int)ceil_div(a.l_end - a.l_first, SEP_LINES);
     a.items_per_xcd = (int)ceil_div(a.items, 8);
     const int blocks = 8 * a.items_per_xcd * (dv.NP / SEP_ATOMS);
+    const bool images = dv.epi.fit_type != 0;
+    if (images) k_zero_flag<<<1, 1, 0, st>>>(dv.multi_flag.as<int>());
+    a.multi_flag = dv.multi_flag.as<int>();
     k_sep_phi<<<blocks, SEP_THREADS, dv.K > 0 ? sizeof(double2) * fit->setup.nx : 0, st>>>(a);
+    if (images) {
+        ImageArgs b{};
+        b.e = dv.epi;
+        b.r0 = r0;
+        b.valid = valid;
+        b.rows = rows;
+        b.N = fit->n_atoms;
+        b.phi = phi;
+        b.colmax = colmax;
+        b.flag = dv.multi_flag.as<int>();
+        b.gate = gate;
+        b.gate_flags = gate_flags;
+        dim3 g((unsigned)ceil_div(valid, 64), (unsigned)ceil_div(fit->n_atoms, 32));
+        k_sep_images<<<g, 256, 0, st>>>(b);
+    }
 }
 
 void sep_slice_chunk(const double *phi, int NP, int64_t rows, int64_t valid, const unsigned long long *colmax,
