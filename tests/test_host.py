@@ -197,3 +197,59 @@ def test_erf_polynomial_matches_library_erfc():
         r = np.linspace(0.5, rc, 5001)
         got = 1.0 / r - sa * P(sa * sa * r * r)
         assert np.max(np.abs(got - erfc(sa * r) / r)) < 4e-15
+
+
+def test_sphere_enumeration_of_the_image_sum_covers_the_cutoff():
+    """The lattice-point ranges of the multi-image sum (rp_fit_common.h, EpiArgs::image_sum_sphere: n2,
+    then n1 given n2, then n0 given both, from the cell metric in triangular form) hold every shift
+    within Rc of the (point, atom) pair, on skewed cells and cutoffs from half a cell to several cells.
+    Brute force over the shift list's box is the reference; a sign or index slip in the metric form
+    drops points here."""
+    rng = np.random.default_rng(17)
+    for trial in range(60):
+        a, b, c = rng.uniform(6.0, 14.0, 3)
+        al, be, ga = rng.uniform(65.0, 115.0, 3)
+        B = synth.triclinic_box(a, b, c, al, be, ga)
+        rc = rng.uniform(0.4, 2.5) * min(a, b, c)
+        G = B @ B.T
+        q00 = G[0, 0]
+        a01, a02 = G[0, 1] / q00, G[0, 2] / q00
+        q11 = G[1, 1] - G[0, 1] * a01
+        a12 = (G[1, 2] - G[0, 2] * a01) / q11
+        q22 = G[2, 2] - q00 * a02 ** 2 - q11 * a12 ** 2
+        assert min(q00, q11, q22) > 0
+        w = rng.uniform(-2, 2, 3)
+        form = q22 * w[2] ** 2 + q11 * (w[1] + a12 * w[2]) ** 2 + q00 * (w[0] + a01 * w[1] + a02 * w[2]) ** 2
+        assert abs(form - np.sum((w @ B) ** 2)) < 1e-9 * form
+        nmax = [int(np.floor(rc / np.linalg.norm(B[i]))) + 1 for i in range(3)]
+        R2 = rc * rc
+
+        def root_up(x):
+            return np.sqrt(x) * (1 + 1e-6) + 1e-9 if x > 0 else 1e-9
+        reach2 = rc / np.sqrt(q22) + 1e-9
+        for _ in range(20):
+            d = rng.uniform(-1.0, 1.0, 3)
+            brute = set()
+            for n0 in range(-nmax[0], nmax[0] + 1):
+                for n1 in range(-nmax[1], nmax[1] + 1):
+                    for n2 in range(-nmax[2], nmax[2] + 1):
+                        if np.sum(((d - (n0, n1, n2)) @ B) ** 2) <= R2:
+                            brute.add((n0, n1, n2))
+            got = set()
+            for n2 in range(max(-nmax[2], int(np.ceil(d[2] - reach2))), min(nmax[2], int(np.floor(d[2] + reach2))) + 1):
+                w2 = d[2] - n2
+                rem2 = R2 * (1 + 1e-12) - q22 * w2 * w2
+                if rem2 < 0:
+                    continue
+                c1, s1 = d[1] + a12 * w2, root_up(rem2 / q11)
+                for n1 in range(max(-nmax[1], int(np.ceil(c1 - s1))), min(nmax[1], int(np.floor(c1 + s1))) + 1):
+                    w1 = d[1] - n1
+                    rem1 = rem2 - q11 * (w1 + a12 * w2) ** 2
+                    if rem1 < 0:
+                        continue
+                    c0, s0 = d[0] + a01 * w1 + a02 * w2, root_up(rem1 / q00)
+                    for n0 in range(max(-nmax[0], int(np.ceil(c0 - s0))), min(nmax[0], int(np.floor(c0 + s0))) + 1):
+                        got.add((n0, n1, n2))
+            assert brute <= got, (trial, brute - got)
+            # the ranges follow the sphere, not the whole box of candidates
+            assert len(got) <= 2 * len(brute) + 12
