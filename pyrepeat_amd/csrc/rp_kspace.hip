@@ -21,7 +21,10 @@
 // run of whole or partial grid lines): one work-group takes up to SEP_LINES consecutive lines and
 // 32 atoms; each wave forms T2 of those lines for 8 atoms (one lane per (atom, h)), then evaluates
 // the points (one lane per (atom, point slot)) as a Horner sum in X(1, i), adds the real-space term
-// (fp64, erfc as one polynomial on the argument range [0, pi]) and writes Phi row-major.
+// where only the nearest image can be within the cutoff (fp64, erfc as one polynomial on the argument
+// range [0, pi]) and writes Phi column-major.  The pairs with several images within the cutoff (cells
+// narrower than twice the cutoff) get their real-space sum from k_sep_images afterwards, whose waves
+// take one atom at 64 neighbouring points so that the image ranges agree across lanes.
 #include "rp_fit_common.h"
 #include "rp_erf_poly.h"
 
