@@ -317,3 +317,34 @@ def test_refined_gram_close_to_fp64(gpu_ctx):
     np.testing.assert_allclose(s3, s64, rtol=2.6e-4)
     # and it differs from fp64 at all (the slices, not a silent fp64 copy)
     assert np.max(np.abs(a3 - a64)) > 0.0
+
+
+@pytest.mark.parametrize("triclinic", [False, True])
+def test_multi_image_basis_matches_fp64_on_mixed_cells(gpu_ctx, tmp_path, triclinic):
+    """A 30 Bohr cell against Rc = 20 Bohr: about a third of the (point, atom) pairs have only their
+    nearest image within the cutoff (the basis kernel's term) and the rest several (k_sep_images adds
+    their sum over the cutoff sphere's lattice points).  The refined form's kept basis times a charge
+    vector equals the fp64 path's, whose GEMM epilogue sums the images over the shift list's box, to
+    fp64 rounding; a pair counted by both kernels or by neither would show at once."""
+    s = synth.tiny(seed=31, n_atoms=12, n=48, L=30.0, triclinic=triclinic)
+    # both kinds of pairs are present: the nearest-image bound over the kept points and atoms
+    c = ChargeCalculator(cube=cube_from_system(s), precision="refined", context=gpu_ctx)
+    xyz, _, _ = c._fit.grid()
+    box = s.axis * np.asarray(s.n_grid)[:, None]
+    inv = np.linalg.inv(box)
+    vol = abs(np.linalg.det(box))
+    h = np.array([vol / np.linalg.norm(np.cross(box[(i + 1) % 3], box[(i + 2) % 3])) for i in range(3)])
+    d = (xyz[::7, None, :] - s.atom_pos[None, :, :]) @ inv
+    w = d - np.rint(d)
+    near_only = np.min((1.0 - np.abs(w)) * h, axis=2) > 20.0
+    print(f"mixed cell (triclinic={triclinic}): {near_only.mean():.2f} of the pairs near-only", flush=True)
+    assert 0.05 < near_only.mean() < 0.95
+    q = c.fit_charges()
+    v_ref = c._fit.phi_matvec(q)
+    c.close()
+    c64 = ChargeCalculator(cube=cube_from_system(s), precision="fp64", context=gpu_ctx)
+    v64 = c64._fit.phi_matvec(q)
+    q64 = c64.fit_charges()
+    c64.close()
+    np.testing.assert_allclose(v_ref, v64, rtol=1e-11, atol=1e-13 * np.abs(v64).max())
+    assert np.max(np.abs(q - q64)) < 1e-8
