@@ -440,11 +440,15 @@ def main():
             "hbm_gbs": round(traffic / t_launch / 1e9, 1) if traffic else None,
             "hbm_frac": round(traffic / t_launch / 8e12, 4) if traffic else None,
             "fk_equivalent_tflops": round(4.0 * M * N * K / launches_per_fit / t_launch / 1e12, 1),
+            "erfc_terms_per_launch": round(frac_in * M * N / launches_per_fit),
+            "erfc_terms_per_s": round(frac_in * M * N / launches_per_fit / t_launch, -6),
             "note": "achieved = the factorized sum's algorithmic fp64 flops of one launch (336 per line and atom, "
                     "86 per kept point and atom, 47 per real-space term) / mean launch time (HIP events on the "
                     "fit's stream); executed = rocprof SQ_INSTS_VALU_{FMA,MUL,ADD,TRANS}_F64 x 64 lanes (idle "
                     "lanes included); traffic = calibrated FETCH_SIZE + WRITE_SIZE; fk_equivalent counts "
-                    "4 M N K, the reference's cos/sin GEMM formulation, which this kernel does not execute"}
+                    "4 M N K, the reference's cos/sin GEMM formulation, which this kernel does not execute; "
+                    "erfc_terms = the real-space terms inside the cutoff (the sphere's share of the cell), each an "
+                    "fp64 polynomial erfc, per launch and per second of the launch"}
     # the int8 Gram (MFMA), from the per-kernel breakdown fits after the timed region
     gms, glaunch = split["kernels"]["gram"]
     g_launch = gms / glaunch * 1e-3
