@@ -216,7 +216,7 @@ struct FitDevice {
     std::vector<int2> tiles;
     // split form: factorized reciprocal sum (k-index box, per-plane partial sums) and the
     // table of grid lines holding kept points
-    DBuf sep_kbox, sep_t1, line_row, line_key, line_cnt, line_off, chunk_lines, multi_flag;
+    DBuf sep_kbox, sep_t1, sep_coef, line_row, line_key, line_cnt, line_off, chunk_lines, multi_flag;
     bool sep_kbox_ok = false;   // sep_kbox holds the index box of the current k-set
     SepPlan sep_plan;           // chunk line ranges of the last split accumulate (the streamed refinement's)
     hipEvent_t ev_plan = nullptr;   // marks the chunk line ranges on the stream

@@ -94,22 +94,29 @@ constexpr int SEP_XMAX = 2048;  // longest axis 0 whose phase row X(1, .) the ba
 // l, k >= 0 are read; the table's row of -l equals the conjugate of the row of l to an ulp.
 constexpr int KF = SK / 2;   // folded pairs per row
 
-// c_a(h, k, l) folded over l into f[0] = c(0), f[l] = P(l), f[KF + l] = Q(l) for one (h, k).
-__device__ __forceinline__ void t1_fold_coef(const int *__restrict__ kbox_row, const double *__restrict__ kvecs,
-                                             const double *__restrict__ kcoef, double ax, double ay, double az,
+// c_a(p) = w_p e^{-i k_p . R_a} for every atom and k-vector, once per fit: [atom][p].
+__global__ __launch_bounds__(256) void k_sep_coef(int N, int K, const double *__restrict__ kvecs,
+                                                  const double *__restrict__ kcoef, const double *__restrict__ atoms,
+                                                  double2 *__restrict__ C) {
+    const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (t >= (int64_t)N * K) return;
+    const int a = (int)(t / K), p = (int)(t - (int64_t)a * K);
+    const double ax = atoms[3 * a], ay = atoms[3 * a + 1], az = atoms[3 * a + 2];
+    // the angle with the rounding of the atom tables: (kx x + ky y) + kz z
+    const double th = (kvecs[3 * p] * ax + kvecs[3 * p + 1] * ay) + kvecs[3 * p + 2] * az;
+    double s, co;
+    sincos(th, &s, &co);
+    const double w = kcoef[p];
+    C[t] = make_double2(w * co, -(w * s));
+}
+
+// c_a(h, k, l) folded over l into f[0] = c(0), f[l] = P(l), f[KF + l] = Q(l) for one (h, k), from the
+// atom's row of k_sep_coef's table.
+__device__ __forceinline__ void t1_fold_coef(const int *__restrict__ kbox_row, const double2 *__restrict__ crow,
                                              double2 (&f)[SK]) {
     auto coef = [&](int ll) {
         const int p = kbox_row[ll];
-        double2 c = make_double2(0.0, 0.0);
-        if (p >= 0) {
-            // the angle with the rounding of the atom tables: (kx x + ky y) + kz z
-            const double th = (kvecs[3 * p] * ax + kvecs[3 * p + 1] * ay) + kvecs[3 * p + 2] * az;
-            double s, co;
-            sincos(th, &s, &co);
-            const double w = kcoef[p];
-            c = make_double2(w * co, -(w * s));
-        }
-        return c;
+        return p >= 0 ? crow[p] : make_double2(0.0, 0.0);
     };
     f[0] = coef(SOFF);
 #pragma unroll
@@ -133,17 +140,16 @@ __device__ __forceinline__ double2 t1_fold_sum(const double2 (&f)[SK], const dou
 
 // Folded T1[m][a][h][.] for planes [blockIdx.y * T1_PLANES, +T1_PLANES); block = atom, lane = (h, k >= 0).
 constexpr int T1_LANES = SH * (KF + 1);
-__global__ __launch_bounds__(64) void k_sep_t1(int N, int nz, const int *__restrict__ kbox,
-                                               const double *__restrict__ kvecs, const double *__restrict__ kcoef,
-                                               const double *__restrict__ atoms, const double2 *__restrict__ ez,
+__global__ __launch_bounds__(64) void k_sep_t1(int N, int K, int nz, const int *__restrict__ kbox,
+                                               const double2 *__restrict__ C, const double2 *__restrict__ ez,
                                                double2 *__restrict__ T1) {
     const int a = blockIdx.x, t = threadIdx.x;
     if (t >= T1_LANES) return;
     const int h = t / (KF + 1), k = t - h * (KF + 1);
-    const double ax = atoms[3 * a], ay = atoms[3 * a + 1], az = atoms[3 * a + 2];
+    const double2 *crow = C + (int64_t)a * K;
     double2 fp[SK], fm[SK];
-    t1_fold_coef(kbox + (h * SK + SOFF + k) * SK, kvecs, kcoef, ax, ay, az, fp);
-    if (k > 0) t1_fold_coef(kbox + (h * SK + SOFF - k) * SK, kvecs, kcoef, ax, ay, az, fm);
+    t1_fold_coef(kbox + (h * SK + SOFF + k) * SK, crow, fp);
+    if (k > 0) t1_fold_coef(kbox + (h * SK + SOFF - k) * SK, crow, fm);
     const int m0 = blockIdx.y * T1_PLANES, m1 = min(m0 + T1_PLANES, nz);
     for (int m = m0; m < m1; ++m) {
         double2 *row = T1 + ((int64_t)m * N + a) * SBOX + h * SK;
@@ -585,6 +591,7 @@ int sep_prepare(rp_fit *fit, FitDevice &dv, int64_t rows, SepPlan &plan) {
             dv.sep_kbox_ok = true;
         }
         RP_CHECK(dv.sep_t1.grow(ctx, sizeof(double2) * (size_t)s.nz * N * SBOX));
+        RP_CHECK(dv.sep_coef.grow(ctx, sizeof(double2) * (size_t)N * K));
     }
     RP_CHECK(dv.multi_flag.grow(ctx, sizeof(int)));
     // line table of the kept points
@@ -612,9 +619,11 @@ int sep_prepare(rp_fit *fit, FitDevice &dv, int64_t rows, SepPlan &plan) {
     if (!dv.ev_plan) RP_HIP(ctx, hipEventCreateWithFlags(&dv.ev_plan, hipEventDisableTiming));
     RP_HIP(ctx, hipEventRecord(dv.ev_plan, st));
     if (K > 0) {
+        k_sep_coef<<<ceil_div((int64_t)N * K, 256), 256, 0, st>>>(N, K, dv.kvecs.as<double>(), dv.kcoef.as<double>(),
+                                                                 fit->d_atoms.as<double>(), dv.sep_coef.as<double2>());
         dim3 g(N, (unsigned)ceil_div(s.nz, T1_PLANES));
-        k_sep_t1<<<g, 64, 0, st>>>(N, s.nz, dv.sep_kbox.as<int>(), dv.kvecs.as<double>(), dv.kcoef.as<double>(),
-                                    fit->d_atoms.as<double>(), dv.ez.as<double2>(), dv.sep_t1.as<double2>());
+        k_sep_t1<<<g, 64, 0, st>>>(N, K, s.nz, dv.sep_kbox.as<int>(), dv.sep_coef.as<double2>(), dv.ez.as<double2>(),
+                                    dv.sep_t1.as<double2>());
         RP_HIP(ctx, hipGetLastError());
     }
     RP_HIP(ctx, hipEventSynchronize(dv.ev_plan));
