@@ -244,9 +244,10 @@ int sep_prepare(rp_fit *fit, FitDevice &dv, int64_t rows, SepPlan &plan);
 void sep_phi_chunk(rp_fit *fit, FitDevice &dv, const SepPlan &plan, int chunk, int64_t r0, int64_t valid,
                    int64_t rows, double *phi, unsigned long long *colmax, hipStream_t st, const int *gate = nullptr,
                    const int *gate_flags = nullptr);
-// Column-major Phi chunk -> int8 slices [P][NP][rows] (P = 3 or 4) with per-column power-of-two scales.
-void sep_slice_chunk(const double *phi, int NP, int64_t rows, int64_t valid, const unsigned long long *colmax,
-                     int8_t *slices, double *scale_out, int P, hipStream_t st);
+// Column-major Phi chunk -> int8 slices [P][NP][rows] (P = 3 or 4) with per-column power-of-two scales,
+// rows [0, span) written (span a multiple of 16, at most rows; rows past `valid` as zeros).
+void sep_slice_chunk(const double *phi, int NP, int64_t rows, int64_t valid, int64_t span,
+                     const unsigned long long *colmax, int8_t *slices, double *scale_out, int P, hipStream_t st);
 __global__ void k_symmetrize(double *gram, int NP, int tile);
 
 }  // namespace rp

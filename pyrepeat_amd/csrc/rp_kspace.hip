@@ -511,10 +511,12 @@ __global__ __launch_bounds__(256) void k_sep_images(ImageArgs p) {
 __global__ void k_zero_flag(int *f) { *f = 0; }
 
 // Column-major Phi -> slices [P][NP][rows] of x / S_col, S_col the power of two with
-// max |Phi_col| / S_col <= 127/128; rows at or past `valid` are zero.  Both sides are streams: a
-// thread takes 16 consecutive rows of one column (128-byte read, one 16-byte write per slice).
+// max |Phi_col| / S_col <= 127/128; rows [valid, span) are zero, rows from span on are not written.
+// Both sides are streams: a thread takes 16 consecutive rows of one column (128-byte read, one
+// 16-byte write per slice).
 template <int P>
 __global__ __launch_bounds__(256) void k_slice_cols(const double *__restrict__ phi, int64_t rows, int64_t valid,
+                                                    int64_t span,
                                                     const unsigned long long *__restrict__ colmax,
                                                     int8_t *__restrict__ out, double *__restrict__ scale_out) {
     const int a = blockIdx.y;
@@ -524,7 +526,7 @@ __global__ __launch_bounds__(256) void k_slice_cols(const double *__restrict__ p
     if (mx > 0.0) S = ldexp(1.0, (int)ceil(log2(mx * (128.0 / 127.0))));
     while (mx / S > 127.0 / 128.0) S *= 2.0;
     if (blockIdx.x == 0 && threadIdx.x == 0) scale_out[a] = S;
-    if (r0 >= rows) return;
+    if (r0 >= span) return;
     const double inv = 1.0 / S;
     const double *src = phi + (int64_t)a * rows + r0;
     double v[16];
@@ -682,13 +684,13 @@ void sep_phi_chunk(rp_fit *fit, FitDevice &dv, const SepPlan &plan, int chunk, i
     }
 }
 
-void sep_slice_chunk(const double *phi, int NP, int64_t rows, int64_t valid, const unsigned long long *colmax,
-                     int8_t *slices, double *scale_out, int P, hipStream_t st) {
-    dim3 g((unsigned)ceil_div(rows, 256 * 16), (unsigned)NP);
+void sep_slice_chunk(const double *phi, int NP, int64_t rows, int64_t valid, int64_t span,
+                     const unsigned long long *colmax, int8_t *slices, double *scale_out, int P, hipStream_t st) {
+    dim3 g((unsigned)ceil_div(span, 256 * 16), (unsigned)NP);
     if (P == 3)
-        k_slice_cols<3><<<g, 256, 0, st>>>(phi, rows, valid, colmax, slices, scale_out);
+        k_slice_cols<3><<<g, 256, 0, st>>>(phi, rows, valid, span, colmax, slices, scale_out);
     else
-        k_slice_cols<4><<<g, 256, 0, st>>>(phi, rows, valid, colmax, slices, scale_out);
+        k_slice_cols<4><<<g, 256, 0, st>>>(phi, rows, valid, span, colmax, slices, scale_out);
 }
 
 }  // namespace rp

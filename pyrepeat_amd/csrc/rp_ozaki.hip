@@ -340,6 +340,9 @@ int accumulate_split_p(rp_fit *fit, FitDevice &dv, bool keep_phi) {
     int chunk = 0;
     for (int64_t r0 = 0; r0 < M; r0 += rows, ++chunk) {
         const int64_t valid = std::min<int64_t>(rows, M - r0);
+        // a short last chunk: each split shrinks to whole stages covering `valid` (the rows it takes
+        // past `valid` are zero slices), so the Gram does not stream the chunk's empty tail
+        const int rps_c = valid == rows ? rps : (int)round_up(ceil_div(valid, splits), gram_kd<P>());
         double *phi = d_phi.as<double>() + (keep_phi ? (int64_t)chunk * NP * rows : 0);
         k_zero_u64<<<ceil_div(NP, 256), 256, 0, st>>>(d_colmax.as<unsigned long long>(), NP);
         {
@@ -348,12 +351,12 @@ int accumulate_split_p(rp_fit *fit, FitDevice &dv, bool keep_phi) {
         }
         {
             KTimer kts(ctx, KT_SLICE);
-            sep_slice_chunk(phi, NP, rows, valid, d_colmax.as<unsigned long long>(), d_slices.as<int8_t>(),
-                            d_scale.as<double>(), P, st);
+            sep_slice_chunk(phi, NP, rows, valid, (int64_t)rps_c * splits, d_colmax.as<unsigned long long>(),
+                            d_slices.as<int8_t>(), d_scale.as<double>(), P, st);
         }
         KTimer ktg(ctx, KT_GRAM);
         ge.accumulate = chunk > 0;
-        k_oz_gram<P><<<n_tiles * splits, NTHR, lds_bytes, st>>>(go, ge, rps, splits);
+        k_oz_gram<P><<<n_tiles * splits, NTHR, lds_bytes, st>>>(go, ge, rps_c, splits);
     }
     {
         KTimer ktr(ctx, KT_REDUCE);
