@@ -420,8 +420,12 @@ rp_fit::~rp_fit() {
 
 static void destroy_ctx(rp_ctx *ctx) {
     (void)hipSetDevice(ctx->device);
+    if (ctx->side) (void)hipStreamSynchronize(ctx->side);
     (void)hipStreamSynchronize(ctx->stream);
     (void)hipStreamDestroy(ctx->stream);
+    if (ctx->side) (void)hipStreamDestroy(ctx->side);
+    if (ctx->ev_fork) (void)hipEventDestroy(ctx->ev_fork);
+    if (ctx->ev_join) (void)hipEventDestroy(ctx->ev_join);
     delete ctx;
 }
 

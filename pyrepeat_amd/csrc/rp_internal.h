@@ -42,6 +42,10 @@ struct DBuf {
 struct Ctx {
     int device = 0;
     hipStream_t stream = nullptr;
+    // a second stream (default priority) for the solve's sensitivity estimate, which then runs beside
+    // the refinement; made on first use, with the events that fork it from and join it to `stream`
+    hipStream_t side = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     uint64_t budget = 0;
     uint64_t in_use = 0;
     std::string err;
@@ -300,6 +304,7 @@ struct rp_fit {
     size_t h_stage_bytes = 0;
     int split_tiles_np = 0;         // NP the split Gram's tile list in ws[9] was built for
     uint32_t bs_base = 0, bs_epoch = 0;   // triangular solves: tickets taken so far, launch epoch
+    uint32_t bs2_base = 0, bs2_epoch = 0; // the same for the solves on the side stream (own counter, flags)
     // refinement (RP_PREC_OZAKI_REFINED): the accumulate keeps every chunk of the fp64 basis in ws[7]
     // ([chunk][NP][phi_rows] doubles); phi_kept says it matches the fit's current system
     bool phi_kept = false;

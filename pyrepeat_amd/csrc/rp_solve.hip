@@ -1707,11 +1707,11 @@ int run_solve(rp_fit *fit, const rp_constraints *con, double *q_out, double *sol
     RP_CHECK(d_wc.grow(ctx, sizeof(double) * nr));
     RP_CHECK(d_bc.grow(ctx, sizeof(double) * nr));
     RP_CHECK(d_dinv.grow(ctx, sizeof(double) * CB * CB * (size_t)ceil_div(nr, CB)));
-    // refinement vectors: u (N), r (N), r1 (nr), z (nr), dx (nr + 1), r2, then the state (16 bytes);
-    // e over the kept points and its block sums
+    // refinement vectors: u (N), r (N), r1 (nr), z (nr), dx (nr + 1), r2, the sensitivity solve's
+    // sz (nr), szz (nr), sdx (nr + 1), then the two states; e over the kept points and its block sums
     const bool refine = fit->opt.precision == RP_PREC_OZAKI_REFINED && (fit->phi_kept || fit->refine_stream);
     DBuf &d_rv = fit->ws[10], &d_e = fit->ws[8];
-    const size_t rv_doubles = 2 * (size_t)N + 3 * (size_t)nr + 2;
+    const size_t rv_doubles = 2 * (size_t)N + 6 * (size_t)nr + 3;
     RP_CHECK(d_rv.grow(ctx, sizeof(double) * round_up((int64_t)rv_doubles, 2) + 2 * sizeof(RefineState)));
     RefineState *d_rs = reinterpret_cast<RefineState *>(d_rv.as<double>() + round_up((int64_t)rv_doubles, 2));
     RefineState *d_sens = d_rs + 1;   // gates the sensitivity solve
@@ -1791,17 +1791,22 @@ int run_solve(rp_fit *fit, const rp_constraints *con, double *q_out, double *sol
     // when a pivot failed); then expansion, results and the pivot flag in one round trip
     // ws[31]: the ticket counter and one flag per block, cleared when (re)allocated; flags compare
     // with a per-solve epoch, so they are never cleared between solves
+    // Two such regions, each half of the buffer (its offset changes only when the buffer is reallocated,
+    // which clears both): the second serves the solves queued on the side stream.
     DBuf &d_sync = fit->ws[31];
     const int nblk = (int)ceil_div(nr, CB);
     const void *sync_before = d_sync.p;
     const size_t sync_bytes_before = d_sync.bytes;
-    RP_CHECK(d_sync.grow(ctx, sizeof(unsigned) * (size_t)(1 + nblk)));
+    RP_CHECK(d_sync.grow(ctx, 2 * sizeof(unsigned) * (size_t)(1 + nblk)));
     if (d_sync.p != sync_before || d_sync.bytes != sync_bytes_before) {
         const int words = (int)(d_sync.bytes / sizeof(unsigned));
         k_fill_u32<<<ceil_div(words, 256), 256, 0, st>>>(d_sync.as<unsigned>(), words, 0u);
         fit->bs_base = 0;
         fit->bs_epoch = 0;
+        fit->bs2_base = 0;
+        fit->bs2_epoch = 0;
     }
+    unsigned *sync2 = d_sync.as<unsigned>() + d_sync.bytes / sizeof(unsigned) / 2;
     static const unsigned spin_limit = [] {
         const char *v = std::getenv("RP_BACKSUB_SPIN");
         return v ? (unsigned)std::strtoul(v, nullptr, 10) : BACKSUB_SPIN;
@@ -1835,27 +1840,48 @@ int run_solve(rp_fit *fit, const rp_constraints *con, double *q_out, double *sol
         fit->bs_base += (unsigned)nblk;
         k_ref_update<<<1, 256, 0, st>>>(d_x.as<double>(), dx, nr, RP_REFINE_TARGET_E, d_cinfo.as<int>(), d_rs);
     };
+    // sensitivity to the fp64 rounding of the system: one more solve with the same factor, from the
+    // solution before refinement (the estimate needs only its magnitudes).  With refinement it runs on
+    // the side stream, beside the refinement's passes over the kept basis, with its own vectors and
+    // ticket region; the main stream waits for it before publishing.
+    {
+        double *sz = d_rv.as<double>() + 2 * (size_t)N + 3 * (size_t)nr + 2, *szz = sz + nr, *sdx = szz + nr;
+        {
+            KTimer kts2(ctx, KT_SOLVE);
+            k_sens_rhs<<<nr, 256, 0, st>>>(d_K2.as<double>(), ld, nr, d_x.as<double>(), SENS_EPS, sz,
+                                           d_cinfo.as<int>(), d_sens);
+        }
+        hipStream_t ss = st;
+        unsigned *ssync = d_sync.as<unsigned>();
+        uint32_t *sbase = &fit->bs_base, *sepoch = &fit->bs_epoch;
+        if (refine) {
+            if (!ctx->side) RP_HIP(ctx, hipStreamCreateWithFlags(&ctx->side, hipStreamNonBlocking));
+            if (!ctx->ev_fork) RP_HIP(ctx, hipEventCreateWithFlags(&ctx->ev_fork, hipEventDisableTiming));
+            if (!ctx->ev_join) RP_HIP(ctx, hipEventCreateWithFlags(&ctx->ev_join, hipEventDisableTiming));
+            RP_HIP(ctx, hipEventRecord(ctx->ev_fork, st));
+            RP_HIP(ctx, hipStreamWaitEvent(ctx->side, ctx->ev_fork, 0));
+            ss = ctx->side;
+            ssync = sync2;
+            sbase = &fit->bs2_base;
+            sepoch = &fit->bs2_epoch;
+        }
+        k_fwdsub<<<nblk, 256, 0, ss>>>(d_K.as<double>(), ld, nr, d_dinv.as<double>(), sz, szz, ssync, *sbase,
+                                       ++*sepoch, spin_limit, d_cinfo.as<int>() + 1, d_cinfo.as<int>(), d_sens);
+        *sbase += (unsigned)nblk;
+        k_backsub<<<nblk, 256, 0, ss>>>(d_K.as<double>(), ld, nr, d_dinv.as<double>(), d_wc.as<double>(), szz, 0.0, sdx,
+                                        ssync, *sbase, ++*sepoch, spin_limit, d_cinfo.as<int>() + 1,
+                                        d_cinfo.as<int>(), nullptr, &d_sens->go);
+        *sbase += (unsigned)nblk;
+        k_sens_max<<<1, 256, 0, ss>>>(sdx, nr, d_cinfo.as<int>(), h_out_dev + 4 + N + n);
+        if (refine) RP_HIP(ctx, hipEventRecord(ctx->ev_join, ctx->side));
+    }
     int refine_queued = 0;
     if (refine) {
-        KTimer ktr(ctx, KT_REFINE);
-        for (; refine_queued < std::min(refine_cap, REFINE_EAGER); ++refine_queued) refine_step();
-    }
-    {
-        // sensitivity to the fp64 rounding of the system: one more solve with the same factor (the
-        // refinement's vectors are free again: r1 <- z, z <- L^-1 z, dx <- the charge change)
-        KTimer kts2(ctx, KT_SOLVE);
-        double *u = d_rv.as<double>(), *r = u + N, *sz = r + N, *szz = sz + nr, *sdx = szz + nr;
-        k_sens_rhs<<<nr, 256, 0, st>>>(d_K2.as<double>(), ld, nr, d_x.as<double>(), SENS_EPS, sz, d_cinfo.as<int>(),
-                                       d_sens);
-        k_fwdsub<<<nblk, 256, 0, st>>>(d_K.as<double>(), ld, nr, d_dinv.as<double>(), sz, szz, d_sync.as<unsigned>(),
-                                       fit->bs_base, ++fit->bs_epoch, spin_limit, d_cinfo.as<int>() + 1,
-                                       d_cinfo.as<int>(), d_sens);
-        fit->bs_base += (unsigned)nblk;
-        k_backsub<<<nblk, 256, 0, st>>>(d_K.as<double>(), ld, nr, d_dinv.as<double>(), d_wc.as<double>(), szz, 0.0, sdx,
-                                        d_sync.as<unsigned>(), fit->bs_base, ++fit->bs_epoch, spin_limit,
-                                        d_cinfo.as<int>() + 1, d_cinfo.as<int>(), nullptr, &d_sens->go);
-        fit->bs_base += (unsigned)nblk;
-        k_sens_max<<<1, 256, 0, st>>>(sdx, nr, d_cinfo.as<int>(), h_out_dev + 4 + N + n);
+        {
+            KTimer ktr(ctx, KT_REFINE);
+            for (; refine_queued < std::min(refine_cap, REFINE_EAGER); ++refine_queued) refine_step();
+        }
+        RP_HIP(ctx, hipStreamWaitEvent(st, ctx->ev_join, 0));
     }
     k_publish<<<ceil_div(N + 1, 256), 256, 0, st>>>(d_x.as<double>(), dc_exp, N, n, d_cinfo.as<int>(), d_rs, h_out_dev);
     RP_HIP(ctx, hipGetLastError());
@@ -1892,6 +1918,8 @@ int run_solve(rp_fit *fit, const rp_constraints *con, double *q_out, double *sol
         k_fill_u32<<<ceil_div(words, 256), 256, 0, st>>>(d_sync.as<unsigned>(), words, 0u);
         fit->bs_base = 0;
         fit->bs_epoch = 0;
+        fit->bs2_base = 0;
+        fit->bs2_epoch = 0;
     }
     if (cinfo != 0) {
         // indefinite or singular A (or a triangular-solve wait that gave up): LU with partial pivoting on
