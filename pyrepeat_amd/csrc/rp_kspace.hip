@@ -349,41 +349,61 @@ __global__ __launch_bounds__(SEP_THREADS) void k_sep_phi(SepArgs p) {
             t2w[wave][li][a_l][s] = t2;
         }
     }
+    // ---- per (line, atom), one lane each: the parts of the nearest-image test and of the displacement
+    // that depend on the line (j, m) only, into entries 7 and 8 of the atom's row (entry 7 held T2 of
+    // h = 7, which is zero: h <= HMAX = 6):
+    //     (t_x, t_y), (t_z, +-b12),  t = w1 b1 + w2 b2 (rows of the cell, fma as the point step uses),
+    //     b12 = min((1 - |w1|) h1, (1 - |w2|) h2), + when the image (n1, n2) is (0, 0), - otherwise,
+    // so that the point step is left with the terms of grid axis 0 (nearest_only() split in two).
+    if (e.fit_type != 0) {
+        const int gl = lane >> 3, ga = ablk * SEP_ATOMS + wave * 8 + (lane & 7);
+        if (gl < n_l && ga < p.N) {
+            const double d1 = fma((double)lj[gl], e.inv_n[1], -e.afrac[3 * ga + 1]);
+            const double d2 = fma((double)lm[gl], e.inv_n[2], -e.afrac[3 * ga + 2]);
+            const double n1 = rint(d1), n2 = rint(d2);
+            const double w1 = d1 - n1, w2 = d2 - n2;
+            const double b12 = fmin((1.0 - fabs(w1)) * e.h[1], (1.0 - fabs(w2)) * e.h[2]);
+            t2w[wave][gl][lane & 7][SH - 1] =
+                make_double2(fma(w1, e.box[3], w2 * e.box[6]), fma(w1, e.box[4], w2 * e.box[7]));
+            t2w[wave][gl][lane & 7][SH] =
+                make_double2(fma(w1, e.box[5], w2 * e.box[8]), (n1 == 0.0 && n2 == 0.0) ? b12 : -b12);
+        }
+    }
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    // ---- points (lane s = point slot)
-    double Rx = 0.0, Ry = 0.0, Rz = 0.0, Fx = 0.0, Fy = 0.0, Fz = 0.0;
-    if (atom) {
-        Rx = e.atoms[3 * a];
-        Ry = e.atoms[3 * a + 1];
-        Rz = e.atoms[3 * a + 2];
-        Fx = e.afrac[3 * a];
-        Fy = e.afrac[3 * a + 1];
-        Fz = e.afrac[3 * a + 2];
-    }
+    // ---- points (lane s = point slot); rows are chunk-relative to the kernel's 32-bit range
     const double alpha = e.sqrt_alpha * e.sqrt_alpha;
-    const int64_t rs = max<int64_t>(lrow[0], p.r0);
-    const int64_t rend = min<int64_t>(lrow[n_l], p.r0 + p.valid);
+    const int rs = (int)max<int64_t>(lrow[0], p.r0);
+    const int rend = (int)min<int64_t>(lrow[n_l], p.r0 + p.valid);
     double cmax = 0.0;
     bool multi = false;   // a pair of this lane has several images within Rc
-    int li = 0, base = lkey[0] * nx, j = lj[0], m = lm[0];
-    int64_t r = rs + s;
-    const int *idx = e.idx + r;
-    double *out = p.phi + (int64_t)a * p.rows + (r - p.r0);
-    int f_next = r < rend ? *idx : 0;
-    for (; r < rend; r += SH, idx += SH, out += SH) {
-        if (r >= lrow[li + 1]) {
-            while (r >= lrow[li + 1]) ++li;
-            base = lkey[li] * nx;
-            m = lm[li];
-            j = lj[li];
-        }
-        const int i = f_next - base;
-        if (r + SH < rend) f_next = idx[SH];
-        double val = 0.0;
-        bool mine = true;   // val is the pair's final value (not left to the multi-image kernel)
-        if (atom) {
+    if (atom) {
+        const double Fx = e.afrac[3 * a];
+        int li = 0, nb = lrow[1], base = lkey[0] * nx;
+        int r = rs + s;
+        const int *idx = e.idx + r;
+        double *out = p.phi + (int64_t)a * p.rows + (r - p.r0);
+        // grid indices are loaded two steps ahead, unconditionally (from the current row when there is no
+        // such step), so a step never waits for the load it issues
+        int f_cur = r < rend ? idx[0] : 0;
+        int f_next = r + SH < rend ? idx[SH] : 0;
+        for (; r < rend; r += SH, idx += SH, out += SH) {
+            if (r >= nb) {
+                do {
+                    ++li;
+                    nb = lrow[li + 1];
+                } while (r >= nb);
+                base = lkey[li] * nx;
+            }
+            const int i = f_cur - base;
+            f_cur = f_next;
+            f_next = idx[r + 2 * SH < rend ? 2 * SH : 0];
+            double2 g01 = make_double2(0.0, 0.0), g2 = make_double2(0.0, 0.0);
+            if (e.fit_type != 0) {
+                g01 = t2w[wave][li][a_l][SH - 1];
+                g2 = t2w[wave][li][a_l][SH];
+            }
             double acc = 0.0;
             if (recip) {
                 // Re sum_h T2[h] X(1, i)^h by Horner from h = HMAX
@@ -398,25 +418,31 @@ __global__ __launch_bounds__(SEP_THREADS) void k_sep_phi(SepArgs p) {
                 }
                 acc = fma(t.x, x.x, fma(-t.y, x.y, t2w[wave][li][a_l][0].x));
             }
-            const double id = (double)i, jd = (double)j, md = (double)m;
+            const double id = (double)i;
             double real = 0.0;
+            bool mine = true;   // val is the pair's final value (not left to the multi-image kernel)
             if (e.fit_type == 0) {
+                const int key = lkey[li];
+                const double jd = (double)(key % ny), md = (double)(key / ny);
                 const double gx = (id * e.geom.ax[0] + jd * e.geom.ax[3]) + md * e.geom.ax[6];
                 const double gy = (id * e.geom.ax[1] + jd * e.geom.ax[4]) + md * e.geom.ax[7];
                 const double gz = (id * e.geom.ax[2] + jd * e.geom.ax[5]) + md * e.geom.ax[8];
-                const double dx = gx - Rx, dy = gy - Ry, dz = gz - Rz;
+                const double dx = gx - e.atoms[3 * a], dy = gy - e.atoms[3 * a + 1], dz = gz - e.atoms[3 * a + 2];
                 const double d = sqrt((dx * dx + dy * dy) + dz * dz);
                 real = d > 1e-10 ? 1.0 / d : 0.0;
             } else {
-                double d0, d1, d2, n0, n1, n2;
-                if (nearest_only(e, id, jd, md, Fx, Fy, Fz, d0, d1, d2, n0, n1, n2)) {
+                // nearest_only() with the line's terms from entries 7 and 8
+                const double d0 = fma(id, e.inv_n[0], -Fx);
+                const double n0 = rint(d0);
+                const double w0 = d0 - n0;
+                const double b0 = (1.0 - fabs(w0)) * e.h[0];
+                if (fmin(b0, fabs(g2.y)) > e.r_cut) {
                     // only the nearest image can be within Rc: displacement w . cell to it
-                    const double w0 = d0 - n0, w1 = d1 - n1, w2 = d2 - n2;
-                    const double dx = fma(w0, e.box[0], fma(w1, e.box[3], w2 * e.box[6]));
-                    const double dy = fma(w0, e.box[1], fma(w1, e.box[4], w2 * e.box[7]));
-                    const double dz = fma(w0, e.box[2], fma(w1, e.box[5], w2 * e.box[8]));
+                    const double dx = fma(w0, e.box[0], g01.x);
+                    const double dy = fma(w0, e.box[1], g01.y);
+                    const double dz = fma(w0, e.box[2], g2.x);
                     const double r2 = fma(dx, dx, fma(dy, dy, dz * dz));
-                    if (r2 <= e.R2 && !(n0 == 0.0 && n1 == 0.0 && n2 == 0.0 && r2 <= 1e-20))
+                    if (r2 <= e.R2 && !(n0 == 0.0 && g2.y > 0.0 && r2 <= 1e-20))
                         real = erfc_over_r(r2, alpha, e.sqrt_alpha);
                 } else {
                     // several images may lie within Rc: k_sep_images adds that sum afterwards
@@ -424,14 +450,19 @@ __global__ __launch_bounds__(SEP_THREADS) void k_sep_phi(SepArgs p) {
                     multi = true;
                 }
             }
-            val = acc + real;
-        } else if (a == p.N) {
-            val = e.v[r];
-        } else if (a == p.N + 1) {
-            val = 1.0;
+            const double val = acc + real;
+            *out = val;
+            if (mine) cmax = fmax(cmax, fabs(val));
         }
-        *out = val;
-        if (mine) cmax = fmax(cmax, fabs(val));
+    } else if (a < p.NP) {
+        // the V column, the ones column and the zero padding of the atom columns
+        const int64_t r1 = rs + s;
+        double *out = p.phi + (int64_t)a * p.rows + (r1 - p.r0);
+        for (int64_t r = r1; r < rend; r += SH, out += SH) {
+            const double val = a == p.N ? e.v[r] : (a == p.N + 1 ? 1.0 : 0.0);
+            *out = val;
+            cmax = fmax(cmax, fabs(val));
+        }
     }
     if (__ballot(multi) != 0 && lane == 0) *p.multi_flag = 1;
     // column maximum over the 8 point slots of the atom
