@@ -372,89 +372,104 @@ __global__ __launch_bounds__(SEP_THREADS) void k_sep_phi(SepArgs p) {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    // ---- points (lane s = point slot); rows are chunk-relative to the kernel's 32-bit range
+    // ---- points (lane s = point slot), in blocks of 64 rows: lane L of the wave loads the grid index of
+    // row rb + L (one coalesced load per block, issued a block ahead) and packs its column i and line
+    // slot li as (i << 3) | li (i < 2048); a step fetches its row's pack from the lane that holds it
+    // (ds_bpermute), so it waits neither on a global load nor on the line table.  Rows are absolute
+    // and fit in 32 bits.
     const double alpha = e.sqrt_alpha * e.sqrt_alpha;
     const int rs = (int)max<int64_t>(lrow[0], p.r0);
     const int rend = (int)min<int64_t>(lrow[n_l], p.r0 + p.valid);
     double cmax = 0.0;
     bool multi = false;   // a pair of this lane has several images within Rc
-    if (atom) {
-        const double Fx = e.afrac[3 * a];
-        int li = 0, nb = lrow[1], base = lkey[0] * nx;
-        int r = rs + s;
-        const int *idx = e.idx + r;
-        double *out = p.phi + (int64_t)a * p.rows + (r - p.r0);
-        // grid indices are loaded two steps ahead, unconditionally (from the current row when there is no
-        // such step), so a step never waits for the load it issues
-        int f_cur = r < rend ? idx[0] : 0;
-        int f_next = r + SH < rend ? idx[SH] : 0;
-        for (; r < rend; r += SH, idx += SH, out += SH) {
-            if (r >= nb) {
-                do {
-                    ++li;
-                    nb = lrow[li + 1];
-                } while (r >= nb);
-                base = lkey[li] * nx;
-            }
-            const int i = f_cur - base;
-            f_cur = f_next;
-            f_next = idx[r + 2 * SH < rend ? 2 * SH : 0];
-            double2 g01 = make_double2(0.0, 0.0), g2 = make_double2(0.0, 0.0);
-            if (e.fit_type != 0) {
-                g01 = t2w[wave][li][a_l][SH - 1];
-                g2 = t2w[wave][li][a_l][SH];
-            }
-            double acc = 0.0;
-            if (recip) {
-                // Re sum_h T2[h] X(1, i)^h by Horner from h = HMAX
-                const double2 x = xs[i];
-                double2 t = t2w[wave][li][a_l][HMAX];
+    {
+        const double Fx = atom ? e.afrac[3 * a] : 0.0;
+        // line boundaries (uniform); slots past the last line never match
+        int bnd[SEP_LINES - 1];
 #pragma unroll
-                for (int q = HMAX - 1; q >= 1; --q) {
-                    const double2 c = t2w[wave][li][a_l][q];
-                    const double tx = fma(t.x, x.x, fma(-t.y, x.y, c.x));
-                    const double ty = fma(t.x, x.y, fma(t.y, x.x, c.y));
-                    t = make_double2(tx, ty);
-                }
-                acc = fma(t.x, x.x, fma(-t.y, x.y, t2w[wave][li][a_l][0].x));
+        for (int q = 0; q < SEP_LINES - 1; ++q)
+            bnd[q] = q + 1 < n_l ? __builtin_amdgcn_readfirstlane(lrow[q + 1]) : 0x7fffffff;
+        double *const out0 = p.phi + (int64_t)a * p.rows - p.r0;
+        int f_blk = e.idx[min(rs + lane, rend - 1)];
+        // every lane runs the block loop (the packs are exchanged across the whole wave); the atom
+        // columns' lanes evaluate the pairs
+        for (int rb = rs; rb < rend; rb += 64) {
+            int pk;
+            {
+                const int rr = rb + lane;
+                int li = 0;
+#pragma unroll
+                for (int q = 0; q < SEP_LINES - 1; ++q) li += rr >= bnd[q] ? 1 : 0;
+                pk = ((f_blk - lkey[li] * nx) << 3) | li;
             }
-            const double id = (double)i;
-            double real = 0.0;
-            bool mine = true;   // val is the pair's final value (not left to the multi-image kernel)
-            if (e.fit_type == 0) {
-                const int key = lkey[li];
-                const double jd = (double)(key % ny), md = (double)(key / ny);
-                const double gx = (id * e.geom.ax[0] + jd * e.geom.ax[3]) + md * e.geom.ax[6];
-                const double gy = (id * e.geom.ax[1] + jd * e.geom.ax[4]) + md * e.geom.ax[7];
-                const double gz = (id * e.geom.ax[2] + jd * e.geom.ax[5]) + md * e.geom.ax[8];
-                const double dx = gx - e.atoms[3 * a], dy = gy - e.atoms[3 * a + 1], dz = gz - e.atoms[3 * a + 2];
-                const double d = sqrt((dx * dx + dy * dy) + dz * dz);
-                real = d > 1e-10 ? 1.0 / d : 0.0;
-            } else {
-                // nearest_only() with the line's terms from entries 7 and 8
-                const double d0 = fma(id, e.inv_n[0], -Fx);
-                const double n0 = rint(d0);
-                const double w0 = d0 - n0;
-                const double b0 = (1.0 - fabs(w0)) * e.h[0];
-                if (fmin(b0, fabs(g2.y)) > e.r_cut) {
-                    // only the nearest image can be within Rc: displacement w . cell to it
-                    const double dx = fma(w0, e.box[0], g01.x);
-                    const double dy = fma(w0, e.box[1], g01.y);
-                    const double dz = fma(w0, e.box[2], g2.x);
-                    const double r2 = fma(dx, dx, fma(dy, dy, dz * dz));
-                    if (r2 <= e.R2 && !(n0 == 0.0 && g2.y > 0.0 && r2 <= 1e-20))
-                        real = erfc_over_r(r2, alpha, e.sqrt_alpha);
+            f_blk = e.idx[min(rb + 64 + lane, rend - 1)];
+            // the next step's pack is fetched a step ahead
+            int pnext = __shfl(pk, s, 64);
+            for (int k = 0; k < 8 && rb + 8 * k < rend; ++k) {
+                const int r = rb + 8 * k + s;
+                const int prow = pnext;
+                if (k < 7) pnext = __shfl(pk, 8 * (k + 1) + s, 64);
+                if (r >= rend || !atom) continue;
+                const int i = prow >> 3, li = prow & 7;
+                double *out = out0 + r;
+                double2 g01 = make_double2(0.0, 0.0), g2 = make_double2(0.0, 0.0);
+                if (e.fit_type != 0) {
+                    g01 = t2w[wave][li][a_l][SH - 1];
+                    g2 = t2w[wave][li][a_l][SH];
+                }
+                double acc = 0.0;
+                if (recip) {
+                    // Re sum_h T2[h] X(1, i)^h by Horner from h = HMAX
+                    const double2 x = xs[i];
+                    double2 t = t2w[wave][li][a_l][HMAX];
+#pragma unroll
+                    for (int q = HMAX - 1; q >= 1; --q) {
+                        const double2 c = t2w[wave][li][a_l][q];
+                        const double tx = fma(t.x, x.x, fma(-t.y, x.y, c.x));
+                        const double ty = fma(t.x, x.y, fma(t.y, x.x, c.y));
+                        t = make_double2(tx, ty);
+                    }
+                    acc = fma(t.x, x.x, fma(-t.y, x.y, t2w[wave][li][a_l][0].x));
+                }
+                const double id = (double)i;
+                double real = 0.0;
+                bool mine = true;   // val is the pair's final value (not left to the multi-image kernel)
+                if (e.fit_type == 0) {
+                    const int key = lkey[li];
+                    const double jd = (double)(key % ny), md = (double)(key / ny);
+                    const double gx = (id * e.geom.ax[0] + jd * e.geom.ax[3]) + md * e.geom.ax[6];
+                    const double gy = (id * e.geom.ax[1] + jd * e.geom.ax[4]) + md * e.geom.ax[7];
+                    const double gz = (id * e.geom.ax[2] + jd * e.geom.ax[5]) + md * e.geom.ax[8];
+                    const double dx = gx - e.atoms[3 * a], dy = gy - e.atoms[3 * a + 1], dz = gz - e.atoms[3 * a + 2];
+                    const double d = sqrt((dx * dx + dy * dy) + dz * dz);
+                    real = d > 1e-10 ? 1.0 / d : 0.0;
                 } else {
-                    // several images may lie within Rc: k_sep_images adds that sum afterwards
-                    mine = false;
-                    multi = true;
+                    // nearest_only() with the line's terms from entries 7 and 8
+                    const double d0 = fma(id, e.inv_n[0], -Fx);
+                    const double n0 = rint(d0);
+                    const double w0 = d0 - n0;
+                    const double b0 = (1.0 - fabs(w0)) * e.h[0];
+                    if (fmin(b0, fabs(g2.y)) > e.r_cut) {
+                        // only the nearest image can be within Rc: displacement w . cell to it
+                        const double dx = fma(w0, e.box[0], g01.x);
+                        const double dy = fma(w0, e.box[1], g01.y);
+                        const double dz = fma(w0, e.box[2], g2.x);
+                        const double r2 = fma(dx, dx, fma(dy, dy, dz * dz));
+                        if (r2 <= e.R2 && !(n0 == 0.0 && g2.y > 0.0 && r2 <= 1e-20))
+                            real = erfc_over_r(r2, alpha, e.sqrt_alpha);
+                    } else {
+                        // several images may lie within Rc: k_sep_images adds that sum afterwards
+                        mine = false;
+                        multi = true;
+                    }
                 }
+                const double val = acc + real;
+                *out = val;
+                if (mine) cmax = fmax(cmax, fabs(val));
             }
-            const double val = acc + real;
-            *out = val;
-            if (mine) cmax = fmax(cmax, fabs(val));
         }
-    } else if (a < p.NP) {
+    }
+    if (!atom && a < p.NP) {
         // the V column, the ones column and the zero padding of the atom columns
         const int64_t r1 = rs + s;
         double *out = p.phi + (int64_t)a * p.rows + (r1 - p.r0);
