@@ -61,11 +61,22 @@ __device__ __forceinline__ double erf_poly_horner(double u, double p) {
     }
 }
 
-// erfc(sqrt(alpha) r) / r for 0 < r <= Rc.  alpha = (pi / Rc)^2 (repeat.py:1087), so u = alpha r^2 lies
-// in [0, pi^2] and erfc(sqrt(alpha) r) / r = 1/r - sqrt(alpha) P(u) with erf(x) = x P(x^2); P is one
-// polynomial (max abs error of the erfc 7.5e-16), so the term needs one rsqrt and no table.
+// 1/sqrt(x) for a positive normal x: the hardware estimate and one second-order Newton step,
+// y = y0 + y0 e (1/2 + 3/8 e) with e = 1 - x y0^2, the steps the library's rsqrt takes; its select for
+// zero, infinite and NaN inputs is left out (bitwise the same result otherwise; a pair at distance 0
+// that is not the excluded self term gives NaN here instead of the library's infinity).
+__device__ __forceinline__ double rsqrt_pos(double x) {
+    const double y0 = __builtin_amdgcn_rsq(x);
+    const double e = fma(-x * y0, y0, 1.0);
+    return fma(y0 * e, fma(e, 0.375, 0.5), y0);
+}
+
+// erfc(sqrt(alpha) r) / r for 0 < r <= Rc (r^2 > 1e-20).  alpha = (pi / Rc)^2 (repeat.py:1087), so
+// u = alpha r^2 lies in [0, pi^2] and erfc(sqrt(alpha) r) / r = 1/r - sqrt(alpha) P(u) with
+// erf(x) = x P(x^2); P is one polynomial (max abs error of the erfc 7.5e-16), so the term needs one
+// rsqrt and no table.
 __device__ __forceinline__ double erfc_over_r(double r2, double alpha, double sqrt_alpha) {
-    const double rinv = rsqrt(r2);
+    const double rinv = rsqrt_pos(r2);
     constexpr uint64_t mu0 = __builtin_bit_cast(uint64_t, -ERF_U0);
     const double u = fma_sgpr_addend(alpha, r2, sgpr_const<mu0>());
     constexpr uint64_t top = __builtin_bit_cast(uint64_t, kErfPoly[ERF_DEG]);
@@ -412,13 +423,10 @@ __global__ __launch_bounds__(SEP_THREADS) void k_sep_phi(SepArgs p) {
                 if (r >= rend || !atom) continue;
                 const int i = prow >> 3, li = prow & 7;
                 double *out = out0 + r;
-                double2 g01 = make_double2(0.0, 0.0), g2 = make_double2(0.0, 0.0);
-                if (e.fit_type != 0) {
-                    g01 = t2w[wave][li][a_l][SH - 1];
-                    g2 = t2w[wave][li][a_l][SH];
-                }
+                // (entries 7 and 8 are read whatever the fit type: fit-type 0 does not use them)
+                const double2 g01 = t2w[wave][li][a_l][SH - 1], g2 = t2w[wave][li][a_l][SH];
                 double acc = 0.0;
-                if (recip) {
+                if (p.T1 != nullptr) {
                     // Re sum_h T2[h] X(1, i)^h by Horner from h = HMAX
                     const double2 x = xs[i];
                     double2 t = t2w[wave][li][a_l][HMAX];
