@@ -88,7 +88,8 @@ constexpr int SH = 8;           // h slots per atom: lanes of a wave are (8 atom
 constexpr int HMAX = 6;         // the k-set has h^2 + k^2 + l^2 < 49, so 0 <= h <= 6
 constexpr int SK = 13;          // k, l = -6..6
 constexpr int SOFF = 6;
-constexpr int SBOX = SH * SK;   // T1 entries per (plane, atom)
+constexpr int SHT = HMAX + 1;   // h rows of T1 (h = 0..6)
+constexpr int SBOX = SHT * SK;  // T1 entries per (plane, atom)
 constexpr int SEP_WAVES = 4;
 constexpr int SEP_ATOMS = 8 * SEP_WAVES;     // atoms per work-group, 8 per wave
 constexpr int SEP_THREADS = 64 * SEP_WAVES;
@@ -150,7 +151,7 @@ __device__ __forceinline__ double2 t1_fold_sum(const double2 (&f)[SK], const dou
 }
 
 // Folded T1[m][a][h][.] for planes [blockIdx.y * T1_PLANES, +T1_PLANES); block = atom, lane = (h, k >= 0).
-constexpr int T1_LANES = SH * (KF + 1);
+constexpr int T1_LANES = SHT * (KF + 1);
 __global__ __launch_bounds__(64) void k_sep_t1(int N, int K, int nz, const int *__restrict__ kbox,
                                                const double2 *__restrict__ C, const double2 *__restrict__ ez,
                                                double2 *__restrict__ T1) {
@@ -336,9 +337,9 @@ __global__ __launch_bounds__(SEP_THREADS) void k_sep_phi(SepArgs p) {
     if (p.T1 != nullptr)
         for (int i = threadIdx.x; i < nx; i += SEP_THREADS) xs[i] = p.ex[nx + i];
     __syncthreads();
-    // ---- T2 of every line (lane s = h); the line key is uniform, so the Y row comes through the
+    // ---- T2 of every line (lane s = h < 7); the line key is uniform, so the Y row comes through the
     // scalar cache
-    if (recip) {
+    if (recip && s < SHT) {
         double2 t1[SK];
         int cur_m = -1;
         for (int li = 0; li < n_l; ++li) {

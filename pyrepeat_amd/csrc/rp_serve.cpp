@@ -136,11 +136,11 @@ uint64_t fit_bytes_bound(const Setup &s, int n_atoms, int esp_dtype, int precisi
             at(6, 8 * (uint64_t)c.NP);
             at(9, 8 * (uint64_t)c.n_tiles);
         }
-        // factorized reciprocal sum: k box, T1 per plane (8 x 13 complex per plane and atom), the
+        // factorized reciprocal sum: k box, T1 per plane (7 x 13 complex per plane and atom), the
         // coefficients w e^{-i k.R} per atom and k-vector,
         // line table of the kept points, per-chunk line ranges
         const uint64_t nb = (uint64_t)ceil_div(M, 256), n_chunks = (uint64_t)ceil_div(M, c.rows);
-        sep += 4 * 8 * 13 * 13 + (K > 0 ? 16 * 8 * 13 * (uint64_t)s.nz * N + 16 * (uint64_t)N * K : 0);
+        sep += 4 * 8 * 13 * 13 + (K > 0 ? 16 * 7 * 13 * (uint64_t)s.nz * N + 16 * (uint64_t)N * K : 0);
         sep += 4 * nb + 8 * (nb + 1) + 4 * (M + 1) + 4 * M + 8 * n_chunks + 16 * 7;
         if (precision == RP_PREC_OZAKI_REFINED) {
             // the streamed refinement: the residual e over the kept points and its block sums, the column
