@@ -46,6 +46,16 @@ struct Ctx {
     // the refinement; made on first use, with the events that fork it from and join it to `stream`
     hipStream_t side = nullptr;
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    // Creates the side stream and its fork and join events if they are not there yet.
+    int ensure_side() {
+        if (!side && hipStreamCreateWithFlags(&side, hipStreamNonBlocking) != hipSuccess)
+            return fail(RP_EHIP, "hipStreamCreateWithFlags failed for the side stream");
+        if (!ev_fork && hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming) != hipSuccess)
+            return fail(RP_EHIP, "hipEventCreateWithFlags failed");
+        if (!ev_join && hipEventCreateWithFlags(&ev_join, hipEventDisableTiming) != hipSuccess)
+            return fail(RP_EHIP, "hipEventCreateWithFlags failed");
+        return RP_OK;
+    }
     uint64_t budget = 0;
     uint64_t in_use = 0;
     std::string err;

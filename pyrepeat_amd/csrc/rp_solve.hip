@@ -1855,9 +1855,7 @@ int run_solve(rp_fit *fit, const rp_constraints *con, double *q_out, double *sol
         unsigned *ssync = d_sync.as<unsigned>();
         uint32_t *sbase = &fit->bs_base, *sepoch = &fit->bs_epoch;
         if (refine) {
-            if (!ctx->side) RP_HIP(ctx, hipStreamCreateWithFlags(&ctx->side, hipStreamNonBlocking));
-            if (!ctx->ev_fork) RP_HIP(ctx, hipEventCreateWithFlags(&ctx->ev_fork, hipEventDisableTiming));
-            if (!ctx->ev_join) RP_HIP(ctx, hipEventCreateWithFlags(&ctx->ev_join, hipEventDisableTiming));
+            RP_CHECK(ctx->ensure_side());
             RP_HIP(ctx, hipEventRecord(ctx->ev_fork, st));
             RP_HIP(ctx, hipStreamWaitEvent(ctx->side, ctx->ev_fork, 0));
             ss = ctx->side;
