@@ -327,6 +327,20 @@ __global__ __launch_bounds__(SEP_THREADS) void k_sep_phi(SepArgs p) {
     const bool recip = atom && p.T1 != nullptr;
     const int l_begin = p.l_first + item * SEP_LINES;
     const int n_l = min(SEP_LINES, p.l_end - l_begin);
+    // loads whose latency runs beside the staging: the atom coordinates of the line-term pass and of the
+    // point step, and the grid indices of the first block of rows
+    const int gl = lane >> 3, ga = ablk * SEP_ATOMS + wave * 8 + (lane & 7);
+    const bool geo = e.fit_type != 0 && gl < n_l && ga < p.N;
+    const double Fx = atom ? e.afrac[3 * a] : 0.0;
+    double gFy = 0.0, gFz = 0.0;
+    if (geo) {
+        gFy = e.afrac[3 * ga + 1];
+        gFz = e.afrac[3 * ga + 2];
+    }
+    // rows of the work-group (absolute; they fit in 32 bits)
+    const int rs = (int)max<int64_t>(__builtin_amdgcn_readfirstlane(p.line_row[l_begin]), p.r0);
+    const int rend = (int)min<int64_t>(__builtin_amdgcn_readfirstlane(p.line_row[l_begin + n_l]), p.r0 + p.valid);
+    int f_blk = e.idx[min(rs + lane, rend - 1)];
     if ((int)threadIdx.x <= n_l) lrow[threadIdx.x] = p.line_row[l_begin + threadIdx.x];
     if ((int)threadIdx.x < n_l) {
         const int key = p.line_key[l_begin + threadIdx.x];
@@ -337,6 +351,25 @@ __global__ __launch_bounds__(SEP_THREADS) void k_sep_phi(SepArgs p) {
     if (p.T1 != nullptr)
         for (int i = threadIdx.x; i < nx; i += SEP_THREADS) xs[i] = p.ex[nx + i];
     __syncthreads();
+    // ---- per (line, atom), one lane each: the parts of the nearest-image test and of the displacement
+    // that depend on the line (j, m) only, into entries 7 and 8 of the atom's row (entries the T2 pass
+    // below leaves alone: h <= HMAX = 6):
+    //     (t_x, t_y), (t_z, +-b12),  t = w1 b1 + w2 b2 (rows of the cell, fma as the point step uses),
+    //     b12 = min((1 - |w1|) h1, (1 - |w2|) h2), + when the image (n1, n2) is (0, 0), - otherwise,
+    // so that the point step is left with the terms of grid axis 0 (nearest_only() split in two).
+    {
+        if (geo) {
+            const double d1 = fma((double)lj[gl], e.inv_n[1], -gFy);
+            const double d2 = fma((double)lm[gl], e.inv_n[2], -gFz);
+            const double n1 = rint(d1), n2 = rint(d2);
+            const double w1 = d1 - n1, w2 = d2 - n2;
+            const double b12 = fmin((1.0 - fabs(w1)) * e.h[1], (1.0 - fabs(w2)) * e.h[2]);
+            t2w[wave][gl][lane & 7][SH - 1] =
+                make_double2(fma(w1, e.box[3], w2 * e.box[6]), fma(w1, e.box[4], w2 * e.box[7]));
+            t2w[wave][gl][lane & 7][SH] =
+                make_double2(fma(w1, e.box[5], w2 * e.box[8]), (n1 == 0.0 && n2 == 0.0) ? b12 : -b12);
+        }
+    }
     // ---- T2 of every line (lane s = h < 7); the line key is uniform, so the Y row comes through the
     // scalar cache
     if (recip && s < SHT) {
@@ -361,26 +394,6 @@ __global__ __launch_bounds__(SEP_THREADS) void k_sep_phi(SepArgs p) {
             t2w[wave][li][a_l][s] = t2;
         }
     }
-    // ---- per (line, atom), one lane each: the parts of the nearest-image test and of the displacement
-    // that depend on the line (j, m) only, into entries 7 and 8 of the atom's row (entry 7 held T2 of
-    // h = 7, which is zero: h <= HMAX = 6):
-    //     (t_x, t_y), (t_z, +-b12),  t = w1 b1 + w2 b2 (rows of the cell, fma as the point step uses),
-    //     b12 = min((1 - |w1|) h1, (1 - |w2|) h2), + when the image (n1, n2) is (0, 0), - otherwise,
-    // so that the point step is left with the terms of grid axis 0 (nearest_only() split in two).
-    if (e.fit_type != 0) {
-        const int gl = lane >> 3, ga = ablk * SEP_ATOMS + wave * 8 + (lane & 7);
-        if (gl < n_l && ga < p.N) {
-            const double d1 = fma((double)lj[gl], e.inv_n[1], -e.afrac[3 * ga + 1]);
-            const double d2 = fma((double)lm[gl], e.inv_n[2], -e.afrac[3 * ga + 2]);
-            const double n1 = rint(d1), n2 = rint(d2);
-            const double w1 = d1 - n1, w2 = d2 - n2;
-            const double b12 = fmin((1.0 - fabs(w1)) * e.h[1], (1.0 - fabs(w2)) * e.h[2]);
-            t2w[wave][gl][lane & 7][SH - 1] =
-                make_double2(fma(w1, e.box[3], w2 * e.box[6]), fma(w1, e.box[4], w2 * e.box[7]));
-            t2w[wave][gl][lane & 7][SH] =
-                make_double2(fma(w1, e.box[5], w2 * e.box[8]), (n1 == 0.0 && n2 == 0.0) ? b12 : -b12);
-        }
-    }
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
@@ -390,19 +403,15 @@ __global__ __launch_bounds__(SEP_THREADS) void k_sep_phi(SepArgs p) {
     // (ds_bpermute), so it waits neither on a global load nor on the line table.  Rows are absolute
     // and fit in 32 bits.
     const double alpha = e.sqrt_alpha * e.sqrt_alpha;
-    const int rs = (int)max<int64_t>(lrow[0], p.r0);
-    const int rend = (int)min<int64_t>(lrow[n_l], p.r0 + p.valid);
     double cmax = 0.0;
     bool multi = false;   // a pair of this lane has several images within Rc
     {
-        const double Fx = atom ? e.afrac[3 * a] : 0.0;
         // line boundaries (uniform); slots past the last line never match
         int bnd[SEP_LINES - 1];
 #pragma unroll
         for (int q = 0; q < SEP_LINES - 1; ++q)
             bnd[q] = q + 1 < n_l ? __builtin_amdgcn_readfirstlane(lrow[q + 1]) : 0x7fffffff;
         double *const out0 = p.phi + (int64_t)a * p.rows - p.r0;
-        int f_blk = e.idx[min(rs + lane, rend - 1)];
         // every lane runs the block loop (the packs are exchanged across the whole wave); the atom
         // columns' lanes evaluate the pairs
         for (int rb = rs; rb < rend; rb += 64) {
