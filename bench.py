@@ -411,10 +411,11 @@ def main():
     M, N, K = split["M"], split["N"], split["K"]
     # dominant kernel of the chosen path: the basis (k_sep_phi, the most time per fit), bound by its fp64
     # vector arithmetic.  Its algorithmic fp64 flops per launch, from the factorized sum's units:
-    #   per (grid line, atom)   336: T2 for h = 0..6, folded by conjugate symmetry into T1[0] plus 6 pairs
-    #                                (P_k Re Y + Q_k Im Y): 24 multiply-adds (48 flops) per h
-    #   per (kept point, atom)   86: Re sum_h T2 X^h by Horner (5 complex steps + the last, 44) and the
-    #                                nearest-image test and displacement (42)
+    #   per (grid line, atom)   358: T2 for h = 0..6, folded by conjugate symmetry into T1[0] plus 6 pairs
+    #                                (P_k Re Y + Q_k Im Y): 24 multiply-adds (48 flops) per h, 336; the line's
+    #                                part of the nearest-image test and of the displacement (axes 1 and 2), 22
+    #   per (kept point, atom)   62: Re sum_h T2 X^h by Horner (5 complex steps + the last, 44) and the
+    #                                point's part of the test and displacement (axis 0) with r^2, 18
     #   per real-space term      47: erfc(sqrt(a) r)/r = 1/r - sqrt(a) P(u), P of degree 21 by Horner
     #                                (u, 21 FMAs, the last FMA) plus the reciprocal square root
     # with the terms inside the cutoff counted as the sphere's share of the cell, (4/3) pi Rc^3 / V.
@@ -423,7 +424,7 @@ def main():
     t_launch = ms / launches * 1e-3
     rows, lines = split["rows"], split["lines_per_chunk"]
     frac_in = min(1.0, 4.0 / 3.0 * np.pi * 20.0 ** 3 / split["volume"])
-    alg_fit = 336.0 * sum(lines) * N + 86.0 * M * N + 47.0 * frac_in * M * N
+    alg_fit = 358.0 * sum(lines) * N + 62.0 * M * N + 47.0 * frac_in * M * N
     alg_launch = alg_fit / launches_per_fit
     achieved = alg_launch / t_launch / 1e12
     executed = flops_from_profile("k_sep_phi", args.workload)
@@ -442,8 +443,8 @@ def main():
             "fk_equivalent_tflops": round(4.0 * M * N * K / launches_per_fit / t_launch / 1e12, 1),
             "erfc_terms_per_launch": round(frac_in * M * N / launches_per_fit),
             "erfc_terms_per_s": round(frac_in * M * N / launches_per_fit / t_launch, -6),
-            "note": "achieved = the factorized sum's algorithmic fp64 flops of one launch (336 per line and atom, "
-                    "86 per kept point and atom, 47 per real-space term) / mean launch time (HIP events on the "
+            "note": "achieved = the factorized sum's algorithmic fp64 flops of one launch (358 per line and atom, "
+                    "62 per kept point and atom, 47 per real-space term) / mean launch time (HIP events on the "
                     "fit's stream); executed = rocprof SQ_INSTS_VALU_{FMA,MUL,ADD,TRANS}_F64 x 64 lanes (idle "
                     "lanes included); traffic = calibrated FETCH_SIZE + WRITE_SIZE; fk_equivalent counts "
                     "4 M N K, the reference's cos/sin GEMM formulation, which this kernel does not execute; "
