@@ -185,6 +185,9 @@ __global__ __launch_bounds__(NTHR) void k_oz_gram(GramOperands op, GramEpi ge, i
     // Work-groups are dispatched round-robin over the 8 XCDs (linear id L -> XCD L % 8).  When the
     // row splits divide by 8, XCD x takes splits x, x + 8, ..., and runs all tiles of one split
     // before the next, so the tiles in flight on an XCD share the k-window its L2 holds.
+    // Otherwise (few row splits: a large Gram) XCD x takes a contiguous range of the (tile, split) items,
+    // tile-fastest, and the tile list is ordered in blocks of 8 x 4 tiles (512 x 512 entries), so the tiles in
+    // flight on an XCD read few distinct columns; the grid is then 8 ranges, the last one short.
     const int L = blockIdx.x;
     int tix, split;
     if (n_splits % 8 == 0) {
@@ -192,8 +195,11 @@ __global__ __launch_bounds__(NTHR) void k_oz_gram(GramOperands op, GramEpi ge, i
         tix = j % ge.n_tiles;
         split = (j / ge.n_tiles) * 8 + (L & 7);
     } else {
-        tix = L % ge.n_tiles;
-        split = L / ge.n_tiles;
+        const int items = ge.n_tiles * n_splits, ipx = (items + 7) / 8;
+        const int w = (L & 7) * ipx + (L >> 3);
+        if (w >= items) return;
+        tix = w % ge.n_tiles;
+        split = w / ge.n_tiles;
     }
     const int2 tile = ge.tiles[tix];
     const int64_t m0 = (int64_t)tile.x * TM, n0 = (int64_t)tile.y * TN;
@@ -298,10 +304,13 @@ int accumulate_split_p(rp_fit *fit, FitDevice &dv, bool keep_phi) {
     RP_CHECK(d_colmax.grow(ctx, sizeof(unsigned long long) * NP));
     RP_CHECK(d_scale.grow(ctx, sizeof(double) * NP));
     // Gram tiles (TM x TN) touching the upper triangle, and row splits
+    // upper tiles in blocks of 8 x 4 tiles (512 x 512 entries), row-major inside a block and across blocks
     std::vector<int2> tiles;
-    for (int bi = 0; bi < NP / TM; ++bi)
-        for (int bj = 0; bj < NP / TN; ++bj)
-            if (bi * TM <= bj * TN + TN - 1) tiles.push_back(make_int2(bi, bj));
+    for (int BI = 0; BI < NP / TM; BI += 8)
+        for (int BJ = 0; BJ < NP / TN; BJ += 4)
+            for (int bi = BI; bi < std::min(BI + 8, NP / TM); ++bi)
+                for (int bj = BJ; bj < std::min(BJ + 4, NP / TN); ++bj)
+                    if (bi * TM <= bj * TN + TN - 1) tiles.push_back(make_int2(bi, bj));
     const int n_tiles = (int)tiles.size();
     const int splits = cp.splits;
     if (n_tiles != cp.n_tiles || rows % (splits * gram_kd<P>()) != 0)
@@ -356,7 +365,8 @@ int accumulate_split_p(rp_fit *fit, FitDevice &dv, bool keep_phi) {
         }
         KTimer ktg(ctx, KT_GRAM);
         ge.accumulate = chunk > 0;
-        k_oz_gram<P><<<n_tiles * splits, NTHR, lds_bytes, st>>>(go, ge, rps_c, splits);
+        k_oz_gram<P><<<splits % 8 == 0 ? n_tiles * splits : 8 * (int)ceil_div((int64_t)n_tiles * splits, 8), NTHR,
+                       lds_bytes, st>>>(go, ge, rps_c, splits);
     }
     {
         KTimer ktr(ctx, KT_REDUCE);
