@@ -287,6 +287,14 @@ def serve_run(R, slots, rate, budget_gb, chosen):
     service = np.array([r["t_done"] - r["t_admit"] for _, r in res])
     queue_wait = np.array([r["t_admit"] - r["arrival"] for _, r in res])
     n_atoms = np.array([s.n_atoms for s in systems])
+    # service time of the saturated run by request size: the requests sorted by atom count into eighths
+    # (with one slot the requests run one after another, so these are the sizes' device times)
+    svc_sat = np.array([r["t_done"] - r["t_admit"] for _, r in sat])
+    order = np.argsort(n_atoms, kind="stable")
+    eighths = [order[i * R // 8:(i + 1) * R // 8] for i in range(8)] if R >= 8 else [order]
+    by_size = [{"atoms": [int(n_atoms[e].min()), int(n_atoms[e].max())],
+                "service_ms": round(1e3 * float(svc_sat[e].mean()), 3),
+                "share": round(float(svc_sat[e].sum() / svc_sat.sum()), 3)} for e in eighths]
     line = {
         "metric": f"charge fits/s + p50/p99 latency, serving stream (Poisson {rate:g} req/s, 20-800 atoms)",
         "value": round(R / span, 4), "unit": "fits/s", "n_gpus": 1, "steps": R, "warmup": slots,
@@ -306,6 +314,7 @@ def serve_run(R, slots, rate, budget_gb, chosen):
         "max_running": int(st_sat["max_running"]), "peak_reserved_gb": round(st_sat["peak_reserved"] / 1e9, 3),
         "open_loop_max_running": int(st_open["max_running"]),
         "mix_vs_alone_max_dq_e": dq_alone, "chosen_vs_fp64_max_dq_e": dq_f64, "sample_requests": sample,
+        "saturated_service_by_size": by_size,
     }
     return line
 
