@@ -479,6 +479,24 @@ def main():
                  "note": "achieved = fp64-equivalent Gram flops F_G = M N (N + 1) + 3 M N of the launch's rows / "
                          f"mean launch time (events, breakdown fits); peak = dense int8 MFMA; each fp product costs "
                          f"P^2 = {P * P} int8 slice products, which executed_int8_tops counts"}
+    # the vdW filter phase (mask, scan, compaction with the ESP gather, mean, centring and bounds), bound by
+    # HBM in principle: its algorithmic bytes B_f = n_total (s_V + 1) + M (4 + s_V) (read the ESP, write the
+    # mask byte, write the kept index and the gathered value) over the phase's device time (HIP events
+    # around its kernels, last breakdown fit)
+    n_total = int(np.prod(system.n_grid))
+    s_v = system.esp_cube.dtype.itemsize
+    b_f = n_total * (s_v + 1) + M * (4 + s_v)
+    t_f = split["timings"]["filter_ms"] * 1e-3
+    f_traffic = traffic_from_profile("filter", args.workload)
+    fk = split["kernels"].get("filter")   # k_filter_mask alone: (ms, launches) over the breakdown fits
+    roof_filter = {"bound": "hbm", "kernel": "filter phase (k_filter_mask, k_scan_counts, k_compact, k_mean, "
+                                             "k_center_and_bounds, k_filter_publish)",
+                   "achieved": round(b_f / t_f / 1e9, 1) if t_f > 0 else None, "peak": 8000.0, "unit": "GB/s",
+                   "frac": round(b_f / t_f / 8e12, 4) if t_f > 0 else None, "traffic": f_traffic,
+                   "algorithmic_bytes": int(b_f), "phase_ms": round(1e3 * t_f, 4),
+                   "mask_kernel_ms": round(fk[0] / fk[1], 4) if fk and fk[1] else None,
+                   "note": "compute-bound on the per-point cell-list scan (k_filter_mask); the phase reads the "
+                           "cube once and writes the mask and the kept points"}
     cpu = None
     if not args.no_cpu:
         _log("cpu baseline (oracle sample) ...")
@@ -537,6 +555,7 @@ def main():
         "kernels_chosen": kms, "kernels_fp64": kms64, "kernels_fp64_factorized": per_fit(f64f),
         "roofline": roof,
         "roofline_gram": roof_gram,
+        "roofline_filter": roof_filter,
         "cpu_baseline": cpu,
         "speedup_vs_cpu": round(fits_s / cpu["value"], 1) if cpu else None,
         "batch": batch,
