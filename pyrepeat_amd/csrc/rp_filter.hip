@@ -65,24 +65,24 @@ __global__ __launch_bounds__(kBlock) void k_filter_mask(FilterArgs a, const int 
         const int by = min(max((int)floor((gy - a.bmin[1]) * a.binv), 0), a.bn[1] - 1);
         const int bz = min(max((int)floor((gz - a.bmin[2]) * a.binv), 0), a.bn[2] - 1);
         bool valid = true, near = false;
-        for (int dz = -1; dz <= 1 && valid; ++dz) {
-            const int z2 = bz + dz;
-            if (z2 < 0 || z2 >= a.bn[2]) continue;
-            for (int dy = -1; dy <= 1 && valid; ++dy) {
-                const int y2 = by + dy;
-                if (y2 < 0 || y2 >= a.bn[1]) continue;
-                const int row = (z2 * a.bn[1] + y2) * a.bn[0];
-                const int x0 = max(bx - 1, 0), x1 = min(bx + 1, a.bn[0] - 1);
-                const int beg = bin_start[row + x0], end = bin_start[row + x1 + 1];
-                for (int t = beg; t < end; ++t) {
-                    const double4 at = bin_atoms[t];
-                    const double s = dist2(gx, gy, gz, at);
-                    if (s <= at.w) {
-                        valid = false;
-                        break;
-                    }
-                    if (s <= bin_smax[t]) near = true;
+        // the 9 bin rows around the point, its own row first: a point inside an atom's r_min sphere (most of
+        // the grid) usually finds that atom there and stops (the predicate does not depend on the order)
+        for (int q = 0; q < 9 && valid; ++q) {
+            const int dz = q == 0 ? 0 : (q <= 4 ? (q <= 2 ? 0 : (q == 3 ? -1 : 1)) : (q <= 6 ? -1 : 1));
+            const int dy = q == 0 ? 0 : (q <= 4 ? (q <= 2 ? (q == 1 ? -1 : 1) : 0) : ((q & 1) ? -1 : 1));
+            const int z2 = bz + dz, y2 = by + dy;
+            if (z2 < 0 || z2 >= a.bn[2] || y2 < 0 || y2 >= a.bn[1]) continue;
+            const int row = (z2 * a.bn[1] + y2) * a.bn[0];
+            const int x0 = max(bx - 1, 0), x1 = min(bx + 1, a.bn[0] - 1);
+            const int beg = bin_start[row + x0], end = bin_start[row + x1 + 1];
+            for (int t = beg; t < end; ++t) {
+                const double4 at = bin_atoms[t];
+                const double s = dist2(gx, gy, gz, at);
+                if (s <= at.w) {
+                    valid = false;
+                    break;
                 }
+                if (s <= bin_smax[t]) near = true;
             }
         }
         if (valid && !near && a.fallback) {
