@@ -109,29 +109,7 @@ __global__ __launch_bounds__(kBlock) void k_filter_mask(FilterArgs a, const int 
 
 // Exclusive scan of per-block counts in one workgroup; out[n_blocks] = total.
 __global__ __launch_bounds__(1024) void k_scan_counts(const int *__restrict__ cnt, int64_t n, int64_t *__restrict__ out) {
-    __shared__ int64_t part[1024];
-    const int64_t per = (n + 1023) / 1024;
-    const int64_t b0 = (int64_t)threadIdx.x * per;
-    const int64_t b1 = min(b0 + per, n);
-    int64_t s = 0;
-    for (int64_t b = b0; b < b1; ++b) s += cnt[b];
-    part[threadIdx.x] = s;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        int64_t run = 0;
-        for (int t = 0; t < 1024; ++t) {
-            int64_t v = part[t];
-            part[t] = run;
-            run += v;
-        }
-        out[n] = run;
-    }
-    __syncthreads();
-    int64_t run = part[threadIdx.x];
-    for (int64_t b = b0; b < b1; ++b) {
-        out[b] = run;
-        run += cnt[b];
-    }
+    block_scan_counts_1024(cnt, n, out);
 }
 
 template <class T>

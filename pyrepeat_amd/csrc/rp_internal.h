@@ -148,6 +148,44 @@ struct FilterState {
         if (rc_ != RP_OK) return rc_; \
     } while (0)
 
+// Exclusive scan of n per-block counts by one work-group of 1024 threads: out[b] = cnt[0] + ... + cnt[b-1],
+// out[n] = the total.  Thread t sums its run of ceil(n / 1024) counts; the 1024 run sums are scanned by
+// wave shuffles and a scan of the 16 wave totals; each thread then writes its run's offsets.  Integer
+// sums, so the result does not depend on the order.
+__device__ inline void block_scan_counts_1024(const int *__restrict__ cnt, int64_t n, int64_t *__restrict__ out) {
+    __shared__ int64_t wsum[16];
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    const int64_t per = (n + 1023) / 1024;
+    const int64_t b0 = min((int64_t)t * per, n), b1 = min(b0 + per, n);
+    int64_t s = 0;
+    for (int64_t b = b0; b < b1; ++b) s += cnt[b];
+    int64_t x = s;   // inclusive scan within the wave
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int64_t y = __shfl_up(x, o, 64);
+        if (lane >= o) x += y;
+    }
+    if (lane == 63) wsum[w] = x;
+    __syncthreads();
+    if (t < 64) {
+        int64_t v = t < 16 ? wsum[t] : 0;
+#pragma unroll
+        for (int o = 1; o < 16; o <<= 1) {
+            const int64_t y = __shfl_up(v, o, 64);
+            if (lane >= o) v += y;
+        }
+        if (t < 16) wsum[t] = v;
+    }
+    __syncthreads();
+    const int64_t base = w ? wsum[w - 1] : 0;
+    if (t == 1023) out[n] = base + x;
+    int64_t run = base + x - s;
+    for (int64_t b = b0; b < b1; ++b) {
+        out[b] = run;
+        run += cnt[b];
+    }
+}
+
 inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
 inline int64_t round_up(int64_t a, int64_t b) { return ceil_div(a, b) * b; }
 
