@@ -1123,6 +1123,165 @@ __global__ __launch_bounds__(256) void k_ref_cols_chunk(const double *__restrict
     if (t == 0) r[j] = G - S * (Et / (double)M);
 }
 
+// The residual pass over one chunk of the basis in a single read of it (N <= RESID_MAX_N atoms):
+// e = V - Phi u on the chunk's rows, and the sums g_j = sum_R Phi[j][R] e_R, s_j = sum_R Phi[j][R],
+// E = sum_R e_R that r_j = g_j - s_j E / M needs (k_ref_finish forms r).  Work-group w takes the 16-row
+// blocks w, w + G, ... of the chunk (G work-groups); each block is loaded once into registers: lane
+// (row quad rq = lane & 3, column lane cl = (lane >> 2) + 16 wave) holds rows 4 rq .. 4 rq + 3 of the
+// columns cl + 128 k, so a wave reads 16 whole 128-byte column segments per k.  The row dot products are
+// summed over the column lanes (shuffles, then the 8 waves' partials in order through LDS), e comes
+// back through LDS, and each lane adds its rows' share to its columns' g and s.  The next block's loads
+// are in flight meanwhile.  At the chunk's end the four row-quad lanes' sums are combined and added to
+// the work-group's running sums in acc ([G][2N + 1]: g, s, E), which carry over between chunks, so the
+// kept and the recomputed (streamed) basis give the same sums bitwise.
+constexpr int RESID_THREADS = 512;
+constexpr int RESID_KC = 8;                    // columns per lane
+constexpr int RESID_MAX_N = 128 * RESID_KC;    // 1024
+constexpr int RESID_MAX_G = 256;               // work-groups
+
+__global__ __launch_bounds__(RESID_THREADS) void k_ref_resid(const double *__restrict__ phi, int64_t rows,
+                                                             int64_t valid, int N, const double *__restrict__ v,
+                                                             const double *__restrict__ u, double *__restrict__ acc,
+                                                             int first, const int *__restrict__ cinfo,
+                                                             const RefineState *__restrict__ rs) {
+    __shared__ double red[RESID_THREADS / 64][16];   // [wave][row]: the wave's partial of the row's dot product
+    __shared__ double es[16];
+    if (!refine_active(cinfo, rs)) return;
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    const int rq = lane & 3, cl = (lane >> 2) + 16 * wave;
+    const int G = gridDim.x, w = blockIdx.x;
+    const int64_t n_blk = (valid + 15) / 16;
+    // a lane's columns past the atoms read the last atom's column with weight 0 and are not stored
+    double uk[RESID_KC];
+    const double *colp[RESID_KC];
+#pragma unroll
+    for (int k = 0; k < RESID_KC; ++k) {
+        const int j = cl + 128 * k;
+        uk[k] = j < N ? u[j] : 0.0;
+        colp[k] = phi + (int64_t)min(j, N - 1) * rows + 4 * rq;
+    }
+    double g[RESID_KC], sc[RESID_KC];
+#pragma unroll
+    for (int k = 0; k < RESID_KC; ++k) g[k] = sc[k] = 0.0;
+    double esum = 0.0;   // threads 0..15: running sum of e over their row of every block
+    double nx[RESID_KC][4], vn = 0.0;
+    // the block's rows of this lane's columns (and, threads 0..15, V of its row), none past the block count
+    auto load = [&](int64_t b) {
+        if (b < n_blk) {
+#pragma unroll
+            for (int k = 0; k < RESID_KC; ++k) {
+                const double2 *p = reinterpret_cast<const double2 *>(colp[k] + 16 * b);
+                const double2 a = p[0], c = p[1];
+                nx[k][0] = a.x;
+                nx[k][1] = a.y;
+                nx[k][2] = c.x;
+                nx[k][3] = c.y;
+            }
+            if (t < 16 && 16 * b + t < valid) vn = v[16 * b + t];
+        }
+    };
+    load(w);
+    for (int64_t b = w; b < n_blk; b += G) {
+        // rows past the chunk's kept points count as zero
+        const int64_t r = 16 * b + 4 * rq;
+        double cu[RESID_KC][4];
+#pragma unroll
+        for (int k = 0; k < RESID_KC; ++k)
+#pragma unroll
+            for (int i = 0; i < 4; ++i) cu[k][i] = r + i < valid ? nx[k][i] : 0.0;
+        const double vc = vn;
+        load(b + G);
+        double p[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int k = 0; k < RESID_KC; ++k)
+#pragma unroll
+            for (int i = 0; i < 4; ++i) p[i] = fma(cu[k][i], uk[k], p[i]);
+#pragma unroll
+        for (int o = 4; o < 64; o <<= 1)
+#pragma unroll
+            for (int i = 0; i < 4; ++i) p[i] += __shfl_xor(p[i], o, 64);
+        if (lane < 4) {
+#pragma unroll
+            for (int i = 0; i < 4; ++i) red[wave][4 * lane + i] = p[i];
+        }
+        __syncthreads();
+        if (t < 16) {
+            double d = 0.0;
+#pragma unroll
+            for (int q = 0; q < RESID_THREADS / 64; ++q) d += red[q][t];
+            const double e = 16 * b + t < valid ? vc - d : 0.0;
+            es[t] = e;
+            esum += e;
+        }
+        __syncthreads();
+        double ev[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) ev[i] = es[4 * rq + i];
+#pragma unroll
+        for (int k = 0; k < RESID_KC; ++k) {
+            g[k] = fma(cu[k][0], ev[0], fma(cu[k][1], ev[1], fma(cu[k][2], ev[2], fma(cu[k][3], ev[3], g[k]))));
+            sc[k] += (cu[k][0] + cu[k][1]) + (cu[k][2] + cu[k][3]);
+        }
+    }
+    // the chunk's sums of the work-group: the four row quads of each column, then the running sums
+#pragma unroll
+    for (int k = 0; k < RESID_KC; ++k) {
+        g[k] += __shfl_xor(g[k], 1, 64);
+        sc[k] += __shfl_xor(sc[k], 1, 64);
+        g[k] += __shfl_xor(g[k], 2, 64);
+        sc[k] += __shfl_xor(sc[k], 2, 64);
+    }
+    double *aw = acc + (int64_t)w * (2 * N + 1);
+    if (rq == 0) {
+#pragma unroll
+        for (int k = 0; k < RESID_KC; ++k) {
+            const int j = cl + 128 * k;
+            if (j >= N) continue;
+            aw[j] = first ? g[k] : aw[j] + g[k];
+            aw[N + j] = first ? sc[k] : aw[N + j] + sc[k];
+        }
+    }
+    if (wave == 0) {
+        double E = t < 16 ? esum : 0.0;
+#pragma unroll
+        for (int o = 1; o < 16; o <<= 1) E += __shfl_xor(E, o, 64);
+        if (t == 0) aw[2 * N] = first ? E : aw[2 * N] + E;
+    }
+}
+
+// r_j = g_j - s_j E / M from the work-groups' sums of k_ref_resid: 32 columns per work-group, eight
+// threads per column, thread q summing the work-groups w = q mod 8 in order, then the eight in order.
+__global__ __launch_bounds__(256) void k_ref_finish(const double *__restrict__ acc, int G, int N, int64_t M,
+                                                    double *__restrict__ r, const int *__restrict__ cinfo,
+                                                    const RefineState *__restrict__ rs) {
+    __shared__ double gp[8][32], sp[8][32], ep[8];
+    if (!refine_active(cinfo, rs)) return;
+    const int t = threadIdx.x, q = t >> 5, jj = t & 31, j = blockIdx.x * 32 + jj;
+    const int64_t ld = 2 * N + 1;
+    double gs = 0.0, ss = 0.0, E = 0.0;
+    for (int w = q; w < G; w += 8) {
+        if (j < N) {
+            gs += acc[w * ld + j];
+            ss += acc[w * ld + N + j];
+        }
+        if (jj == 0) E += acc[w * ld + 2 * N];
+    }
+    gp[q][jj] = gs;
+    sp[q][jj] = ss;
+    if (jj == 0) ep[q] = E;
+    __syncthreads();
+    if (t < 32 && j < N) {
+        double g = 0.0, s = 0.0, e = 0.0;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            g += gp[k][t];
+            s += sp[k][t];
+            e += ep[k];
+        }
+        r[j] = g - s * (e / (double)M);
+    }
+}
+
 // r1 = P^T r + rhs_add - diag_add x - w lambda (into r1), r2 = q_tot - w . x; one work-group.
 __global__ __launch_bounds__(256) void k_ref_rhs(const double *__restrict__ r, const int *__restrict__ red_ptr,
                                                  const int *__restrict__ red_src, const double *__restrict__ red_mult,
@@ -1716,36 +1875,57 @@ int run_solve(rp_fit *fit, const rp_constraints *con, double *q_out, double *sol
     RefineState *d_rs = reinterpret_cast<RefineState *>(d_rv.as<double>() + round_up((int64_t)rv_doubles, 2));
     RefineState *d_sens = d_rs + 1;   // gates the sensitivity solve
     const int n_esum = (int)ceil_div(fit->n_keep, 512);
-    if (refine) RP_CHECK(d_e.grow(ctx, sizeof(double) * ((size_t)fit->n_keep + 2 + (size_t)n_esum)));
-    DBuf &d_cacc = fit->ws[15];   // streamed refinement: the column pass's per-thread partial sums
-    if (refine && fit->refine_stream) RP_CHECK(d_cacc.grow(ctx, sizeof(double) * 4 * 256 * (size_t)N));
+    // up to RESID_MAX_N atoms the residual pass is one fused read of each chunk (k_ref_resid), with its
+    // work-groups' running sums in ws[15]; above, e over the kept points and a second read for Phi^T e
+    const bool fused = N <= RESID_MAX_N;
+    const int g_res = (int)std::min<int64_t>(RESID_MAX_G, std::max<int64_t>(fit->phi_rows / 16, 1));
+    if (refine && !fused) RP_CHECK(d_e.grow(ctx, sizeof(double) * ((size_t)fit->n_keep + 2 + (size_t)n_esum)));
+    DBuf &d_cacc = fit->ws[15];   // fused: the work-groups' sums; streamed: the column pass's partial sums
+    if (refine && fused) RP_CHECK(d_cacc.grow(ctx, sizeof(double) * (size_t)g_res * (2 * (size_t)N + 1)));
+    if (refine && !fused && fit->refine_stream) RP_CHECK(d_cacc.grow(ctx, sizeof(double) * 4 * 256 * (size_t)N));
     // One residual pass of the refinement: u = P x, e = V - Phi u over the kept points and r = Phi_c^T e,
     // from the kept basis, or (streamed) from each chunk recomputed by the basis kernel into ws[2].
     auto residual_pass = [&]() {
         const int64_t M = fit->n_keep, rows = fit->phi_rows, plane = (int64_t)NPg * rows;
-        double *e = d_e.as<double>(), *esum = e + M + 2;
+        double *e = fused ? nullptr : d_e.as<double>(), *esum = fused ? nullptr : e + M + 2;
         double *u = d_rv.as<double>(), *r = u + N;
         const int *ci = d_cinfo.as<int>();
         k_ref_expand<<<ceil_div(N, 256), 256, 0, st>>>(d_x.as<double>(), dc_inv_ptr, dc_inv_set, dc_inv_mult, N, u,
                                                        ci, d_rs);
+        const int n_chunks = (int)ceil_div(M, rows);
         if (!fit->refine_stream) {
             const double *phi = fit->ws[7].as<double>();
+            if (fused) {
+                for (int c = 0; c < n_chunks; ++c) {
+                    const int64_t r0 = (int64_t)c * rows, valid = std::min<int64_t>(rows, M - r0);
+                    k_ref_resid<<<g_res, RESID_THREADS, 0, st>>>(phi + c * plane, rows, valid, N,
+                                                                 fit->d_v.as<double>() + r0, u, d_cacc.as<double>(),
+                                                                 c == 0, ci, d_rs);
+                }
+                k_ref_finish<<<ceil_div(N, 32), 256, 0, st>>>(d_cacc.as<double>(), g_res, N, M, r, ci, d_rs);
+                return;
+            }
             k_ref_rows<<<n_esum, 256, 0, st>>>(phi, rows, plane, N, M, fit->d_v.as<double>(), u, e, esum, ci, d_rs);
             k_ref_cols<<<N, 256, 0, st>>>(phi, rows, plane, M, e, esum, n_esum, r, ci, d_rs);
             return;
         }
         FitDevice &dv = *fit->dev;
         double *phi = fit->ws[2].as<double>();
-        const int n_chunks = (int)ceil_div(M, rows);
         for (int c = 0; c < n_chunks; ++c) {
             const int64_t r0 = (int64_t)c * rows, valid = std::min<int64_t>(rows, M - r0);
             sep_phi_chunk(fit, dv, dv.sep_plan, c, r0, valid, rows, phi, fit->ws[5].as<unsigned long long>(), st,
                           &d_rs->go, ci);
+            if (fused) {
+                k_ref_resid<<<g_res, RESID_THREADS, 0, st>>>(phi, rows, valid, N, fit->d_v.as<double>() + r0, u,
+                                                             d_cacc.as<double>(), c == 0, ci, d_rs);
+                continue;
+            }
             k_ref_rows<<<ceil_div(valid, 512), 256, 0, st>>>(phi, rows, plane, N, valid, fit->d_v.as<double>() + r0, u,
                                                              e + r0, esum + r0 / 512, ci, d_rs);
             k_ref_cols_chunk<<<N, 256, 0, st>>>(phi, rows, valid, e + r0, d_cacc.as<double>(), c == 0,
                                                 c == n_chunks - 1, esum, n_esum, M, r, ci, d_rs);
         }
+        if (fused) k_ref_finish<<<ceil_div(N, 32), 256, 0, st>>>(d_cacc.as<double>(), g_res, N, M, r, ci, d_rs);
     };
 
     hipEvent_t e0, e1;
