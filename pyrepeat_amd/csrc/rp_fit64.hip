@@ -303,6 +303,12 @@ __global__ __launch_bounds__(256) void k_kept_matvec(const double *__restrict__ 
     if (R + 1 < M) out[R + 1] = (a1[0] + a1[1]) + (a1[2] + a1[3]);
 }
 
+// Zero fill (a kernel rather than a memset node, so the launch sequence can be captured in a graph).
+__global__ __launch_bounds__(256) void k_zero_f64(double *__restrict__ p, int64_t n) {
+    const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (t < n) p[t] = 0.0;
+}
+
 }  // namespace
 
 // Lower triangle from the upper one (every element with i <= j has been accumulated).
@@ -335,7 +341,7 @@ int accumulate_f64(rp_fit *fit, FitDevice &dv) {
     const int64_t part_n = (int64_t)n_splits * n_tiles * BM * BN;
     if (n_splits > 1) {
         RP_CHECK(d_part.grow(ctx, sizeof(double) * (size_t)part_n));
-        RP_HIP(ctx, hipMemsetAsync(d_part.p, 0, sizeof(double) * (size_t)part_n, st));
+        k_zero_f64<<<ceil_div(part_n, 256), 256, 0, st>>>(d_part.as<double>(), part_n);
     }
     const int rps = (int)(rows / n_splits);
     for (int64_t r0 = 0; r0 < M; r0 += rows) {
@@ -379,10 +385,6 @@ __global__ __launch_bounds__(256) void k_zero_tail(double *__restrict__ phi, int
     if (r < rows) phi[(int64_t)blockIdx.y * rows + r] = 0.0;
 }
 
-__global__ __launch_bounds__(256) void k_zero_f64(double *__restrict__ p, int64_t n) {
-    const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (t < n) p[t] = 0.0;
-}
 
 }  // namespace
 

@@ -1723,7 +1723,7 @@ __global__ __launch_bounds__(256) void k_publish(const double *__restrict__ x, c
 
 int lu_solve_augmented(Ctx *ctx, double *K2, int ld, int n, int *piv, int *info, double *x, hipStream_t st) {
     const int ncols = n + 1;
-    RP_HIP(ctx, hipMemsetAsync(info, 0, sizeof(int), st));
+    k_fill_i32<<<1, 1, 0, st>>>(info, 0);
     for (int k0 = 0; k0 < n; k0 += NB) {
         const int nb = std::min(NB, n - k0);
         k_panel<<<1, 1024, 0, st>>>(K2, ld, n, k0, nb, piv, info);
