@@ -1164,12 +1164,17 @@ __global__ __launch_bounds__(RESID_THREADS) void k_ref_resid(const double *__res
 #pragma unroll
     for (int k = 0; k < RESID_KC; ++k) g[k] = sc[k] = 0.0;
     double esum = 0.0;   // threads 0..15: running sum of e over their row of every block
+    // column slots k < kc hold atoms (the last one partly); the others stay zero and are skipped
+    const int kc = (N + 127) / 128;
     double nx[RESID_KC][4], vn = 0.0;
+#pragma unroll
+    for (int k = 0; k < RESID_KC; ++k) nx[k][0] = nx[k][1] = nx[k][2] = nx[k][3] = 0.0;
     // the block's rows of this lane's columns (and, threads 0..15, V of its row), none past the block count
     auto load = [&](int64_t b) {
         if (b < n_blk) {
 #pragma unroll
             for (int k = 0; k < RESID_KC; ++k) {
+                if (k >= kc) break;
                 const double2 *p = reinterpret_cast<const double2 *>(colp[k] + 16 * b);
                 const double2 a = p[0], c = p[1];
                 nx[k][0] = a.x;
@@ -1193,9 +1198,11 @@ __global__ __launch_bounds__(RESID_THREADS) void k_ref_resid(const double *__res
         load(b + G);
         double p[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-        for (int k = 0; k < RESID_KC; ++k)
+        for (int k = 0; k < RESID_KC; ++k) {
+            if (k >= kc) break;
 #pragma unroll
             for (int i = 0; i < 4; ++i) p[i] = fma(cu[k][i], uk[k], p[i]);
+        }
 #pragma unroll
         for (int o = 4; o < 64; o <<= 1)
 #pragma unroll
@@ -1219,6 +1226,7 @@ __global__ __launch_bounds__(RESID_THREADS) void k_ref_resid(const double *__res
         for (int i = 0; i < 4; ++i) ev[i] = es[4 * rq + i];
 #pragma unroll
         for (int k = 0; k < RESID_KC; ++k) {
+            if (k >= kc) break;
             g[k] = fma(cu[k][0], ev[0], fma(cu[k][1], ev[1], fma(cu[k][2], ev[2], fma(cu[k][3], ev[3], g[k]))));
             sc[k] += (cu[k][0] + cu[k][1]) + (cu[k][2] + cu[k][3]);
         }
