@@ -174,8 +174,9 @@ struct GramEpi {
     const double *scale;   // per column of Phi (atom)
     const int2 *tiles;     // (tile row in units of TM, tile col in units of TN)
     int n_tiles;
-    double *part;          // [split][tile][TM*TN]
+    double *part;          // [split][tile][TM*TN], then the second halves of halved items [8][halve][TM*TN]
     int accumulate;        // 0: the chunk's partial overwrites the slot (first chunk), 1: adds to it
+    int halve;             // items per XCD run as two half-k items (the last ones of the XCD's range)
 };
 
 // All P^2 products of the P-slice operands, 2P - 1 exact groups s = p + q.
@@ -188,12 +189,22 @@ __global__ __launch_bounds__(NTHR) void k_oz_gram(GramOperands op, GramEpi ge, i
     // Otherwise (few row splits: a large Gram) XCD x takes a contiguous range of the (tile, split) items,
     // tile-fastest, and the tile list is ordered in blocks of 8 x 4 tiles (512 x 512 entries), so the tiles in
     // flight on an XCD read few distinct columns; the grid is then 8 ranges, the last one short.
+    // Balanced last round: when the XCD's items leave a partial last round of at most half its CUs, its
+    // last `halve` items run as two half-k items each (the second half into a slot of its own), so that
+    // round is whole.
     const int L = blockIdx.x;
-    int tix, split;
+    int tix, split, half = -1, item = 0;
     if (n_splits % 8 == 0) {
+        const int n_items = ge.n_tiles * (n_splits / 8);   // per XCD
         const int j = L >> 3;
-        tix = j % ge.n_tiles;
-        split = (j / ge.n_tiles) * 8 + (L & 7);
+        item = j;
+        if (ge.halve > 0 && j >= n_items - ge.halve) {
+            const int h = j - (n_items - ge.halve);
+            item = n_items - ge.halve + (h >> 1);
+            half = h & 1;
+        }
+        tix = item % ge.n_tiles;
+        split = (item / ge.n_tiles) * 8 + (L & 7);
     } else {
         const int items = ge.n_tiles * n_splits, ipx = (items + 7) / 8;
         const int w = (L & 7) * ipx + (L >> 3);
@@ -203,12 +214,20 @@ __global__ __launch_bounds__(NTHR) void k_oz_gram(GramOperands op, GramEpi ge, i
     }
     const int2 tile = ge.tiles[tix];
     const int64_t m0 = (int64_t)tile.x * TM, n0 = (int64_t)tile.y * TN;
-    // this split covers k (= grid rows) [split * rps, (split+1) * rps)
+    // this split covers k (= grid rows) [split * rps, (split+1) * rps); a halved item's first half the
+    // whole stages below the middle, its second half the rest
+    int k0 = split * rows_per_split, k1 = k0 + rows_per_split;
+    if (half >= 0) {
+        const int mid = k0 + (rows_per_split / 2) / gram_kd<P>() * gram_kd<P>();
+        if (half == 0) k1 = mid; else k0 = mid;
+    }
     ivec16 acc[2 * P - 1];
-    gram_mainloop_dma<P>(op, m0, n0, split * rows_per_split, (split + 1) * rows_per_split, acc, lds);
+    gram_mainloop_dma<P>(op, m0, n0, k0, k1, acc, lds);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int wm = wave >> 2, wn = wave & 3;
-    double *dst = ge.part + ((int64_t)split * ge.n_tiles + tix) * (TM * TN);
+    double *dst = half == 1 ? ge.part + ((int64_t)n_splits * ge.n_tiles + (L & 7) * ge.halve +
+                                         (item - (ge.n_tiles * (n_splits / 8) - ge.halve))) * (TM * TN)
+                            : ge.part + ((int64_t)split * ge.n_tiles + tix) * (TM * TN);
     const int c = wn * 32 + M32::d_col(lane);
     const double sc = ge.scale[n0 + c];
 #pragma unroll
@@ -226,17 +245,23 @@ __global__ __launch_bounds__(NTHR) void k_oz_gram(GramOperands op, GramEpi ge, i
 
 // Sum of the row-split partials into the Gram: block (tile, part of REDUCE_PARTS), one entry per
 // thread, the splits summed in order.
+// The halved items are the last `halve` tiles (halve <= n_tiles) of the last 8 row splits, one per XCD;
+// their second halves (slots after the split partials, [XCD][halve]) are summed after the splits.
 constexpr int REDUCE_PARTS = TM * TN / 1024;
 __global__ __launch_bounds__(256) void k_oz_reduce(const double *__restrict__ part, int n_splits, int n_tiles,
-                                                   const int2 *__restrict__ tiles, double *__restrict__ gram, int NP) {
+                                                   const int2 *__restrict__ tiles, double *__restrict__ gram, int NP,
+                                                   int halve) {
     const int tix = blockIdx.x / REDUCE_PARTS;
     const int2 tile = tiles[tix];
     const int e0 = (blockIdx.x % REDUCE_PARTS) * 1024;
+    const int hx = tix - (n_tiles - halve);   // this tile's index among the halved ones, if it is one
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
         const int e = e0 + u * 256 + threadIdx.x;
         double s = 0.0;
         for (int sp = 0; sp < n_splits; ++sp) s += part[((int64_t)sp * n_tiles + tix) * (TM * TN) + e];
+        if (halve > 0 && hx >= 0)
+            for (int x = 0; x < 8; ++x) s += part[((int64_t)n_splits * n_tiles + x * halve + hx) * (TM * TN) + e];
         gram[(int64_t)(tile.x * TM + e / TN) * NP + tile.y * TN + (e % TN)] += s;
     }
 }
@@ -328,7 +353,16 @@ int accumulate_split_p(rp_fit *fit, FitDevice &dv, bool keep_phi) {
         RP_HIP(ctx, hipStreamSynchronize(st));   // the host vector goes out of scope
         fit->split_tiles_np = NP;
     }
-    RP_CHECK(d_part.grow(ctx, sizeof(double) * (size_t)splits * n_tiles * TM * TN));
+    // items per XCD and its CUs: a partial last round of at most half the CUs is made whole by halving
+    // its items (their second halves in 8 * halve more partial slots)
+    int halve = 0;
+    if (splits % 8 == 0) {
+        const int cu_xcd = std::max(1, ctx->prop.multiProcessorCount / 8);
+        const int n_items = n_tiles * (splits / 8);
+        const int rem = n_items % cu_xcd;
+        if (rem > 0 && 2 * rem <= cu_xcd && rem <= 64 && rem <= n_tiles && rps / 2 >= gram_kd<P>()) halve = rem;
+    }
+    RP_CHECK(d_part.grow(ctx, sizeof(double) * ((size_t)splits * n_tiles + 8 * (size_t)halve) * TM * TN));
     // the basis: every chunk kept (refined, when it fits) or one chunk buffer
     const bool refined = keep_phi;
     if (refined) {
@@ -343,7 +377,7 @@ int accumulate_split_p(rp_fit *fit, FitDevice &dv, bool keep_phi) {
     RP_HIP(ctx, hipFuncSetAttribute(reinterpret_cast<const void *>(&k_oz_gram<P>),
                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes));
     GramOperands go{d_slices.as<int8_t>(), (int64_t)NP * rows, rows};
-    GramEpi ge{d_scale.as<double>(), d_tiles.as<int2>(), n_tiles, d_part.as<double>()};
+    GramEpi ge{d_scale.as<double>(), d_tiles.as<int2>(), n_tiles, d_part.as<double>(), 0, halve};
     fit->kspace_products = 0;
 
     int chunk = 0;
@@ -365,13 +399,13 @@ int accumulate_split_p(rp_fit *fit, FitDevice &dv, bool keep_phi) {
         }
         KTimer ktg(ctx, KT_GRAM);
         ge.accumulate = chunk > 0;
-        k_oz_gram<P><<<splits % 8 == 0 ? n_tiles * splits : 8 * (int)ceil_div((int64_t)n_tiles * splits, 8), NTHR,
-                       lds_bytes, st>>>(go, ge, rps_c, splits);
+        k_oz_gram<P><<<splits % 8 == 0 ? n_tiles * splits + 8 * halve : 8 * (int)ceil_div((int64_t)n_tiles * splits, 8),
+                       NTHR, lds_bytes, st>>>(go, ge, rps_c, splits);
     }
     {
         KTimer ktr(ctx, KT_REDUCE);
         k_oz_reduce<<<n_tiles * REDUCE_PARTS, 256, 0, st>>>(d_part.as<double>(), splits, n_tiles, d_tiles.as<int2>(),
-                                              fit->d_gram.as<double>(), NP);
+                                              fit->d_gram.as<double>(), NP, halve);
     }
     RP_HIP(ctx, hipGetLastError());
     fit->phi_kept = keep_phi;

@@ -131,7 +131,10 @@ uint64_t fit_bytes_bound(const Setup &s, int n_atoms, int esp_dtype, int precisi
         if (precision == RP_PREC_FP64_FACT) {
             at(3, 8 * split_cap(c.rows, c.n_tiles) * c.n_tiles * 128 * 128);   // fp64 Gram partials
         } else {
-            at(3, 8 * split_cap(c.rows, c.n_tiles) * c.n_tiles * 64 * 128);
+            // the row-split partials and the second halves of the items that balance the last round
+            // (at most 8 x 64 of them: half the CUs of an XCD, on devices of up to 1024 CUs)
+            const uint64_t slots = split_cap(c.rows, c.n_tiles) * c.n_tiles;
+            at(3, 8 * (slots + std::min<uint64_t>(slots, 8 * 64)) * 64 * 128);
             at(4, (uint64_t)c.slices * c.NP * c.rows);
             at(6, 8 * (uint64_t)c.NP);
             at(9, 8 * (uint64_t)c.n_tiles);
