@@ -247,23 +247,35 @@ __global__ __launch_bounds__(NTHR) void k_oz_gram(GramOperands op, GramEpi ge, i
 // thread, the splits summed in order.
 // The halved items are the last `halve` tiles (halve <= n_tiles) of the last 8 row splits, one per XCD;
 // their second halves (slots after the split partials, [XCD][halve]) are summed after the splits.
-constexpr int REDUCE_PARTS = TM * TN / 1024;
+// One entry per thread; the splits' loads are issued 8 at a time ahead of their (in-order) additions.
+constexpr int REDUCE_PARTS = TM * TN / 256;
 __global__ __launch_bounds__(256) void k_oz_reduce(const double *__restrict__ part, int n_splits, int n_tiles,
                                                    const int2 *__restrict__ tiles, double *__restrict__ gram, int NP,
                                                    int halve) {
     const int tix = blockIdx.x / REDUCE_PARTS;
     const int2 tile = tiles[tix];
-    const int e0 = (blockIdx.x % REDUCE_PARTS) * 1024;
+    const int e = (blockIdx.x % REDUCE_PARTS) * 256 + threadIdx.x;
     const int hx = tix - (n_tiles - halve);   // this tile's index among the halved ones, if it is one
+    const double *src = part + (int64_t)tix * (TM * TN) + e;
+    const int64_t stride = (int64_t)n_tiles * (TM * TN);
+    double s = 0.0;
+    int sp = 0;
+    for (; sp + 8 <= n_splits; sp += 8) {
+        double v[8];
 #pragma unroll
-    for (int u = 0; u < 4; ++u) {
-        const int e = e0 + u * 256 + threadIdx.x;
-        double s = 0.0;
-        for (int sp = 0; sp < n_splits; ++sp) s += part[((int64_t)sp * n_tiles + tix) * (TM * TN) + e];
-        if (halve > 0 && hx >= 0)
-            for (int x = 0; x < 8; ++x) s += part[((int64_t)n_splits * n_tiles + x * halve + hx) * (TM * TN) + e];
-        gram[(int64_t)(tile.x * TM + e / TN) * NP + tile.y * TN + (e % TN)] += s;
+        for (int q = 0; q < 8; ++q) v[q] = src[(sp + q) * stride];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) s += v[q];
     }
+    for (; sp < n_splits; ++sp) s += src[sp * stride];
+    if (halve > 0 && hx >= 0) {
+        double v[8];
+#pragma unroll
+        for (int x = 0; x < 8; ++x) v[x] = part[((int64_t)n_splits * n_tiles + x * halve + hx) * (TM * TN) + e];
+#pragma unroll
+        for (int x = 0; x < 8; ++x) s += v[x];
+    }
+    gram[(int64_t)(tile.x * TM + e / TN) * NP + tile.y * TN + (e % TN)] += s;
 }
 
 __global__ void k_zero_u64(unsigned long long *p, int n) {
