@@ -1125,21 +1125,23 @@ __global__ __launch_bounds__(256) void k_ref_cols_chunk(const double *__restrict
 
 // The residual pass over one chunk of the basis in a single read of it (N <= RESID_MAX_N atoms):
 // e = V - Phi u on the chunk's rows, and the sums g_j = sum_R Phi[j][R] e_R, s_j = sum_R Phi[j][R],
-// E = sum_R e_R that r_j = g_j - s_j E / M needs (k_ref_finish forms r).  Work-group w takes the 16-row
-// blocks w, w + G, ... of the chunk (G work-groups); each block is loaded once into registers: lane
-// (row quad rq = lane & 3, column lane cl = (lane >> 2) + 16 wave) holds rows 4 rq .. 4 rq + 3 of the
-// columns cl + 128 k, so a wave reads 16 whole 128-byte column segments per k.  The row dot products are
-// summed over the column lanes (shuffles, then the 8 waves' partials in order through LDS), e comes
-// back through LDS, and each lane adds its rows' share to its columns' g and s.  The next block's loads
-// are in flight meanwhile.  At the chunk's end the four row-quad lanes' sums are combined and added to
-// the work-group's running sums in acc ([G][2N + 1]: g, s, E), which carry over between chunks, so the
-// kept and the recomputed (streamed) basis give the same sums bitwise.
+// E = sum_R e_R that r_j = g_j - s_j E / M needs (k_ref_finish forms r).  Work-group w takes the passes
+// w, w + G, ... of the chunk (G work-groups; a pass is one or more 16-row blocks, below); each block is
+// loaded once into registers: lane (row quad rq = lane & 3, column lane cl = (lane >> 2) + 16 wave) holds
+// rows 4 rq .. 4 rq + 3 of the columns cl + 128 k, so a wave reads 16 whole 128-byte column segments per
+// k.  The row dot products are summed over the column lanes (shuffles, then the 8 waves' partials in order
+// through LDS), e comes back through LDS, and each lane adds its rows' share to its columns' g and s.  The
+// next pass's loads are in flight meanwhile.  At the chunk's end a column's blocks and its four row-quad
+// lanes' sums are combined and added to the work-group's running sums in acc ([G][2N + 1]: g, s, E),
+// which carry over between chunks, so the kept and the recomputed (streamed) basis give the same sums
+// bitwise.
 constexpr int RESID_THREADS = 512;
-constexpr int RESID_KC = 8;                    // columns per lane
+constexpr int RESID_KC = 8;                    // column slots per lane
 constexpr int RESID_MAX_N = 128 * RESID_KC;    // 1024
 constexpr int RESID_MAX_G = 256;               // work-groups
 
-__global__ __launch_bounds__(RESID_THREADS) void k_ref_resid(const double *__restrict__ phi, int64_t rows,
+// More than 512 atoms: one 16-row block per pass, eight column groups.
+__global__ __launch_bounds__(RESID_THREADS) void k_ref_resid_wide(const double *__restrict__ phi, int64_t rows,
                                                              int64_t valid, int N, const double *__restrict__ v,
                                                              const double *__restrict__ u, double *__restrict__ acc,
                                                              int first, const int *__restrict__ cinfo,
@@ -1255,6 +1257,165 @@ __global__ __launch_bounds__(RESID_THREADS) void k_ref_resid(const double *__res
         for (int o = 1; o < 16; o <<= 1) E += __shfl_xor(E, o, 64);
         if (t == 0) aw[2 * N] = first ? E : aw[2 * N] + E;
     }
+}
+
+// Up to 512 atoms, KCL column groups of 128 atoms (a power of two, 128 KCL >= N): a lane's 8 slots hold its KCL columns of
+// RB = 8 / KCL consecutive 16-row blocks (slot k: block k / KCL, column group k % KCL), so with few atoms
+// one pass of the loop covers 16 RB rows and its two barriers serve them all.
+template <int KCL>
+__global__ __launch_bounds__(RESID_THREADS) void k_ref_resid(const double *__restrict__ phi, int64_t rows,
+                                                             int64_t valid, int N, const double *__restrict__ v,
+                                                             const double *__restrict__ u, double *__restrict__ acc,
+                                                             int first, const int *__restrict__ cinfo,
+                                                             const RefineState *__restrict__ rs) {
+    constexpr int RB = RESID_KC / KCL;   // row blocks per pass
+    __shared__ double red[RESID_THREADS / 64][16 * RB];   // [wave][row]: the wave's partial of the row's product
+    __shared__ double es[16 * RB];
+    if (!refine_active(cinfo, rs)) return;
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    const int rq = lane & 3, cl = (lane >> 2) + 16 * wave;
+    const int G = gridDim.x, w = blockIdx.x;
+    const int64_t n_blk = (valid + 15) / 16, n_pass = (n_blk + RB - 1) / RB;
+    // column groups past the atoms (gg >= kc) are neither loaded nor summed; a lane's column past the atoms
+    // reads the last atom's column with weight 0 and is not stored
+    const int kc = (N + 127) / 128;
+    double uk[KCL];
+    const double *colp[KCL];
+#pragma unroll
+    for (int gg = 0; gg < KCL; ++gg) {
+        const int j = cl + 128 * gg;
+        uk[gg] = j < N ? u[j] : 0.0;
+        colp[gg] = phi + (int64_t)min(j, N - 1) * rows + 4 * rq;
+    }
+    double g[RESID_KC], sc[RESID_KC];
+#pragma unroll
+    for (int k = 0; k < RESID_KC; ++k) g[k] = sc[k] = 0.0;
+    double esum = 0.0;   // threads below 16 RB: running sum of e over their row of every pass
+    double nx[RESID_KC][4], vn = 0.0;
+#pragma unroll
+    for (int k = 0; k < RESID_KC; ++k) nx[k][0] = nx[k][1] = nx[k][2] = nx[k][3] = 0.0;
+    // a pass's rows of this lane's columns (and, threads below 16 RB, V of their row)
+    auto load = [&](int64_t ps) {
+        if (ps < n_pass) {
+#pragma unroll
+            for (int k = 0; k < RESID_KC; ++k) {
+                const int b = k / KCL, gg = k % KCL;
+                const int64_t blk = RB * ps + b;
+                if (gg >= kc || blk >= n_blk) continue;
+                const double2 *p = reinterpret_cast<const double2 *>(colp[gg] + 16 * blk);
+                const double2 a = p[0], c = p[1];
+                nx[k][0] = a.x;
+                nx[k][1] = a.y;
+                nx[k][2] = c.x;
+                nx[k][3] = c.y;
+            }
+            if (t < 16 * RB && 16 * RB * ps + t < valid) vn = v[16 * RB * ps + t];
+        }
+    };
+    load(w);
+    for (int64_t ps = w; ps < n_pass; ps += G) {
+        // rows past the chunk's kept points count as zero
+        double cu[RESID_KC][4];
+#pragma unroll
+        for (int k = 0; k < RESID_KC; ++k) {
+            const int64_t r = 16 * (RB * ps + k / KCL) + 4 * rq;
+#pragma unroll
+            for (int i = 0; i < 4; ++i) cu[k][i] = r + i < valid ? nx[k][i] : 0.0;
+        }
+        const double vc = vn;
+        load(ps + G);
+#pragma unroll
+        for (int b = 0; b < RB; ++b) {
+            double p[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+            for (int gg = 0; gg < KCL; ++gg) {
+                if (gg >= kc) break;
+#pragma unroll
+                for (int i = 0; i < 4; ++i) p[i] = fma(cu[b * KCL + gg][i], uk[gg], p[i]);
+            }
+#pragma unroll
+            for (int o = 4; o < 64; o <<= 1)
+#pragma unroll
+                for (int i = 0; i < 4; ++i) p[i] += __shfl_xor(p[i], o, 64);
+            if (lane < 4) {
+#pragma unroll
+                for (int i = 0; i < 4; ++i) red[wave][16 * b + 4 * lane + i] = p[i];
+            }
+        }
+        __syncthreads();
+        if (t < 16 * RB) {
+            double d = 0.0;
+#pragma unroll
+            for (int q = 0; q < RESID_THREADS / 64; ++q) d += red[q][t];
+            const double e = 16 * RB * ps + t < valid ? vc - d : 0.0;
+            es[t] = e;
+            esum += e;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int b = 0; b < RB; ++b) {
+            double ev[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) ev[i] = es[16 * b + 4 * rq + i];
+#pragma unroll
+            for (int gg = 0; gg < KCL; ++gg) {
+                if (gg >= kc) break;
+                const int k = b * KCL + gg;
+                g[k] = fma(cu[k][0], ev[0], fma(cu[k][1], ev[1], fma(cu[k][2], ev[2], fma(cu[k][3], ev[3], g[k]))));
+                sc[k] += (cu[k][0] + cu[k][1]) + (cu[k][2] + cu[k][3]);
+            }
+        }
+    }
+    // the chunk's sums of the work-group: a column's blocks in order, its four row quads, then the running sums
+    double gs[KCL], ss[KCL];
+#pragma unroll
+    for (int gg = 0; gg < KCL; ++gg) {
+        gs[gg] = g[gg];
+        ss[gg] = sc[gg];
+#pragma unroll
+        for (int b = 1; b < RB; ++b) {
+            gs[gg] += g[b * KCL + gg];
+            ss[gg] += sc[b * KCL + gg];
+        }
+        gs[gg] += __shfl_xor(gs[gg], 1, 64);
+        ss[gg] += __shfl_xor(ss[gg], 1, 64);
+        gs[gg] += __shfl_xor(gs[gg], 2, 64);
+        ss[gg] += __shfl_xor(ss[gg], 2, 64);
+    }
+    double *aw = acc + (int64_t)w * (2 * N + 1);
+    if (rq == 0) {
+#pragma unroll
+        for (int gg = 0; gg < KCL; ++gg) {
+            const int j = cl + 128 * gg;
+            if (j >= N) continue;
+            aw[j] = first ? gs[gg] : aw[j] + gs[gg];
+            aw[N + j] = first ? ss[gg] : aw[N + j] + ss[gg];
+        }
+    }
+    // the e sums of the rows, in order (the red array is free after the loop's second barrier)
+    __syncthreads();
+    if (t < 16 * RB) red[0][t] = esum;
+    __syncthreads();
+    if (t == 0) {
+        double E = 0.0;
+        for (int q = 0; q < 16 * RB; ++q) E += red[0][q];
+        aw[2 * N] = first ? E : aw[2 * N] + E;
+    }
+}
+
+// The residual kernel for N atoms: up to 512, column groups rounded up to a power of two.
+inline void launch_ref_resid(int N, int G, hipStream_t st, const double *phi, int64_t rows, int64_t valid,
+                             const double *v, const double *u, double *acc, int first, const int *cinfo,
+                             const RefineState *rs) {
+    const int kc = (N + 127) / 128;
+    if (kc <= 1)
+        k_ref_resid<1><<<G, RESID_THREADS, 0, st>>>(phi, rows, valid, N, v, u, acc, first, cinfo, rs);
+    else if (kc <= 2)
+        k_ref_resid<2><<<G, RESID_THREADS, 0, st>>>(phi, rows, valid, N, v, u, acc, first, cinfo, rs);
+    else if (kc <= 4)
+        k_ref_resid<4><<<G, RESID_THREADS, 0, st>>>(phi, rows, valid, N, v, u, acc, first, cinfo, rs);
+    else
+        k_ref_resid_wide<<<G, RESID_THREADS, 0, st>>>(phi, rows, valid, N, v, u, acc, first, cinfo, rs);
 }
 
 // r_j = g_j - s_j E / M from the work-groups' sums of k_ref_resid: 32 columns per work-group, eight
@@ -1906,9 +2067,8 @@ int run_solve(rp_fit *fit, const rp_constraints *con, double *q_out, double *sol
             if (fused) {
                 for (int c = 0; c < n_chunks; ++c) {
                     const int64_t r0 = (int64_t)c * rows, valid = std::min<int64_t>(rows, M - r0);
-                    k_ref_resid<<<g_res, RESID_THREADS, 0, st>>>(phi + c * plane, rows, valid, N,
-                                                                 fit->d_v.as<double>() + r0, u, d_cacc.as<double>(),
-                                                                 c == 0, ci, d_rs);
+                    launch_ref_resid(N, g_res, st, phi + c * plane, rows, valid, fit->d_v.as<double>() + r0, u,
+                                     d_cacc.as<double>(), c == 0, ci, d_rs);
                 }
                 k_ref_finish<<<ceil_div(N, 32), 256, 0, st>>>(d_cacc.as<double>(), g_res, N, M, r, ci, d_rs);
                 return;
@@ -1924,8 +2084,8 @@ int run_solve(rp_fit *fit, const rp_constraints *con, double *q_out, double *sol
             sep_phi_chunk(fit, dv, dv.sep_plan, c, r0, valid, rows, phi, fit->ws[5].as<unsigned long long>(), st,
                           &d_rs->go, ci);
             if (fused) {
-                k_ref_resid<<<g_res, RESID_THREADS, 0, st>>>(phi, rows, valid, N, fit->d_v.as<double>() + r0, u,
-                                                             d_cacc.as<double>(), c == 0, ci, d_rs);
+                launch_ref_resid(N, g_res, st, phi, rows, valid, fit->d_v.as<double>() + r0, u, d_cacc.as<double>(),
+                                 c == 0, ci, d_rs);
                 continue;
             }
             k_ref_rows<<<ceil_div(valid, 512), 256, 0, st>>>(phi, rows, plane, N, valid, fit->d_v.as<double>() + r0, u,
