@@ -421,8 +421,9 @@ int accumulate_f64_factorized(rp_fit *fit, FitDevice &dv) {
         const int64_t valid = std::min<int64_t>(rows, M - r0);
         {
             KTimer kt(ctx, KT_KSPACE);
+            // the fp64 Gram reads the padding columns: they are written as zeros
             sep_phi_chunk(fit, dv, plan, chunk, r0, valid, rows, d_phi.as<double>(),
-                          d_colmax.as<unsigned long long>(), st);
+                          d_colmax.as<unsigned long long>(), st, nullptr, nullptr, true);
         }
         if (valid < rows) {
             dim3 gz((unsigned)ceil_div(rows - valid, 256), (unsigned)NP);

@@ -238,15 +238,18 @@ int accumulate_f64_factorized(rp_fit *fit, FitDevice &dv);
 // Per accumulate: the k-index box, T1 for every plane, and the line table of the kept points.
 int sep_prepare(rp_fit *fit, FitDevice &dv, int64_t rows, SepPlan &plan);
 // Phi rows [r0, r0 + valid) of one chunk (k-space + real space + augmented columns), column-major
-// [NP][rows] fp64, with the column maxima of |Phi| (bit patterns) into colmax.
+// [NP][rows] fp64, with the column maxima of |Phi| (bit patterns) into colmax.  The padding columns
+// [N + 2, NP) are written as zeros only when zero_pad is set (a consumer that reads them); otherwise they
+// are left as they are and no consumer reads them.
 // gate (nullable): the launch does nothing unless gate[0] != 0 and gate_flags[0] == gate_flags[1] == 0 (a
 // refinement step the device has decided to skip).
 void sep_phi_chunk(rp_fit *fit, FitDevice &dv, const SepPlan &plan, int chunk, int64_t r0, int64_t valid,
                    int64_t rows, double *phi, unsigned long long *colmax, hipStream_t st, const int *gate = nullptr,
-                   const int *gate_flags = nullptr);
+                   const int *gate_flags = nullptr, bool zero_pad = false);
 // Column-major Phi chunk -> int8 slices [P][NP][rows] (P = 3 or 4) with per-column power-of-two scales,
-// rows [0, span) written (span a multiple of 16, at most rows; rows past `valid` as zeros).
-void sep_slice_chunk(const double *phi, int NP, int64_t rows, int64_t valid, int64_t span,
+// rows [0, span) written (span a multiple of 16, at most rows; rows past `valid` as zeros); the columns
+// from ncols on (the padding) are zero slices, their Phi not read.
+void sep_slice_chunk(const double *phi, int NP, int ncols, int64_t rows, int64_t valid, int64_t span,
                      const unsigned long long *colmax, int8_t *slices, double *scale_out, int P, hipStream_t st);
 __global__ void k_symmetrize(double *gram, int NP, int tile);
 

@@ -267,6 +267,7 @@ struct SepArgs {
     int64_t r0, valid;
     const double2 *T1;          // null when there is no reciprocal part (fit-type 0)
     int N, NP;
+    int ncols;                  // columns written: N + 2 (atoms, V, ones), or NP with the zero padding
     const double2 *ex, *ey;
     EpiArgs e;
     double *phi;                // [NP][rows]: column-major, chunk-local rows
@@ -465,8 +466,8 @@ __global__ __launch_bounds__(SEP_THREADS) void k_sep_phi(SepArgs p) {
             }
         }
     }
-    if (!atom && a < p.NP) {
-        // the V column, the ones column and the zero padding of the atom columns
+    if (!atom && a < p.ncols) {
+        // the V column, the ones column and (when asked for) the zero padding of the atom columns
         const int64_t r1 = rs + s;
         double *out = p.phi + (int64_t)a * p.rows + (r1 - p.r0);
         for (int64_t r = r1; r < rend; r += SH, out += SH) {
@@ -558,7 +559,7 @@ __global__ void k_zero_flag(int *f) { *f = 0; }
 // 16-byte write per slice).
 template <int P>
 __global__ __launch_bounds__(256) void k_slice_cols(const double *__restrict__ phi, int64_t rows, int64_t valid,
-                                                    int64_t span,
+                                                    int64_t span, int ncols,
                                                     const unsigned long long *__restrict__ colmax,
                                                     int8_t *__restrict__ out, double *__restrict__ scale_out) {
     const int a = blockIdx.y;
@@ -572,7 +573,10 @@ __global__ __launch_bounds__(256) void k_slice_cols(const double *__restrict__ p
     const double inv = 1.0 / S;
     const double *src = phi + (int64_t)a * rows + r0;
     double v[16];
-    if (r0 + 16 <= valid) {
+    if (a >= ncols) {   // padding column: zero slices, its Phi entries are not written
+#pragma unroll
+        for (int q = 0; q < 16; ++q) v[q] = 0.0;
+    } else if (r0 + 16 <= valid) {
 #pragma unroll
         for (int q = 0; q < 8; ++q) {
             const double2 t = reinterpret_cast<const double2 *>(src)[q];
@@ -683,7 +687,8 @@ int sep_prepare(rp_fit *fit, FitDevice &dv, int64_t rows, SepPlan &plan) {
 }
 
 void sep_phi_chunk(rp_fit *fit, FitDevice &dv, const SepPlan &plan, int chunk, int64_t r0, int64_t valid, int64_t rows,
-                   double *phi, unsigned long long *colmax, hipStream_t st, const int *gate, const int *gate_flags) {
+                   double *phi, unsigned long long *colmax, hipStream_t st, const int *gate, const int *gate_flags,
+                   bool zero_pad) {
     SepArgs a{};
     a.gate = gate;
     a.gate_flags = gate_flags;
@@ -697,6 +702,7 @@ void sep_phi_chunk(rp_fit *fit, FitDevice &dv, const SepPlan &plan, int chunk, i
     a.T1 = dv.K > 0 ? dv.sep_t1.as<double2>() : nullptr;
     a.N = fit->n_atoms;
     a.NP = dv.NP;
+    a.ncols = zero_pad ? dv.NP : fit->n_atoms + 2;
     a.ex = dv.ex.as<double2>();
     a.ey = dv.ey.as<double2>();
     a.e = dv.epi;
@@ -704,7 +710,7 @@ void sep_phi_chunk(rp_fit *fit, FitDevice &dv, const SepPlan &plan, int chunk, i
     a.colmax = colmax;
     a.items = (int)ceil_div(a.l_end - a.l_first, SEP_LINES);
     a.items_per_xcd = (int)ceil_div(a.items, 8);
-    const int blocks = 8 * a.items_per_xcd * (dv.NP / SEP_ATOMS);
+    const int blocks = 8 * a.items_per_xcd * (int)ceil_div(a.ncols, SEP_ATOMS);
     const bool images = dv.epi.fit_type != 0;
     if (images) k_zero_flag<<<1, 1, 0, st>>>(dv.multi_flag.as<int>());
     a.multi_flag = dv.multi_flag.as<int>();
@@ -726,13 +732,13 @@ void sep_phi_chunk(rp_fit *fit, FitDevice &dv, const SepPlan &plan, int chunk, i
     }
 }
 
-void sep_slice_chunk(const double *phi, int NP, int64_t rows, int64_t valid, int64_t span,
+void sep_slice_chunk(const double *phi, int NP, int ncols, int64_t rows, int64_t valid, int64_t span,
                      const unsigned long long *colmax, int8_t *slices, double *scale_out, int P, hipStream_t st) {
     dim3 g((unsigned)ceil_div(span, 256 * 16), (unsigned)NP);
     if (P == 3)
-        k_slice_cols<3><<<g, 256, 0, st>>>(phi, rows, valid, span, colmax, slices, scale_out);
+        k_slice_cols<3><<<g, 256, 0, st>>>(phi, rows, valid, span, ncols, colmax, slices, scale_out);
     else
-        k_slice_cols<4><<<g, 256, 0, st>>>(phi, rows, valid, span, colmax, slices, scale_out);
+        k_slice_cols<4><<<g, 256, 0, st>>>(phi, rows, valid, span, ncols, colmax, slices, scale_out);
 }
 
 }  // namespace rp

@@ -406,7 +406,7 @@ int accumulate_split_p(rp_fit *fit, FitDevice &dv, bool keep_phi) {
         }
         {
             KTimer kts(ctx, KT_SLICE);
-            sep_slice_chunk(phi, NP, rows, valid, (int64_t)rps_c * splits, d_colmax.as<unsigned long long>(),
+            sep_slice_chunk(phi, NP, N + 2, rows, valid, (int64_t)rps_c * splits, d_colmax.as<unsigned long long>(),
                             d_slices.as<int8_t>(), d_scale.as<double>(), P, st);
         }
         KTimer ktg(ctx, KT_GRAM);
