@@ -54,7 +54,7 @@ struct RefineState {
 // twice the largest per-step contraction measured (5.3e-3, a 5000-atom cell with cond(A) = 4.9e10).
 constexpr double RHO_PRIOR = 1e-2;
 // Refinement steps queued with the solve; more only after a synchronisation shows they are needed.
-constexpr int REFINE_EAGER = 2;
+constexpr int REFINE_EAGER = 1;
 // Relative perturbation of the sensitivity estimate: 8 units in the last place of an fp64 entry.
 constexpr double SENS_EPS = 8.0 * 2.220446049250313e-16;
 
