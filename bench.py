@@ -468,7 +468,15 @@ def main():
     P = split["slices"]
     executed = 2.0 * rows * tiles * 64 * 128 * P * P / g_launch / 1e12    # int8 ops issued, P^2 slice products
     g_traffic = traffic_from_profile("k_oz_gram", args.workload)
-    g_alg_bytes = P * np_ * rows + 2 * 8.0 * split["plan"]["splits"] * tiles * 64 * 128
+    # slices once, the row-split partial tiles read and written once, and the slots of the items that the
+    # library halves to complete an XCD's last round (its rule, for the 32 CUs of an MI355X XCD)
+    n_splits = split["plan"]["splits"]
+    halve = 0
+    if n_splits % 8 == 0:
+        rem = (tiles * (n_splits // 8)) % 32
+        if 0 < rem <= 16 and rem <= tiles and (rows // n_splits) // 2 >= (128 if P == 3 else 64):
+            halve = rem
+    g_alg_bytes = P * np_ * rows + 2 * 8.0 * (n_splits * tiles + 8 * halve) * 64 * 128
     roof_gram = {"bound": "mfma", "kernel": "k_oz_gram", "achieved": round(alg_per_launch / g_launch / 1e12, 3),
                  "peak": PEAK_INT8_TOPS, "unit": "TFLOP/s",
                  "frac": round(alg_per_launch / g_launch / 1e12 / PEAK_INT8_TOPS, 5),
