@@ -244,8 +244,20 @@ inline ChunkPlan plan_chunks(int64_t M, int N, int K, int precision) {
                 if (bi * 64 <= bj * 128 + 127) ++c.n_tiles;
         // every row split must be whole stages of the Gram: 128 rows deep with 3 slices, 64 with 4
         const int64_t kd = gram_stage_rows(c.slices);
-        while ((int64_t)c.n_tiles * c.splits < 1024 && c.rows / (c.splits * 2) >= 1024 && (c.rows / (c.splits * 2)) % kd == 0)
-            c.splits *= 2;
+        auto row_splits = [&](int64_t r) {
+            int sp = 1;
+            while ((int64_t)c.n_tiles * sp < 1024 && r / (sp * 2) >= 1024 && (r / (sp * 2)) % kd == 0) sp *= 2;
+            return sp;
+        };
+        c.splits = row_splits(c.rows);
+        // more than one chunk: equal chunks of whole 2048-row blocks where that keeps the row splits.  The
+        // chunk length is then no power of two, so the residual pass's reads across all columns at one row
+        // offset (columns one chunk length apart) spread over the memory channels.
+        if (M > c.rows) {
+            const int64_t nch = (M + c.rows - 1) / c.rows;
+            const int64_t bal = round_up((M + nch - 1) / nch, 2048);
+            if (bal < c.rows && row_splits(bal) == c.splits) c.rows = bal;
+        }
     }
     return c;
 }
